@@ -1,0 +1,185 @@
+"""Ghost-fill benchmark: effective HBM GB/s of one full ghost fill on one GPU.
+
+Workload (default): the large single forest -- 8^3 roots of 32^3 cells,
+cell-centred, two refinement levels over 4 seeded spheres (about 12% of the
+volume refined, 2:1 balanced, about 5.2k blocks), G=3, 20 Float64 variables
+(Gaussian bumps plus 1e-4 noise), PointValue(6,6), reflecting z-low face with
+alternating parities, outflow elsewhere.
+
+A step is one full ghost fill of that field set, inputs resident in HBM.
+Effective bytes per fill = 16 x (values written), the reference's accounting
+(bench/copies.jl:115-125): one 8-byte read and one 8-byte write per value.
+
+Reported: the staged fill (headline), the plain per-group fill beside it,
+the roofline figure of the staged kernel, and the restated reference CPU fill
+(the oracle) on a bounded sample, on this host's cores.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "treeamr.jl_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "ghost-fill effective HBM GB/s (% of peak); batched vs plain fill time, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_fill(fn, steps, warmup, stream):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    times = []
+    t_all0 = time.perf_counter()
+    for _ in range(steps):
+        start.record(stream)
+        fn()
+        stop.record(stream)
+        stop.synchronize()
+        times.append(start.elapsed_time(stop))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_all0
+    times.sort()
+    return dict(min_ms=times[0], median_ms=times[len(times) // 2],
+                mean_ms=sum(times) / len(times), wall_s=wall)
+
+
+def cpu_baseline(sched_builder, seconds=12.0):
+    """The oracle (the restated reference CPU fill, threaded by block owner as
+    run_phase!(::CPU) is) on the same mesh with one of the 20 variables."""
+    import numpy as np
+    import oracle
+    from treeamr_hip import FieldSet, GhostSchedule
+    from treeamr_hip import workloads as W
+    f, ops, kw = sched_builder(1)
+    kw = dict(kw)
+    nvars = kw.pop("nvars")
+    fs = FieldSet(f, nvars, device="cpu", fill=0.0, **kw)
+    W.fill_gaussians(fs, 5)
+    sched = GhostSchedule(fs, ops)
+    work = fs.work.numpy()
+    fac = fs.factors.numpy().copy() if fs.factors is not None else None
+    nthreads = oracle.max_threads()
+    oracle.fill(sched._descptr, fs.layout, work, fac, nthreads=nthreads)  # warm
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        oracle.fill(sched._descptr, fs.layout, work, fac, nthreads=nthreads)
+        reps += 1
+        if time.perf_counter() - t0 > seconds or reps >= 50:
+            break
+    dt = (time.perf_counter() - t0) / reps
+    values = sched.points() * nvars
+    return dict(value=16.0 * values / dt / 1e9, unit="GB/s", cores=nthreads, kind="port",
+                sample=f"the same mesh with 1 of the 20 variables ({values} values per fill), "
+                       f"{reps} fills, {dt * 1e3:.1f} ms per fill, OpenMP threads pinned by block owner")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nvars", type=int, default=20)
+    ap.add_argument("--layout", default="dense", choices=["dense", "arena"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path "
+              f"(asked for --gpus {args.gpus}); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from treeamr_hip import FieldSet, GhostSchedule, fill_ghosts
+    from treeamr_hip import workloads as W
+
+    def builder(nvars):
+        f, ops, kw, _ = W.config5(nvars=nvars)
+        return f, ops, kw
+
+    log("[bench] building mesh and schedule")
+    t0 = time.perf_counter()
+    f, ops, kw = builder(args.nvars)
+    kw = dict(kw)
+    nvars = kw.pop("nvars")
+    sched_t0 = time.perf_counter()
+    fs = FieldSet(f, nvars, device="cuda", layout=args.layout, fill=0.0, **kw)
+    W.fill_gaussians(fs, 5)
+    sched = GhostSchedule(fs, ops)
+    build_s = time.perf_counter() - sched_t0
+    values = sched.points() * nvars
+    bytes_eff = 16.0 * values
+    log(f"[bench] {f.nleaves} blocks, {values} values/fill, {bytes_eff / 1e9:.2f} GB effective; "
+        f"setup {time.perf_counter() - t0:.1f}s")
+    stream = torch.cuda.current_stream()
+
+    plain = None
+    if not args.no_plain:
+        log("[bench] timing plain per-group fill")
+        plain = time_fill(lambda: fill_ghosts(fs, sched, method="plain"), args.steps, args.warmup, stream)
+        log(f"[bench] plain: {plain}")
+    log("[bench] timing staged fill")
+    staged = time_fill(lambda: fill_ghosts(fs, sched, method="staged"), args.steps, args.warmup, stream)
+    log(f"[bench] staged: {staged}")
+    head = staged
+    t_ms = head["mean_ms"]
+    gbs = bytes_eff / (t_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] timing the CPU baseline (oracle)")
+        cpu = cpu_baseline(builder)
+        log(f"[bench] cpu: {cpu}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(gbs, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Gaussian bumps + 1e-4 noise, seed 5)",
+        "config": {
+            "workload": "large single forest: 8^3 roots, N=32, cell, G=3, 20 vars, PointValue(6,6), "
+                        "2 levels over 4 spheres, reflecting z-lo + outflow",
+            "blocks": f.nleaves,
+            "values_per_fill": values,
+            "layout": args.layout,
+            "fill": "staged",
+        },
+        "pct_of_peak": round(100.0 * gbs / HBM_PEAK_GBS, 2),
+        "staged_ms": {k: round(v, 4) for k, v in staged.items()},
+        "plain_ms": {k: round(v, 4) for k, v in plain.items()} if plain else None,
+        "speedup_vs_plain": round(plain["mean_ms"] / staged["mean_ms"], 3) if plain else None,
+        "schedule_build_s": round(build_s, 3),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,240 @@
+/*
+ * tah.h -- C ABI of the MI355X ghost-fill engine for TreeAMR.jl.
+ *
+ * The reference has no FFI: its fill is the Julia call
+ *     fill_ghosts!(fs::FieldSet, schedule::GhostSchedule; boundary=nothing)
+ * (src/ghosts.jl:1783-1794), which checks the pair (check_fill,
+ * src/ghosts.jl:2058-2115) and replays the schedule stage by stage
+ * (exchange_ghosts!, src/ghosts.jl:1099-1160), one KernelAbstractions
+ * launch per transfer group (run_group!, src/ghosts.jl:420-499).
+ * This header is what a Julia shim `ccall`s in its place: the schedule is
+ * flattened once per (schedule, forest generation) into a tah_schedule_desc
+ * and uploaded as a tah_plan; every fill then runs on the device.
+ *
+ * Conventions: every function returns a tah_status; on failure
+ * tah_last_error() holds a message (thread-local). Nothing is launched when
+ * a check fails. Indices in descriptors are 0-based (the reference's are
+ * 1-based: a shim subtracts one). Field data is Float64 only
+ * (check_floattype, src/device.jl:53-61). Device pointers are hipMalloc'd
+ * (or the arena of tah_arena_alloc); streams are hipStream_t passed as void*.
+ * Two fills over one working array must not overlap (src/ghosts.jl:1773).
+ */
+#ifndef TAH_H
+#define TAH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum tah_status {
+  TAH_OK = 0,
+  TAH_EINVAL = 1,      /* bad argument (ArgumentError in the reference) */
+  TAH_ESTALE = 2,      /* schedule built for another forest generation */
+  TAH_ELAYOUT = 3,     /* work layout does not match the schedule */
+  TAH_EHIP = 4,        /* a HIP runtime call failed */
+  TAH_ENOMEM = 5,      /* device allocation failed */
+  TAH_EUNSUPPORTED = 6 /* a feature this library does not implement */
+} tah_status;
+
+/* Message of the last failure on this thread. */
+const char* tah_last_error(void);
+
+/* ------------------------------------------------------------------ */
+/* Host side: the forest, the operators and the schedule builder.       */
+/* ------------------------------------------------------------------ */
+
+typedef struct tah_forest tah_forest;
+typedef struct tah_schedule tah_schedule;
+
+/* Forest(roots; N, periodic, reflecting, outflow, extents)
+ * (src/forest.jl:314-341). reflecting/outflow: [dim][2] flags (lo, hi);
+ * extents: [dim][2] or NULL for (0, roots[d]). */
+tah_status tah_forest_create(int32_t dim, const int32_t* roots, int32_t N,
+                             const int32_t* periodic, const int32_t* reflecting,
+                             const int32_t* outflow, const double* extents,
+                             tah_forest** out);
+tah_status tah_forest_destroy(tah_forest* f);
+/* nleaves (src/forest.jl:1052) and generation (src/forest.jl:1040). */
+tah_status tah_forest_info(const tah_forest* f, int32_t* nleaves, int64_t* generation);
+/* The sorted leaf keys: root, level, coords[3] per leaf (src/morton.jl:25). */
+tah_status tah_forest_leaves(const tah_forest* f, int32_t* root, int32_t* level,
+                             uint32_t* coords);
+/* refine! of the given leaves (0-based leaf indices), src/forest.jl:1761. */
+tah_status tah_forest_refine(tah_forest* f, int32_t n, const int32_t* leaves);
+/* balance! (src/forest.jl:1837) and isbalanced (src/forest.jl:1892). */
+tah_status tah_forest_balance(tah_forest* f);
+tah_status tah_forest_isbalanced(const tah_forest* f, int32_t* balanced);
+
+enum { TAH_POINTVALUE = 0, TAH_CONSERVATIVE = 1 };
+/* Operators(; prolongation, restriction, family, bounded,
+ * vertex_prolongation) (src/operators.jl:314-415). */
+typedef struct tah_operators {
+  int32_t family;
+  int32_t prolongation;
+  int32_t restriction;
+  int32_t bounded;
+  int32_t vertex_prolongation; /* 0 for none */
+} tah_operators;
+/* The constructor's own checks (src/operators.jl:327-411). */
+tah_status tah_operators_check(const tah_operators* ops);
+
+enum { TAH_PARITY_NONE = 0, TAH_PARITY_EVEN = 1, TAH_PARITY_ODD = 2 };
+enum { TAH_CELL = 0, TAH_VERTEX = 1 };
+
+/* GhostSchedule(forest, ops; G, centering, parity)
+ * (src/schedule.jl:3075-3292). G, centering: per dimension; parity:
+ * nvars x dim (variable-major) or NULL. */
+tah_status tah_schedule_create(const tah_forest* f, const tah_operators* ops,
+                               const int32_t* G, const int32_t* centering,
+                               int32_t nvars, const int8_t* parity,
+                               tah_schedule** out);
+tah_status tah_schedule_destroy(tah_schedule* s);
+/* isstale (src/schedule.jl:872-874): TAH_ESTALE once the forest changed. */
+tah_status tah_schedule_check(const tah_schedule* s);
+
+/* The factor table of the mirrored transfers, factors[v + nvars*(col-1)],
+ * col = 1..3^dim (parityfactors, src/storage.jl:536-550). */
+tah_status tah_parity_factors(int32_t dim, int32_t nvars, const int8_t* parity,
+                              double* out);
+
+/* --- The flattened schedule (north-star (a)) ------------------------- */
+
+enum { TAH_COPY = 1, TAH_RESTRICT = 2, TAH_PROLONG = 3 };
+/* Stage tags, in fill order: stage P (-1), stage 0 (0), phase 1 (1), the
+ * prolongations onto level l (2 + l) (transfer_stage,
+ * src/schedule.jl:640-658). */
+enum { TAH_STAGE_P = -1, TAH_STAGE_UNIFY = 0, TAH_STAGE_PHASE1 = 1 };
+
+/* One dimension of a transfer (Stencil1D, src/schedule.jl:64-93):
+ * target rows targetfirst .. targetfirst+ntarget-1 (0-based); row i reads
+ * source rows srcstart[i] .. srcstart[i]+order-1 with weights
+ * weights[order*i .. order*i+order-1]. */
+typedef struct tah_stencil1d {
+  int32_t targetfirst;
+  int32_t ntarget;
+  int32_t order;
+  int32_t unit; /* one point of weight one */
+  const int32_t* srcstart;
+  const double* weights;
+} tah_stencil1d;
+
+/* One transfer group (TransferGroup, src/schedule.jl:150-197). Dimensions
+ * beyond dim are one-point unit stencils. */
+typedef struct tah_group_desc {
+  int32_t kind;       /* TAH_COPY / TAH_RESTRICT / TAH_PROLONG */
+  int32_t stage;      /* index into stage_tags */
+  int32_t factorcol;  /* 0: unscaled, else the 1-based factor column */
+  int32_t ntransfers;
+  const int32_t* targetblocks; /* 0-based, non-decreasing */
+  const int32_t* sourceblocks; /* 0-based */
+  tah_stencil1d stencils[3];
+} tah_group_desc;
+
+typedef struct tah_schedule_desc {
+  int32_t dim;
+  int32_t nblocks;
+  int32_t stored[3];       /* N + 2G + c per dimension; 1 beyond dim */
+  int32_t nstages;
+  const int32_t* stage_tags;
+  /* stagegroups (src/schedule.jl:2876-2931): stages [first[k], last[k]]
+   * may run as one. */
+  int32_t nstagegroups;
+  const int32_t* stagegroup_first;
+  const int32_t* stagegroup_last;
+  int32_t ngroups;
+  const tah_group_desc* groups; /* sorted by stage */
+  int64_t generation;           /* forest generation it was built for */
+} tah_schedule_desc;
+
+/* The schedule's flattened form; pointers stay valid while s lives. */
+tah_status tah_schedule_desc_get(const tah_schedule* s, const tah_schedule_desc** out);
+
+/* The outer-face regions (the reference's hook regions, filled by the
+ * application, never by the fill): n, then per region block, direction[3]
+ * and the 0-based half-open box lo[3], hi[3]. */
+tah_status tah_schedule_boundaries(const tah_schedule* s, int32_t* n, int32_t* block,
+                                   int32_t* direction, int32_t* lo, int32_t* hi);
+
+/* Effective values written per fill (bench/copies.jl:115-125): the sum
+ * over groups of ntransfers x box points, times nvars by the caller. */
+tah_status tah_schedule_points(const tah_schedule* s, int64_t* points);
+
+/* ------------------------------------------------------------------ */
+/* Device side.                                                          */
+/* ------------------------------------------------------------------ */
+
+/* The working array: element (i, j, k, v, b) lives at
+ * stride[0]*i + stride[1]*j + stride[2]*k + stride[3]*v + stride[4]*b,
+ * stride[0] == 1. The reference's dense layout work[i,j,k,var,block]
+ * (src/storage.jl:196-200) is the unpadded case. */
+typedef struct tah_layout {
+  int32_t stored[3];
+  int32_t nvars;
+  int32_t nblocks;
+  int32_t pad_;
+  int64_t stride[5];
+} tah_layout;
+
+/* The dense layout of a schedule's field set. */
+tah_status tah_layout_dense(const tah_schedule_desc* d, int32_t nvars, tah_layout* out);
+/* The padded arena layout (north-star (b)): rows padded so that the low
+ * and high ghost slabs of each row start on 32-byte sectors. */
+tah_status tah_layout_arena(const tah_schedule_desc* d, int32_t nvars, tah_layout* out);
+/* Elements a layout spans (what to allocate, in doubles). */
+tah_status tah_layout_size(const tah_layout* l, int64_t* elements);
+
+typedef struct tah_ctx tah_ctx;
+typedef struct tah_plan tah_plan;
+typedef struct tah_graph tah_graph;
+
+tah_status tah_ctx_create(int32_t device, tah_ctx** out);
+tah_status tah_ctx_destroy(tah_ctx* ctx);
+
+/* Upload a schedule's tables for one layout (todevice, src/device.jl:13-33):
+ * the library owns the device copy; the caller keeps its host arrays. Every
+ * index is validated against the layout here, before any launch. */
+tah_status tah_plan_create(tah_ctx* ctx, const tah_schedule_desc* d, const tah_layout* l,
+                           tah_plan** out);
+tah_status tah_plan_destroy(tah_plan* p);
+/* Values one fill writes (per variable: x nvars already applied). */
+tah_status tah_plan_values(const tah_plan* p, int64_t* values);
+
+/* The plain per-group fill: one launch per transfer group, a flat index per
+ * written value, as the reference's device path launches
+ * (launch_transfer!, src/ghosts.jl:507-521). factors: device table
+ * nvars x 3^dim, or NULL when no group is mirrored. */
+tah_status tah_fill_plain(tah_plan* p, double* work, const double* factors, void* stream);
+
+/* The staged fill: one launch per stage group over a tile list of every
+ * group of its stages, class-specialised kernels. Same results, bit for bit. */
+tah_status tah_fill(tah_plan* p, double* work, const double* factors, void* stream);
+
+/* Many independent fills (forests, field sets) in one launch per stage
+ * position. */
+tah_status tah_fill_batched(tah_ctx* ctx, int32_t n, tah_plan* const* plans,
+                            double* const* works, const double* const* factors, void* stream);
+
+/* A batched fill captured once in a hipGraph, replayed by tah_graph_launch.
+ * The arrays and tables named at creation must stay alive. */
+tah_status tah_graph_create(tah_ctx* ctx, int32_t n, tah_plan* const* plans,
+                            double* const* works, const double* const* factors,
+                            tah_graph** out);
+tah_status tah_graph_launch(tah_graph* g, void* stream);
+tah_status tah_graph_destroy(tah_graph* g);
+
+/* The adjoint of the fill (the fill is linear): stages in reverse, each
+ * source accumulating w x adj[target] over a precomputed inverted schedule
+ * in a fixed order (no float atomics), then the targets zeroed. */
+tah_status tah_fill_adjoint(tah_plan* p, double* adj, const double* factors, void* stream);
+
+/* Pick tile shapes and work-group sizes per stencil class by timing on the
+ * device; cached in the file at cache_path (may be NULL) by mesh signature. */
+tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char* cache_path);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TAH_H */

@@ -1,0 +1,74 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes access to liboracle.so, the CPU
+restatement of the reference's ghost fill (oracle.c). Imported by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg; never by the
+product package.
+
+Functions:
+  fill(desc, layout, work, factors, nthreads=0, skip=True)   run_phase!(::CPU) per stage
+  fill_adjoint(desc, layout, adj, factors)                   CPU transpose (scatter form)
+  write_counts(desc, layout)                                 writes per stored point
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            build()
+        L = C.CDLL(LIBPATH)
+        for name in ("oracle_fill", "oracle_fill_adjoint", "oracle_write_counts"):
+            getattr(L, name).restype = C.c_int
+        L.oracle_fill.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+        L.oracle_fill_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_write_counts.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_max_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags.c_contiguous
+        return C.c_void_p(a.ctypes.data)
+    return C.c_void_p(a.data_ptr())  # a CPU torch tensor
+
+
+def fill(desc_ptr, layout, work, factors=None, nthreads=0, skip=True):
+    rc = lib().oracle_fill(C.cast(desc_ptr, C.c_void_p), C.cast(C.pointer(layout), C.c_void_p),
+                           _ptr(work), _ptr(factors), int(nthreads), 1 if skip else 0)
+    if rc:
+        raise RuntimeError(f"oracle_fill failed ({rc})")
+
+
+def fill_adjoint(desc_ptr, layout, adj, factors=None):
+    rc = lib().oracle_fill_adjoint(C.cast(desc_ptr, C.c_void_p), C.cast(C.pointer(layout), C.c_void_p),
+                                   _ptr(adj), _ptr(factors))
+    if rc:
+        raise RuntimeError(f"oracle_fill_adjoint failed ({rc})")
+
+
+def write_counts(desc_ptr, layout):
+    n = layout.nblocks * layout.stored[0] * layout.stored[1] * layout.stored[2]
+    counts = np.zeros(n, np.int32)
+    lib().oracle_write_counts(C.cast(desc_ptr, C.c_void_p), C.cast(C.pointer(layout), C.c_void_p),
+                              _ptr(counts))
+    return counts.reshape(layout.nblocks, layout.stored[2], layout.stored[1], layout.stored[0])
+
+
+def max_threads():
+    return lib().oracle_max_threads()

@@ -1,0 +1,222 @@
+/*
+ * oracle.c -- TEST INFRASTRUCTURE ONLY. The CPU restatement of TreeAMR.jl's
+ * ghost fill, used by tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg as the checker and the CPU baseline. The product path
+ * (libtah) never links, loads or calls it.
+ *
+ * What it restates, on the flattened schedule of include/tah.h:
+ *   - run_phase!(::CPU) (src/ghosts.jl:572-593): a phase is one parallel
+ *     loop; thread t owns the contiguous block range equalsplit(nblocks, T, t)
+ *     (src/threading.jl:50-70) and runs, for every group, the transfers whose
+ *     target block it owns, found by bisection of the non-decreasing
+ *     targetblocks;
+ *   - transfer_kernel! (src/ghosts.jl:372-402) per written value: a unit copy
+ *     is 0 + u (copy_load, :127-128); otherwise stencil_sum (:58-118): one
+ *     accumulator from +0, m1 innermost, the tap weight formed as
+ *     ((w1*w2)*w3) and added as acc += w*u, with the CPU's one-tap skip on a
+ *     column holding a single nonzero weight (single_rows,
+ *     src/schedule.jl:86-89); then scaled (:369-370): acc*f + 0 when the group
+ *     has a factor column;
+ *   - exchange_ghosts! (src/ghosts.jl:1099-1160): stages in order (P, 0,
+ *     phase 1, prolongation levels coarsest first).
+ * Built with -ffp-contract=off so that no product and sum fuse.
+ *
+ * Also: oracle_fill_adjoint, the transpose of the same linear map assembled
+ * on the CPU from the schedule (stages reversed; within a stage every
+ * source accumulates w*f*adj[target], then the targets are zeroed). It has
+ * no counterpart in the reference (which has no autodiff); it is pinned by
+ * the dot-product identity <fill(x), y> = <x, fill^T(y)> in tests/.
+ */
+#include <math.h>
+#include <omp.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/tah.h"
+
+typedef struct {
+  int32_t lo, hi; /* 0-based, half-open */
+} span_t;
+
+/* src/threading.jl:66-70, 0-based and half-open. */
+static span_t equalsplit(int32_t n, int32_t p, int32_t i) {
+  int32_t len = n / p, extra = n % p;
+  int32_t lo = i * len + (i < extra ? i : extra);
+  span_t s = {lo, lo + len + (i < extra ? 1 : 0)};
+  return s;
+}
+
+/* The transfers of a group whose target block lies in [lo, hi):
+ * searchsortedfirst / searchsortedlast on targetblocks. */
+static span_t owned_transfers(const tah_group_desc* g, int32_t lo, int32_t hi) {
+  int32_t a = 0, b = g->ntransfers;
+  while (a < b) {
+    int32_t m = (a + b) / 2;
+    if (g->targetblocks[m] < lo) a = m + 1; else b = m;
+  }
+  int32_t first = a;
+  a = first;
+  b = g->ntransfers;
+  while (a < b) {
+    int32_t m = (a + b) / 2;
+    if (g->targetblocks[m] < hi) a = m + 1; else b = m;
+  }
+  span_t s = {first, a};
+  return s;
+}
+
+/* The row of a column's single nonzero weight (0-based), or -1. */
+static int single_row(const tah_stencil1d* s, int32_t col) {
+  int nz = 0, row = -1;
+  for (int m = 0; m < s->order; ++m)
+    if (s->weights[(int64_t)s->order * col + m] != 0.0) { ++nz; row = m; }
+  return nz == 1 ? row : -1;
+}
+
+static void run_transfers(const tah_group_desc* g, const tah_layout* L, double* work,
+                          const double* factors, int32_t t0, int32_t t1, int skip) {
+  const tah_stencil1d* S = g->stencils;
+  const int64_t* st = L->stride;
+  int unit = S[0].unit && S[1].unit && S[2].unit && S[0].order == 1 && S[1].order == 1 &&
+             S[2].order == 1;
+  int32_t n0 = S[0].ntarget, n1 = S[1].ntarget, n2 = S[2].ntarget;
+  for (int32_t t = t0; t < t1; ++t) {
+    double* tb = work + st[4] * g->targetblocks[t];
+    const double* sb = work + st[4] * g->sourceblocks[t];
+    for (int32_t v = 0; v < L->nvars; ++v) {
+      double f = g->factorcol ? factors[v + (int64_t)L->nvars * (g->factorcol - 1)] : 1.0;
+      for (int32_t k = 0; k < n2; ++k)
+        for (int32_t j = 0; j < n1; ++j)
+          for (int32_t i = 0; i < n0; ++i) {
+            int32_t b0 = S[0].srcstart[i], b1 = S[1].srcstart[j], b2 = S[2].srcstart[k];
+            double acc;
+            if (unit) {
+              acc = 0.0 + sb[st[0] * b0 + st[1] * b1 + st[2] * b2 + st[3] * v];
+            } else {
+              int lo0 = 0, hi0 = S[0].order, lo1 = 0, hi1 = S[1].order, lo2 = 0, hi2 = S[2].order;
+              if (skip) {
+                int r;
+                if (S[0].order > 1 && (r = single_row(&S[0], i)) >= 0) { lo0 = r; hi0 = r + 1; }
+                if (S[1].order > 1 && (r = single_row(&S[1], j)) >= 0) { lo1 = r; hi1 = r + 1; }
+                if (S[2].order > 1 && (r = single_row(&S[2], k)) >= 0) { lo2 = r; hi2 = r + 1; }
+              }
+              const double* w0 = S[0].weights + (int64_t)S[0].order * i;
+              const double* w1 = S[1].weights + (int64_t)S[1].order * j;
+              const double* w2 = S[2].weights + (int64_t)S[2].order * k;
+              acc = 0.0;
+              for (int m2 = lo2; m2 < hi2; ++m2)
+                for (int m1 = lo1; m1 < hi1; ++m1)
+                  for (int m0 = lo0; m0 < hi0; ++m0) {
+                    double w = (w0[m0] * w1[m1]) * w2[m2];
+                    acc += w * sb[st[0] * (b0 + m0) + st[1] * (b1 + m1) + st[2] * (b2 + m2) + st[3] * v];
+                  }
+            }
+            if (g->factorcol) acc = acc * f + 0.0;
+            tb[st[0] * (S[0].targetfirst + i) + st[1] * (S[1].targetfirst + j) +
+               st[2] * (S[2].targetfirst + k) + st[3] * v] = acc;
+          }
+    }
+  }
+}
+
+/* One full fill. nthreads <= 0: all OpenMP threads. skip: the CPU's one-tap
+ * skip (1) or the device's full sum (0). */
+int oracle_fill(const tah_schedule_desc* d, const tah_layout* L, double* work,
+                const double* factors, int nthreads, int skip) {
+  if (!d || !L || !work) return 1;
+  for (int32_t gi = 0; gi < d->ngroups; ++gi)
+    if (d->groups[gi].factorcol && !factors) return 2;
+  int T = nthreads > 0 ? nthreads : omp_get_max_threads();
+  if (T > d->nblocks) T = d->nblocks > 0 ? d->nblocks : 1;
+  for (int32_t stage = 0; stage < d->nstages; ++stage) {
+#pragma omp parallel num_threads(T)
+    {
+      int t = omp_get_thread_num();
+      int nt = omp_get_num_threads();
+      span_t own = equalsplit(d->nblocks, nt, t);
+      for (int32_t gi = 0; gi < d->ngroups; ++gi) {
+        const tah_group_desc* g = &d->groups[gi];
+        if (g->stage != stage) continue;
+        span_t tr = owned_transfers(g, own.lo, own.hi);
+        if (tr.lo < tr.hi) run_transfers(g, L, work, factors, tr.lo, tr.hi, skip);
+      }
+    }
+  }
+  return 0;
+}
+
+/* The transpose of oracle_fill, serial, scatter form. */
+int oracle_fill_adjoint(const tah_schedule_desc* d, const tah_layout* L, double* adj,
+                        const double* factors) {
+  if (!d || !L || !adj) return 1;
+  const int64_t* st = L->stride;
+  for (int32_t stage = d->nstages - 1; stage >= 0; --stage) {
+    for (int32_t gi = 0; gi < d->ngroups; ++gi) {
+      const tah_group_desc* g = &d->groups[gi];
+      if (g->stage != stage) continue;
+      if (g->factorcol && !factors) return 2;
+      const tah_stencil1d* S = g->stencils;
+      for (int32_t t = 0; t < g->ntransfers; ++t) {
+        double* tb = adj + st[4] * g->targetblocks[t];
+        double* sb = adj + st[4] * g->sourceblocks[t];
+        for (int32_t v = 0; v < L->nvars; ++v) {
+          double f = g->factorcol ? factors[v + (int64_t)L->nvars * (g->factorcol - 1)] : 1.0;
+          for (int32_t k = 0; k < S[2].ntarget; ++k)
+            for (int32_t j = 0; j < S[1].ntarget; ++j)
+              for (int32_t i = 0; i < S[0].ntarget; ++i) {
+                double a = tb[st[0] * (S[0].targetfirst + i) + st[1] * (S[1].targetfirst + j) +
+                              st[2] * (S[2].targetfirst + k) + st[3] * v] * f;
+                const double* w0 = S[0].weights + (int64_t)S[0].order * i;
+                const double* w1 = S[1].weights + (int64_t)S[1].order * j;
+                const double* w2 = S[2].weights + (int64_t)S[2].order * k;
+                for (int m2 = 0; m2 < S[2].order; ++m2)
+                  for (int m1 = 0; m1 < S[1].order; ++m1)
+                    for (int m0 = 0; m0 < S[0].order; ++m0) {
+                      double w = (w0[m0] * w1[m1]) * w2[m2];
+                      sb[st[0] * (S[0].srcstart[i] + m0) + st[1] * (S[1].srcstart[j] + m1) +
+                         st[2] * (S[2].srcstart[k] + m2) + st[3] * v] += w * a;
+                    }
+              }
+        }
+      }
+    }
+    /* Then every target of the stage is zeroed: it was overwritten. */
+    for (int32_t gi = 0; gi < d->ngroups; ++gi) {
+      const tah_group_desc* g = &d->groups[gi];
+      if (g->stage != stage) continue;
+      const tah_stencil1d* S = g->stencils;
+      for (int32_t t = 0; t < g->ntransfers; ++t) {
+        double* tb = adj + st[4] * g->targetblocks[t];
+        for (int32_t v = 0; v < L->nvars; ++v)
+          for (int32_t k = 0; k < S[2].ntarget; ++k)
+            for (int32_t j = 0; j < S[1].ntarget; ++j)
+              for (int32_t i = 0; i < S[0].ntarget; ++i)
+                tb[st[0] * (S[0].targetfirst + i) + st[1] * (S[1].targetfirst + j) +
+                   st[2] * (S[2].targetfirst + k) + st[3] * v] = 0.0;
+      }
+    }
+  }
+  return 0;
+}
+
+/* How many times each stored point is written by one fill (the partition
+ * oracle write_counts, test/ghost_oracles.jl:228-255): counts has one int32
+ * per stored spatial point per block (variables share a box). */
+int oracle_write_counts(const tah_schedule_desc* d, const tah_layout* L, int32_t* counts) {
+  int64_t per = (int64_t)L->stored[0] * L->stored[1] * L->stored[2];
+  for (int32_t gi = 0; gi < d->ngroups; ++gi) {
+    const tah_group_desc* g = &d->groups[gi];
+    const tah_stencil1d* S = g->stencils;
+    for (int32_t t = 0; t < g->ntransfers; ++t)
+      for (int32_t k = 0; k < S[2].ntarget; ++k)
+        for (int32_t j = 0; j < S[1].ntarget; ++j)
+          for (int32_t i = 0; i < S[0].ntarget; ++i)
+            counts[per * g->targetblocks[t] +
+                   (S[0].targetfirst + i) +
+                   (int64_t)L->stored[0] * ((S[1].targetfirst + j) + (int64_t)L->stored[1] * (S[2].targetfirst + k))] += 1;
+  }
+  return 0;
+}
+
+int oracle_max_threads(void) { return omp_get_max_threads(); }

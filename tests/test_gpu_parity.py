@@ -1,0 +1,136 @@
+"""GPU parity: libtah's fills against the oracle (the CPU restatement of the
+reference fill) on the same seeded inputs, bit for bit.
+
+The reference checks its device fill against the serial CPU fill the same
+way (test/gpu_tests.jl:476-499) and its flat launch against the shaped one
+with the full sum, on awkward data, NaNs compared as one pattern
+(test/ghost_tests.jl:507-560, test/helpers/ghost.jl:7-45).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from treeamr_hip import FieldSet, GhostSchedule, fill_ghosts
+from treeamr_hip import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(t):
+    return t.detach().cpu().contiguous().view(torch.int64)
+
+
+def _prepare(forest, ops, kw, layout="dense", seed=0, data="uniform"):
+    kw = dict(kw)
+    nvars = kw.pop("nvars")
+    fs = FieldSet(forest, nvars, device="cuda", layout=layout, fill=float("nan"), **kw)
+    if data == "uniform":
+        W.fill_uniform(fs, seed)
+    else:
+        W.fill_gaussians(fs, seed)
+    sched = GhostSchedule(fs, ops)
+    return fs, sched
+
+
+def _oracle_result(fs, sched, skip=True):
+    host = fs.work.detach().cpu().numpy().copy()
+    fac = fs.factors.cpu().numpy().copy() if fs.factors is not None else None
+    oracle.fill(sched._descptr, fs.layout, host, fac, nthreads=0, skip=skip)
+    return torch.from_numpy(host)
+
+
+def _check(fs, sched, method):
+    print(f"[parity] method={method} blocks={fs.nblocks} nvars={fs.nvars}", flush=True)
+    expected = _oracle_result(fs, sched)
+    fill_ghosts(fs, sched, method=method)
+    torch.cuda.synchronize()
+    got = fs.work.detach().cpu()
+    assert not torch.isnan(fs.array()).any(), "a ghost was left unwritten"
+    mism = (_bits(got) != expected.view(torch.int64)).sum().item()
+    assert mism == 0, f"{mism} values differ from the oracle"
+
+
+@pytest.mark.parametrize("method", ["plain", "staged"])
+def test_config1_bit_exact(method):
+    f, ops, kw, _ = W.config1()
+    fs, sched = _prepare(f, ops, kw, seed=0)
+    _check(fs, sched, method)
+
+
+@pytest.mark.parametrize("method", ["plain", "staged"])
+@pytest.mark.parametrize("layout", ["dense", "arena"])
+def test_config5_small_bit_exact(method, layout):
+    f, ops, kw, _ = W.config5(nvars=3, roots=4, N=16)
+    fs, sched = _prepare(f, ops, kw, layout=layout, seed=1, data="gaussian")
+    _check(fs, sched, method)
+
+
+@pytest.mark.parametrize("seed", [2, 3, 4, 5])
+def test_random_forests_bit_exact(seed):
+    f, ops, G, nvars = W.random_forest(seed, max_blocks=600)
+    fs, sched = _prepare(f, ops, dict(nvars=nvars[0], G=G), seed=seed)
+    _check(fs, sched, "staged")
+
+
+def test_config5_full_mesh_one_variable():
+    """The full-size large-forest mesh (about 5k blocks of 32^3), one
+    variable: every value against the oracle."""
+    f, ops, kw, _ = W.config5(nvars=1)
+    fs, sched = _prepare(f, ops, kw, seed=7, data="gaussian")
+    _check(fs, sched, "staged")
+
+
+def test_awkward_data_full_sum():
+    """Signed zeros, subnormals, huge values, NaN and Inf in own points: the
+    device's full sum against the oracle's full sum (skip=False); finite
+    values bit for bit, NaNs as one pattern."""
+    f, ops, kw, _ = W.config5(nvars=2, roots=2, N=16)
+    fs, sched = _prepare(f, ops, kw, seed=3)
+    a = fs.array()
+    own = (slice(None), slice(None)) + fs.own_slices()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    x = a[own]
+    pick = torch.randint(0, 8, x.shape, generator=g, device="cuda")
+    specials = torch.tensor([0.0, -0.0, 5e-324, -2.2e-308, 1e300, -1e300, float("inf"), float("nan")],
+                            dtype=torch.float64, device="cuda")
+    mask = torch.rand(x.shape, generator=g, device="cuda", dtype=torch.float64) < 0.05
+    a[own] = torch.where(mask, specials[pick], x)
+    expected = _oracle_result(fs, sched, skip=False)
+    fill_ghosts(fs, sched, method="staged")
+    torch.cuda.synchronize()
+    got = fs.work.detach().cpu()
+    nan_g, nan_e = torch.isnan(got), torch.isnan(expected)
+    assert torch.equal(nan_g, nan_e)
+    keep = ~nan_g
+    assert torch.equal(_bits(got)[keep], expected.view(torch.int64)[keep])
+
+
+def test_fill_is_idempotent_and_interiors_untouched():
+    f, ops, kw, _ = W.config5(nvars=2, roots=4, N=16)
+    fs, sched = _prepare(f, ops, kw, seed=4, data="gaussian")
+    own = (slice(None), slice(None)) + fs.own_slices()
+    before = fs.array()[own].clone()
+    fill_ghosts(fs, sched)
+    once = fs.work.clone()
+    fill_ghosts(fs, sched)
+    torch.cuda.synchronize()
+    assert torch.equal(_bits(once), _bits(fs.work))
+    assert torch.equal(_bits(before), _bits(fs.array()[own]))
+
+
+def test_plain_equals_staged_full_config5():
+    """At the full large-forest size (20 variables): the plain per-group fill
+    and the staged fill agree bit for bit."""
+    f, ops, kw, _ = W.config5(nvars=20)
+    fs, sched = _prepare(f, ops, kw, seed=5, data="gaussian")
+    init = fs.work.clone()
+    fill_ghosts(fs, sched, method="plain")
+    plain = fs.work.clone()
+    fs.work.copy_(init)
+    del init
+    fill_ghosts(fs, sched, method="staged")
+    torch.cuda.synchronize()
+    assert not torch.isnan(fs.array()).any()
+    assert torch.equal(plain.view(torch.int64), fs.work.view(torch.int64))

@@ -1,0 +1,252 @@
+// C ABI, host side: forests, operators, schedules and their flattened
+// descriptors. Nothing here touches a GPU.
+#include <cstring>
+#include <memory>
+
+#include "api_common.hpp"
+#include "schedule.hpp"
+
+namespace tah {
+namespace {
+thread_local std::string last_error;
+}
+void set_last_error(const std::string& m) { last_error = m; }
+}  // namespace tah
+
+using namespace tah;
+
+struct tah_forest {
+  std::unique_ptr<Forest> f;
+};
+
+struct tah_schedule {
+  std::unique_ptr<GhostSchedule> s;
+  // The flattened, 0-based form handed out through tah_schedule_desc_get.
+  std::vector<std::array<std::vector<int32_t>, kMaxDim>> srcstart0;
+  std::vector<tah_group_desc> gdesc;
+  std::vector<int32_t> tags, sgfirst, sglast;
+  tah_schedule_desc desc{};
+};
+
+extern "C" {
+
+const char* tah_last_error(void) { return tah::last_error.c_str(); }
+
+tah_status tah_forest_create(int32_t dim, const int32_t* roots, int32_t N, const int32_t* periodic,
+                             const int32_t* reflecting, const int32_t* outflow,
+                             const double* extents, tah_forest** out) {
+  return guard([&] {
+    if (!roots || !out) fail("null argument");
+    if (dim < 1 || dim > kMaxDim) fail("dimension must be 1, 2 or 3");
+    int r[kMaxDim];
+    bool per[kMaxDim], refl[2 * kMaxDim], outf[2 * kMaxDim];
+    for (int d = 0; d < dim; ++d) {
+      r[d] = roots[d];
+      per[d] = periodic ? periodic[d] != 0 : false;
+      for (int s = 0; s < 2; ++s) {
+        refl[2 * d + s] = reflecting ? reflecting[2 * d + s] != 0 : false;
+        outf[2 * d + s] = outflow ? outflow[2 * d + s] != 0 : false;
+      }
+    }
+    auto f = std::make_unique<tah_forest>();
+    f->f = std::make_unique<Forest>(dim, r, N, per, refl, outf, extents);
+    *out = f.release();
+  });
+}
+
+tah_status tah_forest_destroy(tah_forest* f) {
+  delete f;
+  return TAH_OK;
+}
+
+tah_status tah_forest_info(const tah_forest* f, int32_t* nleaves, int64_t* generation) {
+  return guard([&] {
+    if (!f) fail("null forest");
+    if (nleaves) *nleaves = f->f->nleaves();
+    if (generation) *generation = f->f->generation;
+  });
+}
+
+tah_status tah_forest_leaves(const tah_forest* f, int32_t* root, int32_t* level, uint32_t* coords) {
+  return guard([&] {
+    if (!f) fail("null forest");
+    const auto& L = f->f->leaves;
+    for (size_t i = 0; i < L.size(); ++i) {
+      if (root) root[i] = L[i].root;
+      if (level) level[i] = L[i].level;
+      if (coords)
+        for (int d = 0; d < kMaxDim; ++d) coords[3 * i + d] = L[i].c[d];
+    }
+  });
+}
+
+tah_status tah_forest_refine(tah_forest* f, int32_t n, const int32_t* leaves) {
+  return guard([&] {
+    if (!f) fail("null forest");
+    std::vector<Key> keys;
+    for (int32_t i = 0; i < n; ++i) {
+      if (leaves[i] < 0 || leaves[i] >= f->f->nleaves()) fail("leaf index out of range");
+      keys.push_back(f->f->leaves[leaves[i]]);
+    }
+    f->f->refine(keys);
+  });
+}
+
+tah_status tah_forest_balance(tah_forest* f) {
+  return guard([&] {
+    if (!f) fail("null forest");
+    f->f->balance();
+  });
+}
+
+tah_status tah_forest_isbalanced(const tah_forest* f, int32_t* balanced) {
+  return guard([&] {
+    if (!f || !balanced) fail("null argument");
+    *balanced = f->f->isbalanced() ? 1 : 0;
+  });
+}
+
+tah_status tah_operators_check(const tah_operators* ops) {
+  return guard([&] {
+    if (!ops) fail("null operators");
+    Operators o(ops->family, ops->prolongation, ops->restriction, ops->bounded != 0,
+                ops->vertex_prolongation);
+    (void)o;
+  });
+}
+
+static std::vector<std::array<int8_t, kMaxDim>> parities_of(int dim, int nvars, const int8_t* parity) {
+  std::vector<std::array<int8_t, kMaxDim>> par;
+  if (!parity) return par;
+  for (int v = 0; v < nvars; ++v) {
+    std::array<int8_t, kMaxDim> p{0, 0, 0};
+    for (int d = 0; d < dim; ++d) {
+      p[d] = parity[v * dim + d];
+      if (p[d] < 0 || p[d] > 2) fail("parity must be none (0), even (1) or odd (2)");
+    }
+    par.push_back(p);
+  }
+  return par;
+}
+
+tah_status tah_schedule_create(const tah_forest* f, const tah_operators* ops, const int32_t* G,
+                               const int32_t* centering, int32_t nvars, const int8_t* parity,
+                               tah_schedule** out) {
+  return guard([&] {
+    if (!f || !ops || !G || !out) fail("null argument");
+    const Forest& F = *f->f;
+    if (nvars <= 0) fail("nvars must be positive");
+    Operators o(ops->family, ops->prolongation, ops->restriction, ops->bounded != 0,
+                ops->vertex_prolongation);
+    int g[kMaxDim] = {0, 0, 0}, c[kMaxDim] = {0, 0, 0};
+    for (int d = 0; d < F.D; ++d) {
+      g[d] = G[d];
+      c[d] = centering ? centering[d] : 0;
+    }
+    auto par = parities_of(F.D, nvars, parity);
+    auto S = std::make_unique<tah_schedule>();
+    S->s = std::make_unique<GhostSchedule>(F, o, g, c, par);
+    GhostSchedule& gs = *S->s;
+    S->srcstart0.resize(gs.groups.size());
+    S->gdesc.resize(gs.groups.size());
+    for (const Stage& st : gs.stages) S->tags.push_back(st.tag);
+    for (size_t i = 0; i < gs.groups.size(); ++i) {
+      const TransferGroup& tg = gs.groups[i];
+      tah_group_desc& gd = S->gdesc[i];
+      gd.kind = tg.kind;
+      gd.stage = -1;
+      for (size_t k = 0; k < gs.stages.size(); ++k)
+        if (gs.stages[k].tag == tg.stage) gd.stage = int32_t(k);
+      gd.factorcol = tg.factorcol;
+      gd.ntransfers = int32_t(tg.targetblocks.size());
+      gd.targetblocks = tg.targetblocks.data();
+      gd.sourceblocks = tg.sourceblocks.data();
+      for (int d = 0; d < kMaxDim; ++d) {
+        const Stencil1D& s = tg.stencils[d];
+        auto& v = S->srcstart0[i][d];
+        v.resize(s.srcstart.size());
+        for (size_t r = 0; r < v.size(); ++r) v[r] = s.srcstart[r] - 1;
+        gd.stencils[d].targetfirst = s.targetfirst - 1;
+        gd.stencils[d].ntarget = s.ntarget();
+        gd.stencils[d].order = s.order;
+        gd.stencils[d].unit = s.unit ? 1 : 0;
+        gd.stencils[d].srcstart = v.data();
+        gd.stencils[d].weights = s.weights.data();
+      }
+    }
+    for (auto& sg : gs.stagegroups) {
+      S->sgfirst.push_back(sg[0]);
+      S->sglast.push_back(sg[1]);
+    }
+    tah_schedule_desc& d = S->desc;
+    d.dim = gs.D;
+    d.nblocks = F.nleaves();
+    for (int k = 0; k < kMaxDim; ++k) d.stored[k] = gs.stored[k];
+    d.nstages = int32_t(S->tags.size());
+    d.stage_tags = S->tags.data();
+    d.nstagegroups = int32_t(S->sgfirst.size());
+    d.stagegroup_first = S->sgfirst.data();
+    d.stagegroup_last = S->sglast.data();
+    d.ngroups = int32_t(S->gdesc.size());
+    d.groups = S->gdesc.data();
+    d.generation = gs.generation;
+    *out = S.release();
+  });
+}
+
+tah_status tah_schedule_destroy(tah_schedule* s) {
+  delete s;
+  return TAH_OK;
+}
+
+tah_status tah_schedule_check(const tah_schedule* s) {
+  return guard([&] {
+    if (!s) fail("null schedule");
+    if (s->s->forest->generation != s->s->generation)
+      fail("the schedule is stale: the forest has changed since it was built; rebuild it", kStale);
+  });
+}
+
+tah_status tah_parity_factors(int32_t dim, int32_t nvars, const int8_t* parity, double* out) {
+  return guard([&] {
+    if (!parity || !out) fail("null argument");
+    if (dim < 1 || dim > kMaxDim) fail("dimension must be 1, 2 or 3");
+    auto f = parity_factors(dim, parities_of(dim, nvars, parity));
+    std::memcpy(out, f.data(), f.size() * sizeof(double));
+  });
+}
+
+tah_status tah_schedule_desc_get(const tah_schedule* s, const tah_schedule_desc** out) {
+  return guard([&] {
+    if (!s || !out) fail("null argument");
+    *out = &s->desc;
+  });
+}
+
+tah_status tah_schedule_boundaries(const tah_schedule* s, int32_t* n, int32_t* block, int32_t* direction,
+                                   int32_t* lo, int32_t* hi) {
+  return guard([&] {
+    if (!s || !n) fail("null argument");
+    const auto& B = s->s->boundaries;
+    *n = int32_t(B.size());
+    for (size_t i = 0; i < B.size(); ++i) {
+      if (block) block[i] = B[i].block;
+      for (int d = 0; d < kMaxDim; ++d) {
+        if (direction) direction[3 * i + d] = B[i].direction[d];
+        if (lo) lo[3 * i + d] = B[i].lo[d] - 1;
+        if (hi) hi[3 * i + d] = B[i].hi[d];
+      }
+    }
+  });
+}
+
+tah_status tah_schedule_points(const tah_schedule* s, int64_t* points) {
+  return guard([&] {
+    if (!s || !points) fail("null argument");
+    int64_t n = 0;
+    for (const auto& g : s->s->groups) n += g.points() * int64_t(g.targetblocks.size());
+    *points = n;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,69 @@
+// Device-side types shared by the fill kernels and the plan builder.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace tah {
+
+// Division of a 32-bit index by a run-time constant through a
+// multiply-high, as the reference's LinearShape does with multiplicative
+// inverses (src/threading.jl:240-267). Exact for every 32-bit numerator.
+struct FastDiv {
+  uint32_t d = 1, m = 0, s = 0;
+  FastDiv() = default;
+  explicit FastDiv(uint32_t div) : d(div) {
+    s = 0;
+    while ((uint64_t(1) << s) < div) ++s;
+    m = uint32_t(((uint64_t(1) << 32) * ((uint64_t(1) << s) - div)) / div + 1);
+  }
+  __host__ __device__ inline uint32_t div(uint32_t n) const {
+#ifdef __HIP_DEVICE_COMPILE__
+    uint32_t hi = __umulhi(n, m);
+#else
+    uint32_t hi = uint32_t((uint64_t(n) * m) >> 32);
+#endif
+    return uint32_t((uint64_t(hi) + n) >> s);
+  }
+};
+
+// One transfer group as the kernels read it: the flattened Stencil1D of
+// each dimension and the block lists, in device memory.
+struct alignas(16) DevGroup {
+  int32_t kind;
+  int32_t factorcol;
+  int32_t ntransfers;
+  int32_t unit;
+  int32_t n[3];       // box extents
+  int32_t order[3];   // stencil widths
+  int32_t tfirst[3];  // first target row, 0-based
+  int32_t points;     // n0*n1*n2
+  FastDiv dn[3];      // division by n0, n1, n2
+  int32_t pad;
+  const int32_t* tblocks;
+  const int32_t* sblocks;
+  const int32_t* srcstart[3];
+  const double* weights[3];
+};
+
+// One fill of one working array.
+struct alignas(16) Job {
+  const DevGroup* groups;
+  double* work;
+  const double* factors;
+  int64_t stride[5];
+  int32_t nvars;
+  int32_t pad;
+};
+
+// A slice of one transfer's values: group, transfer, first flat element of
+// the (box x nvars) space.
+struct alignas(16) Tile {
+  int32_t job;
+  int32_t group;
+  int32_t transfer;
+  int32_t e0;
+};
+
+}  // namespace tah

@@ -1,0 +1,474 @@
+"""Forest, Operators, FieldSet, GhostSchedule and fill_ghosts over libtah.
+
+Names, argument meaning and refusals follow the reference's API for the
+fill path: Forest (src/forest.jl:314-341), refine!/balance!
+(src/forest.jl:1761,1837), Operators (src/operators.jl:314-415),
+FieldSet (src/storage.jl:417-472), GhostSchedule (src/schedule.jl:3069-3292),
+fill_ghosts! and check_fill (src/ghosts.jl:1783-1794, 2058-2115).
+"""
+import ctypes as C
+from collections import namedtuple
+
+import numpy as np
+
+from . import _abi
+from ._abi import check, lib
+
+PointValue = 0
+Conservative = 1
+NoParity, EvenParity, OddParity = 0, 1, 2
+
+MortonKey = namedtuple("MortonKey", ["root", "level", "coords"])
+
+
+def cellcentered(D):
+    return ("cell",) * D
+
+
+def vertexcentered(D):
+    return ("vertex",) * D
+
+
+def _i32(seq):
+    arr = (C.c_int32 * len(seq))(*[int(x) for x in seq])
+    return arr
+
+
+class Forest:
+    """A brick of roots refined into a leaf-only octree (src/forest.jl:248-341)."""
+
+    def __init__(self, roots, *, N, periodic=None, reflecting=None, outflow=None, extents=None):
+        D = len(roots)
+        self.D = D
+        self.roots = tuple(int(r) for r in roots)
+        self.N = int(N)
+        self.periodic = tuple(bool(p) for p in (periodic or (False,) * D))
+        self.reflecting = tuple(tuple(bool(x) for x in f) for f in (reflecting or ((False, False),) * D))
+        self.outflow = tuple(tuple(bool(x) for x in f) for f in (outflow or ((False, False),) * D))
+        self.extents = tuple((float(a), float(b)) for a, b in
+                             (extents or tuple((0.0, float(r)) for r in self.roots)))
+        refl = [int(x) for f in self.reflecting for x in f]
+        outf = [int(x) for f in self.outflow for x in f]
+        ext = (C.c_double * (2 * D))(*[x for e in self.extents for x in e])
+        h = C.c_void_p()
+        check(lib().tah_forest_create(D, _i32(self.roots), self.N, _i32(self.periodic), _i32(refl),
+                                      _i32(outf), ext, C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _abi is not None and _abi._lib is not None:
+            _abi._lib.tah_forest_destroy(h)
+            self._h = None
+
+    @property
+    def nleaves(self):
+        n = C.c_int32()
+        check(lib().tah_forest_info(self._h, C.byref(n), None))
+        return n.value
+
+    @property
+    def generation(self):
+        g = C.c_int64()
+        check(lib().tah_forest_info(self._h, None, C.byref(g)))
+        return g.value
+
+    def leaf_arrays(self):
+        """(root[n], level[n], coords[n,3]) of the sorted leaves."""
+        n = self.nleaves
+        root = np.zeros(n, np.int32)
+        level = np.zeros(n, np.int32)
+        coords = np.zeros((n, 3), np.uint32)
+        check(lib().tah_forest_leaves(self._h, root.ctypes.data_as(_abi.I32P),
+                                      level.ctypes.data_as(_abi.I32P),
+                                      coords.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return root, level, coords
+
+    @property
+    def leaves(self):
+        root, level, coords = self.leaf_arrays()
+        return [MortonKey(int(r), int(l), tuple(int(c) for c in cc[:self.D]))
+                for r, l, cc in zip(root, level, coords)]
+
+    def refine(self, leaf_indices):
+        """refine! of the leaves at the given (0-based) indices."""
+        idx = np.ascontiguousarray(np.asarray(list(leaf_indices), dtype=np.int32))
+        check(lib().tah_forest_refine(self._h, len(idx), idx.ctypes.data_as(_abi.I32P)))
+        return self
+
+    def balance(self):
+        check(lib().tah_forest_balance(self._h))
+        return self
+
+    def isbalanced(self):
+        b = C.c_int32()
+        check(lib().tah_forest_isbalanced(self._h, C.byref(b)))
+        return bool(b.value)
+
+    # --- geometry (src/geometry.jl) ---------------------------------------
+    def root_spacing(self):
+        """Block width at level 0 per dimension."""
+        return np.array([(e[1] - e[0]) / r for e, r in zip(self.extents, self.roots)])
+
+    def block_geometry(self):
+        """Per leaf: the low corner of its block (n, D) and its cell size h (n,)."""
+        root, level, coords = self.leaf_arrays()
+        rp = np.zeros((len(root), self.D), np.int64)
+        r = root.astype(np.int64).copy()
+        for d in range(self.D):
+            rp[:, d] = r % self.roots[d]
+            r //= self.roots[d]
+        H = self.root_spacing()
+        scale = np.ldexp(1.0, -level.astype(np.int64))
+        lows = np.zeros((len(root), self.D))
+        for d in range(self.D):
+            g = rp[:, d] * (1 << 0) * np.ldexp(1.0, level) + coords[:, d]
+            lows[:, d] = self.extents[d][0] + g * H[d] * scale
+        h = H[0] * scale / self.N
+        return lows, h, level
+
+    def block_centers(self):
+        lows, h, level = self.block_geometry()
+        return lows + 0.5 * (h * self.N)[:, None]
+
+
+class Operators:
+    """Operators(; prolongation, restriction, family, bounded, vertex_prolongation)."""
+
+    def __init__(self, *, prolongation=None, restriction=None, family=PointValue, bounded=False,
+                 vertex_prolongation=None):
+        for name, p in (("prolongation", prolongation), ("restriction", restriction)):
+            if p is None:
+                raise ValueError(f"Operators has no default order: pass {name} explicitly")
+        self.family = int(family)
+        self.prolongation = int(prolongation)
+        self.restriction = int(restriction)
+        self.bounded = bool(bounded)
+        self.vertex_prolongation = int(vertex_prolongation or 0)
+        check(lib().tah_operators_check(C.byref(self._c())))
+
+    def _c(self):
+        return _abi.tah_operators(self.family, self.prolongation, self.restriction,
+                                  int(self.bounded), self.vertex_prolongation)
+
+
+def _centering_codes(centering, D):
+    if isinstance(centering, str):
+        centering = (centering,) * D
+    codes = []
+    for c in centering:
+        if c in ("cell", 0):
+            codes.append(0)
+        elif c in ("vertex", 1):
+            codes.append(1)
+        else:
+            raise ValueError(f"centering must be 'cell' or 'vertex', got {c!r}")
+    if len(codes) != D:
+        raise ValueError(f"centering needs one entry per dimension, got {centering}")
+    return tuple(codes)
+
+
+class Context:
+    """One device (tah_ctx)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().tah_ctx_create(int(device), C.byref(h)))
+        self._h = h
+        self.device = device
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _abi is not None and _abi._lib is not None:
+            _abi._lib.tah_ctx_destroy(h)
+
+
+_contexts = {}
+
+
+def default_context(device=0):
+    if device not in _contexts:
+        _contexts[device] = Context(device)
+    return _contexts[device]
+
+
+class GhostSchedule:
+    """The precomputed fill of one forest for one field-set layout."""
+
+    def __init__(self, fs_or_forest, operators, *, G=None, centering=None, nvars=None, parity=None):
+        if isinstance(fs_or_forest, FieldSet):
+            fs = fs_or_forest
+            forest, G, centering, nvars, parity = fs.forest, fs.G, fs.centering, fs.nvars, fs.parity
+        else:
+            forest = fs_or_forest
+            if G is None:
+                raise ValueError("GhostSchedule over a forest needs G")
+            centering = centering or cellcentered(forest.D)
+            nvars = nvars or 1
+        D = forest.D
+        self.forest = forest
+        self.operators = operators
+        self.G = tuple(int(g) for g in (G if hasattr(G, "__len__") else (G,) * D))
+        self.centering = _centering_codes(centering, D)
+        self.parity = None if parity is None else tuple(tuple(int(x) for x in p) for p in parity)
+        self.nvars = int(nvars)
+        par = None
+        if self.parity is not None:
+            flat = [x for p in self.parity for x in p]
+            par = (C.c_int8 * len(flat))(*flat)
+        h = C.c_void_p()
+        check(lib().tah_schedule_create(forest._h, C.byref(operators._c()), _i32(self.G),
+                                        _i32(self.centering), self.nvars, par, C.byref(h)))
+        self._h = h
+        self.generation = forest.generation
+        dp = C.POINTER(_abi.tah_schedule_desc)()
+        check(lib().tah_schedule_desc_get(h, C.byref(dp)))
+        self.desc = dp.contents
+        self._descptr = dp
+        self._plans = {}
+
+    def __del__(self):
+        self._plans = {}
+        h = getattr(self, "_h", None)
+        if h and _abi is not None and _abi._lib is not None:
+            _abi._lib.tah_schedule_destroy(h)
+
+    def isstale(self):
+        return self.forest.generation != self.generation
+
+    @property
+    def stored(self):
+        return tuple(self.desc.stored[:3])
+
+    def stage_tags(self):
+        return [self.desc.stage_tags[i] for i in range(self.desc.nstages)]
+
+    def stagegroups(self):
+        return [(self.desc.stagegroup_first[i], self.desc.stagegroup_last[i])
+                for i in range(self.desc.nstagegroups)]
+
+    def groups(self):
+        """Python view of the flattened groups (for tests and reporting)."""
+        out = []
+        for gi in range(self.desc.ngroups):
+            g = self.desc.groups[gi]
+            st = []
+            for d in range(3):
+                s = g.stencils[d]
+                st.append(dict(targetfirst=s.targetfirst, ntarget=s.ntarget, order=s.order,
+                               unit=bool(s.unit),
+                               srcstart=np.ctypeslib.as_array(s.srcstart, (s.ntarget,)).copy()
+                               if s.ntarget else np.zeros(0, np.int32),
+                               weights=np.ctypeslib.as_array(s.weights, (s.ntarget * s.order,))
+                               .reshape(s.ntarget, s.order).copy() if s.ntarget else
+                               np.zeros((0, s.order))))
+            n = g.ntransfers
+            out.append(dict(kind=g.kind, stage=g.stage, factorcol=g.factorcol, ntransfers=n,
+                            targetblocks=np.ctypeslib.as_array(g.targetblocks, (n,)).copy() if n else
+                            np.zeros(0, np.int32),
+                            sourceblocks=np.ctypeslib.as_array(g.sourceblocks, (n,)).copy() if n else
+                            np.zeros(0, np.int32),
+                            stencils=st))
+        return out
+
+    def boundaries(self):
+        n = C.c_int32()
+        check(lib().tah_schedule_boundaries(self._h, C.byref(n), None, None, None, None))
+        k = n.value
+        block = np.zeros(max(k, 1), np.int32)
+        direction = np.zeros((max(k, 1), 3), np.int32)
+        lo = np.zeros((max(k, 1), 3), np.int32)
+        hi = np.zeros((max(k, 1), 3), np.int32)
+        check(lib().tah_schedule_boundaries(self._h, C.byref(n), block.ctypes.data_as(_abi.I32P),
+                                            direction.ctypes.data_as(_abi.I32P),
+                                            lo.ctypes.data_as(_abi.I32P), hi.ctypes.data_as(_abi.I32P)))
+        return [(int(block[i]), tuple(direction[i]), tuple(lo[i]), tuple(hi[i])) for i in range(k)]
+
+    def points(self):
+        """Spatial points one fill writes (times nvars: values)."""
+        p = C.c_int64()
+        check(lib().tah_schedule_points(self._h, C.byref(p)))
+        return p.value
+
+    def plan(self, layout, ctx=None):
+        """The device plan of this schedule for a layout (uploaded once)."""
+        ctx = ctx or default_context()
+        key = (tuple(layout.stride), layout.nvars, ctx.device)
+        if key not in self._plans:
+            self._plans[key] = Plan(ctx, self, layout)
+        return self._plans[key]
+
+
+class Plan:
+    """A schedule's tables in HBM for one layout (tah_plan)."""
+
+    def __init__(self, ctx, schedule, layout):
+        self.ctx = ctx
+        self.schedule = schedule
+        self.layout = layout
+        h = C.c_void_p()
+        check(lib().tah_plan_create(ctx._h, schedule._descptr, C.byref(layout), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _abi is not None and _abi._lib is not None:
+            _abi._lib.tah_plan_destroy(h)
+
+    def values(self):
+        v = C.c_int64()
+        check(lib().tah_plan_values(self._h, C.byref(v)))
+        return v.value
+
+
+def _layout_of(desc_ptr, nvars, kind):
+    L = _abi.tah_layout()
+    fn = lib().tah_layout_dense if kind == "dense" else lib().tah_layout_arena
+    check(fn(desc_ptr, int(nvars), C.byref(L)))
+    return L
+
+
+class FieldSet:
+    """Field data over a forest: work[i, j, k, var, block] with explicit strides.
+
+    layout="dense" is the reference's array (src/storage.jl:196-200);
+    layout="arena" pads rows to 32-byte sectors (one HBM arena, Morton order).
+    """
+
+    def __init__(self, forest, nvars, *, G, centering=None, parity=None, device="cuda",
+                 layout="dense", fill=0.0):
+        import torch
+        D = forest.D
+        self.forest = forest
+        self.nvars = int(nvars)
+        self.G = tuple(int(g) for g in (G if hasattr(G, "__len__") else (G,) * D))
+        self.centering = _centering_codes(centering or cellcentered(D), D)
+        for d in range(D):
+            if forest.N < 2 * self.G[d]:
+                raise ValueError(f"N must be >= 2G in every dimension, got N={forest.N}, G={self.G}")
+        hasrefl = any(any(f) for f in forest.reflecting)
+        if parity is not None:
+            parity = tuple(tuple(int(x) for x in p) for p in parity)
+            if len(parity) != self.nvars:
+                raise ValueError("parity needs one entry per variable")
+            for v, p in enumerate(parity):
+                for d in range(D):
+                    if any(forest.reflecting[d]) and p[d] == NoParity:
+                        raise ValueError(f"variable {v + 1} has NoParity in dimension {d + 1}, which has "
+                                         "a reflecting face. Give it EvenParity or OddParity.")
+        elif hasrefl:
+            raise ValueError("a field set over a forest with reflecting faces needs a parity per variable")
+        self.parity = parity
+        self.stored = tuple(forest.N + 2 * self.G[d] + self.centering[d] for d in range(D)) + (1,) * (3 - D)
+        self.device = device
+        self.layout_kind = layout
+        # The layout needs only the stored size and the block count.
+        desc = _abi.tah_schedule_desc()
+        desc.dim = D
+        desc.nblocks = forest.nleaves
+        for d in range(3):
+            desc.stored[d] = self.stored[d]
+        self.layout = _layout_of(C.byref(desc), self.nvars, layout)
+        n = C.c_int64()
+        check(lib().tah_layout_size(C.byref(self.layout), C.byref(n)))
+        self.nelements = n.value
+        self.work = torch.full((self.nelements,), float(fill), dtype=torch.float64, device=device)
+        self.factors = None
+        if parity is not None:
+            ncols = 3 ** D
+            f = np.zeros(self.nvars * ncols)
+            flat = [x for p in parity for x in p]
+            check(lib().tah_parity_factors(D, self.nvars, (C.c_int8 * len(flat))(*flat),
+                                           f.ctypes.data_as(_abi.DP)))
+            self.factors = torch.tensor(f, dtype=torch.float64, device=device)
+        self.generation = forest.generation
+
+    @property
+    def nblocks(self):
+        return self.layout.nblocks
+
+    def array(self):
+        """A strided view indexed [block, var, k, j, i]."""
+        s = self.layout.stride
+        S = self.stored
+        return self.work.as_strided((self.nblocks, self.nvars, S[2], S[1], S[0]),
+                                    (s[4], s[3], s[2], s[1], 1))
+
+    def own_slices(self):
+        """Slices of the own range along (k, j, i)."""
+        D = self.forest.D
+        sl = []
+        for d in (2, 1, 0):
+            if d < D:
+                sl.append(slice(self.G[d], self.G[d] + self.forest.N + self.centering[d]))
+            else:
+                sl.append(slice(0, 1))
+        return tuple(sl)
+
+    def coordinates(self):
+        """Positions of every stored point: three arrays (nblocks, S2, S1, S0)."""
+        lows, h, _ = self.forest.block_geometry()
+        D = self.forest.D
+        out = []
+        S = self.stored
+        for d in range(3):
+            if d >= D:
+                out.append(np.zeros((self.nblocks, S[2], S[1], S[0])))
+                continue
+            I = np.arange(S[d]) - self.G[d]
+            off = I + (0.5 if self.centering[d] == 0 else 0.0)
+            x = lows[:, d][:, None] + off[None, :] * h[:, None]
+            shape = [self.nblocks, 1, 1, 1]
+            shape[3 - d] = S[d]
+            out.append(np.broadcast_to(x.reshape(shape), (self.nblocks, S[2], S[1], S[0])))
+        return out
+
+
+def _check_fill(fs, schedule):
+    """check_fill (src/ghosts.jl:2058-2115), raised before any launch."""
+    if fs.forest is not schedule.forest:
+        raise ValueError("the field set and the schedule were built over different forests")
+    if schedule.isstale():
+        raise _abi.StaleScheduleError(_abi.TAH_ESTALE, "the schedule is stale: the forest has changed "
+                                      "since it was built (refine, coarsen or regrid); rebuild it")
+    check(lib().tah_schedule_check(schedule._h))
+    if fs.G != schedule.G:
+        raise ValueError(f"the field set has G={fs.G}, the schedule was built for G={schedule.G}")
+    if fs.centering != schedule.centering:
+        raise ValueError("the field set and the schedule differ in centering")
+    if schedule.parity is not None and fs.parity != schedule.parity:
+        raise ValueError("the schedule was built for other parities than the field set's")
+
+
+def _stream_ptr(fs):
+    import torch
+    return C.c_void_p(torch.cuda.current_stream(fs.work.device).cuda_stream)
+
+
+def fill_ghosts(fs, schedule, *, boundary=None, method="staged", ctx=None):
+    """fill_ghosts!(fs, schedule): every ghost of fs.work from the schedule, on the GPU.
+
+    method="staged" (one launch per stage group, the default) or "plain" (one
+    launch per transfer group, the reference's device form).
+    """
+    if boundary is not None:
+        raise _abi.UnsupportedError(_abi.TAH_EUNSUPPORTED, "boundary hooks are not supported by "
+                                    "the device fill; pre-fill the outer-face regions instead")
+    _check_fill(fs, schedule)
+    if fs.work.device.type != "cuda":
+        raise RuntimeError("fill_ghosts runs on the GPU only: this field set lives on "
+                           f"{fs.work.device}; there is no CPU fallback")
+    plan = schedule.plan(fs.layout, ctx)
+    fptr = C.c_void_p(fs.factors.data_ptr()) if fs.factors is not None else None
+    fn = lib().tah_fill if method == "staged" else lib().tah_fill_plain
+    check(fn(plan._h, C.c_void_p(fs.work.data_ptr()), fptr, _stream_ptr(fs)))
+    return fs
+
+
+def fill_ghosts_adjoint(fs, schedule, adj, *, ctx=None):
+    """The transpose of the fill applied in place to adj (a tensor shaped like fs.work)."""
+    _check_fill(fs, schedule)
+    plan = schedule.plan(fs.layout, ctx)
+    fptr = C.c_void_p(fs.factors.data_ptr()) if fs.factors is not None else None
+    check(lib().tah_fill_adjoint(plan._h, C.c_void_p(adj.data_ptr()), fptr, _stream_ptr(fs)))
+    return adj

@@ -1,0 +1,153 @@
+"""Seeded synthetic meshes and fields of the benchmark configurations.
+
+Each builder returns (forest, operators, fieldset-kwargs, description). Data
+generators write own points only; ghosts start as NaN (or a given value) so
+that a ghost the fill misses stays visible.
+
+  config1: 2x2x2 periodic roots, N=8, cell, G=2, 4 vars, Conservative(3,2);
+           every root refined once, then every level-1 leaf whose centre is
+           within 0.3 of the domain centre: 120 blocks (56 + 64).
+  config5: 8x8x8 roots, N=32, cell, G=3, 20 vars, PointValue(6,6); refined
+           twice where leaves meet 4 seeded spheres, 2:1 balanced; reflecting
+           z-low face (parities alternating even/odd in z), outflow elsewhere.
+  config3: 64 independent forests drawn from seeds 2..65 (see random_forest).
+"""
+import math
+
+import numpy as np
+
+from .core import (Conservative, EvenParity, FieldSet, Forest, OddParity, Operators,
+                   PointValue)
+
+
+def config1_forest():
+    f = Forest((2, 2, 2), N=8, periodic=(True,) * 3, extents=((0.0, 1.0),) * 3)
+    f.refine(range(f.nleaves))
+    c = f.block_centers()
+    idx = [i for i in range(f.nleaves) if np.linalg.norm(c[i] - 0.5) < 0.3]
+    f.refine(idx)
+    f.balance()
+    return f
+
+
+def config1():
+    f = config1_forest()
+    ops = Operators(prolongation=3, restriction=2, family=Conservative)
+    return f, ops, dict(nvars=4, G=2), "config1: 2^3 periodic roots, N=8, cell, G=2, 4 vars, Conservative(3,2)"
+
+
+def _spheres(seed, n, rlo, rhi):
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(0.15, 0.85, size=(n, 3))
+    radii = rng.uniform(rlo, rhi, size=n)
+    return centres, radii
+
+
+def _box_meets_sphere(lo, hi, c, r):
+    q = np.clip(c[None, :], lo, hi)
+    return np.sum((q - c[None, :]) ** 2, axis=1) <= r * r
+
+
+def config5_forest(roots=8, N=32, seed=5, levels=2, spheres=4, radius=(0.12, 0.16)):
+    f = Forest((roots,) * 3, N=N, extents=((0.0, 1.0),) * 3,
+               reflecting=((False, False), (False, False), (True, False)),
+               outflow=((True, True), (True, True), (False, True)))
+    centres, radii = _spheres(seed, spheres, *radius)
+    for lvl in range(levels):
+        lows, h, level = f.block_geometry()
+        hi = lows + (h * f.N)[:, None]
+        hit = np.zeros(f.nleaves, bool)
+        for c, r in zip(centres, radii):
+            hit |= _box_meets_sphere(lows, hi, c, r)
+        hit &= level == lvl
+        f.refine(np.nonzero(hit)[0])
+        f.balance()
+    return f
+
+
+def config5(nvars=20, roots=8, N=32):
+    f = config5_forest(roots=roots, N=N)
+    ops = Operators(prolongation=6, restriction=6)
+    parity = tuple((EvenParity, EvenParity, EvenParity if v % 2 == 0 else OddParity)
+                   for v in range(nvars))
+    return f, ops, dict(nvars=nvars, G=3, parity=parity), (
+        f"config5: {roots}^3 roots, N={N}, cell, G=3, {nvars} vars, PointValue(6,6), "
+        "reflecting z-lo, outflow elsewhere")
+
+
+def fill_uniform(fs, seed, lo=-1.0, hi=1.0):
+    """Own points uniform in [lo, hi) from a seeded generator (torch, on fs's device)."""
+    import torch
+    g = torch.Generator(device=fs.work.device)
+    g.manual_seed(seed)
+    a = fs.array()
+    own = (slice(None), slice(None)) + fs.own_slices()
+    shape = a[own].shape
+    a[own] = torch.rand(shape, generator=g, dtype=torch.float64, device=fs.work.device) * (hi - lo) + lo
+    return fs
+
+
+def fill_gaussians(fs, seed, noise=1e-4):
+    """Own points = a Gaussian bump per variable (seeded centre and width) plus noise."""
+    import torch
+    rng = np.random.default_rng(seed)
+    dev = fs.work.device
+    lows, h, _ = fs.forest.block_geometry()
+    lows_t = torch.tensor(lows, dtype=torch.float64, device=dev)
+    h_t = torch.tensor(h, dtype=torch.float64, device=dev)
+    N = fs.forest.N
+    a = fs.array()
+    own = fs.own_slices()
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    idx = torch.arange(N, dtype=torch.float64, device=dev) + 0.5
+    x = [lows_t[:, d, None] + idx[None, :] * h_t[:, None] for d in range(3)]  # (nb, N)
+    for v in range(fs.nvars):
+        c = rng.uniform(0.2, 0.8, size=3)
+        s = rng.uniform(0.05, 0.2)
+        ex = [torch.exp(-((x[d] - c[d]) ** 2) / (2 * s * s)) for d in range(3)]
+        val = ex[2][:, :, None, None] * ex[1][:, None, :, None] * ex[0][:, None, None, :]
+        val = val + noise * (torch.rand(val.shape, generator=g, dtype=torch.float64, device=dev) * 2 - 1)
+        a[(slice(None), v) + own] = val
+    return fs
+
+
+def make_fieldset(forest, device="cuda", layout="dense", fill=float("nan"), **kw):
+    return FieldSet(forest, kw.pop("nvars"), device=device, layout=layout, fill=fill, **kw)
+
+
+def random_forest(seed, max_blocks=4096):
+    """One forest of the batched configuration, drawn from `seed`: brick of
+    r^3 roots (r in 1..4), N in {8,16,32}, 1-4 refinement levels around 1-3
+    spheres of radius 0.05-0.3, G in {2,3}, 1-3 field sets of 2-10 variables.
+    All faces periodic; PointValue of order 2G. Cell centering (vertex
+    centering is not yet supported by this library). Draws whose blocks
+    exceed max_blocks are redrawn with fewer levels."""
+    rng = np.random.default_rng(seed)
+    r = int(rng.integers(1, 5))
+    N = int(rng.choice([8, 16, 32]))
+    levels = int(rng.integers(1, 5))
+    nsph = int(rng.integers(1, 4))
+    centres = rng.uniform(0.1, 0.9, size=(nsph, 3))
+    radii = rng.uniform(0.05, 0.3, size=nsph)
+    G = int(rng.choice([2, 3]))
+    nsets = int(rng.integers(1, 4))
+    nvars = [int(rng.integers(2, 11)) for _ in range(nsets)]
+    while True:
+        f = Forest((r,) * 3, N=N, periodic=(True,) * 3, extents=((0.0, 1.0),) * 3)
+        for lvl in range(levels):
+            lows, h, level = f.block_geometry()
+            hi = lows + (h * f.N)[:, None]
+            hit = np.zeros(f.nleaves, bool)
+            for c, rad in zip(centres, radii):
+                hit |= _box_meets_sphere(lows, hi, c, rad)
+            hit &= level == lvl
+            f.refine(np.nonzero(hit)[0])
+            f.balance()
+            if f.nleaves > max_blocks:
+                break
+        if f.nleaves <= max_blocks or levels == 1:
+            break
+        levels -= 1
+    ops = Operators(prolongation=2 * G, restriction=2 * G)
+    return f, ops, G, nvars
