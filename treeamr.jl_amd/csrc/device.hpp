@@ -39,17 +39,32 @@ struct alignas(16) DevGroup {
   int32_t order[3];   // stencil widths
   int32_t tfirst[3];  // first target row, 0-based
   int32_t points;     // n0*n1*n2
+  int32_t ca[3];      // a unit copy's source row of target row 0 ...
+  int32_t cs[3];      // ... and its step per row (1, -1 or 0)
   FastDiv dn[3];      // division by n0, n1, n2
-  int32_t pad;
   const int32_t* tblocks;
   const int32_t* sblocks;
   const int32_t* srcstart[3];
   const double* weights[3];
 };
 
+// A sub-box of an interpolating group in which every thread computes the
+// same shape of targets: R_d rows along dimension d (R_d = 2: a pair of
+// rows whose windows start SHIFT apart), rows[d] listing them unit by unit.
+struct alignas(16) DevSub {
+  int32_t group;
+  int32_t units;   // cnt0*cnt1*cnt2
+  int32_t cnt[3];
+  int32_t pad;
+  FastDiv dc[2];   // division by cnt0, cnt1
+  FastDiv du;      // division by units
+  const int32_t* rows[3];
+};
+
 // One fill of one working array.
 struct alignas(16) Job {
   const DevGroup* groups;
+  const DevSub* subs;
   double* work;
   const double* factors;
   int64_t stride[5];
@@ -57,8 +72,7 @@ struct alignas(16) Job {
   int32_t pad;
 };
 
-// A slice of one transfer's values: group, transfer, first flat element of
-// the (box x nvars) space.
+// A slice of work: job, group (or sub-box), transfer, first flat element.
 struct alignas(16) Tile {
   int32_t job;
   int32_t group;
