@@ -45,6 +45,13 @@ const char* tah_last_error(void);
 /* Host side: the forest, the operators and the schedule builder.       */
 /* ------------------------------------------------------------------ */
 
+/* unit_lagrange_weights(p, num//den) (src/operators.jl:675-682): the
+ * weights of nodes 0..p-1 at num/den, exact and rounded once to double. */
+tah_status tah_lagrange_weights(int32_t p, int64_t num, int64_t den, double* out);
+/* conservative_prolong_weights(p) (src/schedule.jl:1097-1120): the low and
+ * high subcell weight vectors of odd order p. */
+tah_status tah_conservative_weights(int32_t p, double* low, double* high);
+
 typedef struct tah_forest tah_forest;
 typedef struct tah_schedule tah_schedule;
 

@@ -1,0 +1,385 @@
+"""CPU tests: the C ABI's symbols, the operators and their weights, the
+forest, the schedule (partition, polynomial exactness, boundary faces), and
+the oracle itself. No GPU is touched here.
+
+Modelled on the reference's own tests: test/ghost_tests.jl (weights KATs
+:5-110, check_operators :112-212, partition :214-245, polynomial exactness
+:247-301), test/ghost_oracles.jl (write_counts :228-255, exchange_error
+:111-117), test/outflow_tests.jl (:123-193) and test/tree_tests.jl.
+"""
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from treeamr_hip import (Conservative, EvenParity, FieldSet, Forest, GhostSchedule, OddParity,
+                         Operators, StaleScheduleError, UnsupportedError)
+from treeamr_hip import _abi
+from treeamr_hip import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# --- the boundary ------------------------------------------------------------
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "tah.h")).read()
+    declared = set(re.findall(r"^\s*(?:tah_status|const char\*)\s+(tah_\w+)\s*\(", header, re.M))
+    assert len(declared) > 30
+    lib = _abi.lib()
+    missing = [n for n in sorted(declared) if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the Python bindings type every one of them
+    assert declared - {"tah_last_error"} <= set(_abi.SIGNATURES)
+
+
+# --- weights (test/ghost_tests.jl:27-53, 56-97) ------------------------------
+
+def _lagrange(p, num, den):
+    out = np.zeros(p)
+    _abi.check(_abi.lib().tah_lagrange_weights(p, num, den, out.ctypes.data_as(_abi.DP)))
+    return out
+
+
+def _conservative(p):
+    lo, hi = np.zeros(p), np.zeros(p)
+    _abi.check(_abi.lib().tah_conservative_weights(p, lo.ctypes.data_as(_abi.DP),
+                                                   hi.ctypes.data_as(_abi.DP)))
+    return lo, hi
+
+
+def test_lagrange_known_answers():
+    # [1/4, 3/4] a quarter cell off centre, [1/2, 1/2] at a midpoint,
+    # the order-4 midpoint [-1/16, 9/16, 9/16, -1/16], a node [0, 0, 0, 1].
+    assert list(_lagrange(2, 3, 4)) == [0.25, 0.75]
+    assert list(_lagrange(2, 1, 2)) == [0.5, 0.5]
+    assert list(_lagrange(4, 3, 2)) == [-1 / 16, 9 / 16, 9 / 16, -1 / 16]
+    assert list(_lagrange(4, 3, 1)) == [0.0, 0.0, 0.0, 1.0]
+
+
+def test_weights_match_exact_rational_fixtures():
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "weights.json")))
+    for e in fx["lagrange"]:
+        got = _lagrange(e["p"], e["num"], e["den"])
+        assert list(got) == e["float"], e
+    for e in fx["conservative"]:
+        lo, hi = _conservative(e["p"])
+        assert list(lo) == e["low_float"] and list(hi) == e["high_float"], e
+
+
+def test_conservative_known_answers():
+    lo, hi = _conservative(3)
+    assert list(lo) == [1 / 8, 1.0, -1 / 8] and list(hi) == [-1 / 8, 1.0, 1 / 8]
+    lo, hi = _conservative(1)
+    assert list(lo) == [1.0] and list(hi) == [1.0]
+
+
+# --- operators and their refusals (test/ghost_tests.jl:5-25, 112-212) --------
+
+@pytest.mark.parametrize("kw", [
+    dict(prolongation=1, restriction=2), dict(prolongation=3, restriction=2),
+    dict(prolongation=2, restriction=0), dict(prolongation=2, restriction=2, family=Conservative),
+    dict(prolongation=3, restriction=4, family=Conservative)])
+def test_operators_refused(kw):
+    with pytest.raises(ValueError):
+        Operators(**kw)
+
+
+def test_operators_need_both_orders():
+    with pytest.raises(ValueError, match="no default order"):
+        Operators(prolongation=2)
+
+
+def test_check_operators_against_the_layout():
+    f = Forest((2, 2, 2), N=8, periodic=(True,) * 3)
+    with pytest.raises(ValueError, match="needs G >= 3"):
+        GhostSchedule(f, Operators(prolongation=6, restriction=2), G=2)
+    f16 = Forest((2, 2, 2), N=6, periodic=(True,) * 3)
+    with pytest.raises(ValueError, match="N >="):
+        GhostSchedule(f16, Operators(prolongation=6, restriction=6), G=3)
+    with pytest.raises(ValueError, match="N must be >= 2G"):
+        FieldSet(f, 1, G=5, device="cpu")
+
+
+def test_bounded_and_vertex_refused_as_unsupported():
+    with pytest.raises(UnsupportedError):
+        Operators(prolongation=4, restriction=4, bounded=True)
+    f = Forest((2, 2, 2), N=8, periodic=(True,) * 3)
+    with pytest.raises(UnsupportedError):
+        GhostSchedule(f, Operators(prolongation=4, restriction=4), G=2, centering="vertex")
+
+
+# --- the forest (test/tree_tests.jl) ---------------------------------------
+
+def test_forest_basics_and_balance():
+    f = Forest((2, 2, 2), N=8, periodic=(True,) * 3)
+    assert f.nleaves == 8 and f.generation == 0
+    f.refine([0])
+    assert f.nleaves == 15 and f.generation == 1
+    keys = f.leaves
+    assert keys[0] == (0, 1, (0, 0, 0)) and keys[7] == (0, 1, (1, 1, 1))
+    f.refine([0])  # a level-2 leaf next to level-0 roots: unbalanced
+    assert not f.isbalanced()
+    f.balance()
+    assert f.isbalanced()
+    with pytest.raises(ValueError):
+        Forest((2, 2, 2), N=7)
+    with pytest.raises(ValueError):
+        Forest((2, 2, 2), N=8, periodic=(True, False, False), reflecting=((True, False),) * 3)
+
+
+def test_config1_mesh_is_120_blocks():
+    f = W.config1_forest()
+    _, _, level = f.block_geometry()
+    assert f.nleaves == 120 and list(np.bincount(level)) == [0, 56, 64]
+
+
+def test_curve_order_is_sorted_and_unique():
+    f = W.config5_forest(roots=2, N=16)
+    keys = f.leaves
+    assert len(set(keys)) == len(keys)
+    # descendants follow their root: roots non-decreasing along the curve
+    roots = [k.root for k in keys]
+    assert roots == sorted(roots)
+
+
+# --- the schedule --------------------------------------------------------------
+
+def _ownmask(fs):
+    counts_shape = (fs.nblocks,) + tuple(reversed(fs.stored))
+    own = np.zeros(counts_shape, bool)
+    own[(slice(None),) + fs.own_slices()] = True
+    return own
+
+
+@pytest.mark.parametrize("build", ["config1", "config5_small", "random"])
+def test_every_ghost_written_exactly_once(build):
+    """The partition oracle (test/ghost_oracles.jl:228-255): no own point is
+    written, every ghost exactly once (periodic, reflecting and outflow faces
+    leave no hook region)."""
+    if build == "config1":
+        f, ops, kw, _ = W.config1()
+    elif build == "config5_small":
+        f, ops, kw, _ = W.config5(nvars=2, roots=4, N=16)
+    else:
+        f, ops, G, nv = W.random_forest(7, max_blocks=400)
+        kw = dict(nvars=nv[0], G=G)
+    kw = dict(kw)
+    fs = FieldSet(f, kw.pop("nvars"), device="cpu", **kw)
+    s = GhostSchedule(fs, ops)
+    wc = oracle.write_counts(s._descptr, fs.layout)
+    own = _ownmask(fs)
+    assert wc[own].max() == 0
+    assert wc[~own].min() == 1 and wc[~own].max() == 1
+    assert s.points() == int(wc.sum())
+
+
+def _nested(N, refl=None, outf=None, periodic=False):
+    f = Forest((2, 2, 2), N=N, periodic=(periodic,) * 3, extents=((0, 1),) * 3,
+               reflecting=refl, outflow=outf)
+    for it in range(2):
+        c = f.block_centers()
+        f.refine([i for i in range(f.nleaves) if np.all(c[i] < 0.5 - 0.1 * it)])
+        f.balance()
+    return f
+
+
+def _poly(deg):
+    return lambda x, v: sum(0.3 * v + 0.7 * d + 0.11 * (d + v) * x[d] ** e
+                            for d in range(3) for e in range(deg + 1))
+
+
+def _exchange_error(f, ops, fn, G, nvars=2):
+    """exchange_error (test/ghost_oracles.jl:111-117): own points and the
+    outer-face regions hold fn, the fill computes every other ghost; the
+    worst deviation from fn anywhere."""
+    fs = FieldSet(f, nvars, G=G, device="cpu", fill=float("nan"))
+    s = GhostSchedule(fs, ops)
+    X = fs.coordinates()
+    a = fs.array().numpy()
+    for v in range(nvars):
+        exact = fn(X, v + 1)
+        a[(slice(None), v) + fs.own_slices()] = exact[(slice(None),) + fs.own_slices()]
+        for (b, _, lo, hi) in s.boundaries():
+            sl = (slice(lo[2], hi[2]), slice(lo[1], hi[1]), slice(lo[0], hi[0]))
+            a[(b, v) + sl] = exact[(b,) + sl]
+    oracle.fill(s._descptr, fs.layout, fs.work.numpy(), None, nthreads=4)
+    assert not np.isnan(a).any(), "a ghost was left unwritten"
+    return max(float(np.abs(a[:, v] - fn(X, v + 1)).max()) for v in range(nvars))
+
+
+@pytest.mark.parametrize("p", [2, 4, 6])
+def test_pointvalue_polynomial_exactness(p):
+    """Order p reproduces degree p-1 and not degree p (test/ghost_tests.jl:247-281)."""
+    f = _nested(N=8)
+    ops = Operators(prolongation=p, restriction=p)
+    assert _exchange_error(f, ops, _poly(p - 1), G=p // 2) < 1e-12
+    assert _exchange_error(f, ops, _poly(p), G=p // 2) > 1e-10
+
+
+GAUSS_X = [-0.906179845938664, -0.5384693101056831, 0.0, 0.5384693101056831, 0.906179845938664]
+GAUSS_W = [0.23692688505618908, 0.47862867049936647, 0.5688888888888889,
+           0.47862867049936647, 0.23692688505618908]
+
+
+def _cell_averages(fn, X, h):
+    tot = 0.0
+    for i, (xi, wi) in enumerate(zip(GAUSS_X, GAUSS_W)):
+        for j, (xj, wj) in enumerate(zip(GAUSS_X, GAUSS_W)):
+            for k, (xk, wk) in enumerate(zip(GAUSS_X, GAUSS_W)):
+                pt = [X[0] + h * xi / 2, X[1] + h * xj / 2, X[2] + h * xk / 2]
+                tot = tot + wi * wj * wk / 8 * fn(pt)
+    return tot
+
+
+@pytest.mark.parametrize("p", [1, 3, 5])
+def test_conservative_cell_average_exactness(p):
+    """The conservative family reproduces cell averages of degree p-1, not p
+    (test/ghost_tests.jl:99-110, test/ghost_oracles.jl:14-84)."""
+    f = _nested(N=8)
+    G = max(1, (p - 1) // 2)
+    ops = Operators(prolongation=p, restriction=2, family=Conservative)
+
+    def err(deg):
+        fn = lambda x: sum(0.7 * d + 0.31 * (d + 1) * x[d] ** e for d in range(3) for e in range(deg + 1))
+        fs = FieldSet(f, 1, G=G, device="cpu", fill=float("nan"))
+        s = GhostSchedule(fs, ops)
+        _, h, _ = f.block_geometry()
+        X = fs.coordinates()
+        hh = h[:, None, None, None]
+        exact = _cell_averages(fn, X, hh)
+        a = fs.array().numpy()
+        a[(slice(None), 0) + fs.own_slices()] = exact[(slice(None),) + fs.own_slices()]
+        for (b, _, lo, hi) in s.boundaries():
+            sl = (slice(lo[2], hi[2]), slice(lo[1], hi[1]), slice(lo[0], hi[0]))
+            a[(b, 0) + sl] = exact[(b,) + sl]
+        oracle.fill(s._descptr, fs.layout, fs.work.numpy(), None, nthreads=4)
+        return float(np.abs(a[:, 0] - exact).max())
+
+    assert err(p - 1) < 1e-11
+    if p > 1:
+        assert err(p) > 1e-9
+
+
+def test_outflow_constant_field_stays_constant():
+    """A constant field is 1.0 everywhere after a fill over outflow faces
+    (test/outflow_tests.jl:179-193)."""
+    out = ((True, True),) * 3
+    f = _nested(N=8, outf=out)
+    fs = FieldSet(f, 2, G=3, device="cpu", fill=float("nan"))
+    a = fs.array().numpy()
+    a[(slice(None), slice(None)) + fs.own_slices()] = 1.0
+    s = GhostSchedule(fs, Operators(prolongation=6, restriction=6))
+    assert s.boundaries() == []
+    oracle.fill(s._descptr, fs.layout, fs.work.numpy(), None)
+    assert (a == 1.0).all()
+
+
+def test_outflow_repeats_nearest_own_point_on_a_uniform_mesh():
+    """Outflow ghosts repeat the nearest owned point (test/outflow_tests.jl:123-151)."""
+    f = Forest((2, 2, 2), N=8, extents=((0, 1),) * 3, outflow=((True, True),) * 3)
+    fs = FieldSet(f, 1, G=2, device="cpu", fill=float("nan"))
+    rng = np.random.default_rng(3)
+    a = fs.array().numpy()
+    a[(slice(None), slice(None)) + fs.own_slices()] = rng.uniform(-1, 1, size=(8, 1, 8, 8, 8))
+    s = GhostSchedule(fs, Operators(prolongation=4, restriction=4))
+    oracle.fill(s._descptr, fs.layout, fs.work.numpy(), None)
+    # assemble the global 16^3 own field and compare every ghost with it clamped
+    root, level, coords = f.leaf_arrays()
+    glob = np.zeros((16, 16, 16))
+    for b in range(8):
+        r = root[b]
+        px, py, pz = r % 2, (r // 2) % 2, r // 4
+        glob[pz * 8:(pz + 1) * 8, py * 8:(py + 1) * 8, px * 8:(px + 1) * 8] = a[b, 0, 2:10, 2:10, 2:10]
+    for b in range(8):
+        r = root[b]
+        px, py, pz = r % 2, (r // 2) % 2, r // 4
+        for k in range(12):
+            for j in range(12):
+                for i in range(12):
+                    gz = np.clip(pz * 8 + k - 2, 0, 15)
+                    gy = np.clip(py * 8 + j - 2, 0, 15)
+                    gx = np.clip(px * 8 + i - 2, 0, 15)
+                    assert a[b, 0, k, j, i] == glob[gz, gy, gx]
+
+
+def test_reflecting_parities_on_a_uniform_mesh():
+    """Across a reflecting face a ghost is its mirror point times the
+    variable's parity sign, bit for bit (an even variable mirrored, an odd one
+    negated; the reference's factor rule, src/storage.jl:536-550)."""
+    f = Forest((1, 1, 2), N=8, extents=((0, 1), (0, 1), (0, 2)), periodic=(True, True, False),
+               reflecting=((False, False), (False, False), (True, True)))
+    par = ((EvenParity,) * 3, (EvenParity, EvenParity, OddParity))
+    fs = FieldSet(f, 2, G=2, device="cpu", fill=float("nan"), parity=par)
+    rng = np.random.default_rng(4)
+    a = fs.array().numpy()
+    a[(slice(None), slice(None)) + fs.own_slices()] = rng.uniform(-1, 1, size=(2, 2, 8, 8, 8))
+    s = GhostSchedule(fs, Operators(prolongation=4, restriction=4))
+    oracle.fill(s._descptr, fs.layout, fs.work.numpy(), fs.factors.numpy())
+    # block 0 spans z in [0,1): its low z ghosts k=0,1 mirror own k=3,2
+    for v, sign in ((0, 1.0), (1, -1.0)):
+        for k, ks in ((0, 3), (1, 2)):
+            np.testing.assert_array_equal(a[0, v, k, 2:10, 2:10], sign * a[0, v, ks, 2:10, 2:10] + 0.0)
+
+
+def test_stale_schedule_refused():
+    f = Forest((2, 2, 2), N=8, periodic=(True,) * 3)
+    fs = FieldSet(f, 1, G=1, device="cpu")
+    s = GhostSchedule(fs, Operators(prolongation=2, restriction=2))
+    f.refine([0])
+    f.balance()
+    from treeamr_hip.core import _check_fill
+    with pytest.raises(StaleScheduleError):
+        _check_fill(fs, s)
+
+
+def test_parity_factor_table():
+    par = np.array([[1, 1, 1], [1, 1, 2]], np.int8).ravel()
+    out = np.zeros(2 * 27)
+    _abi.check(_abi.lib().tah_parity_factors(3, 2, par.ctypes.data_as(_abi.C.POINTER(_abi.C.c_int8)),
+                                             out.ctypes.data_as(_abi.DP)))
+    f = out.reshape(27, 2)  # column-major: factors[v + nvars*(col-1)]
+    assert (f[0] == 1).all()                 # no mirror
+    assert list(f[9]) == [1.0, -1.0]         # mirrored along z (state 1): sigma
+    assert list(f[18]) == [1.0, 0.0]         # wall projection along z (state 2): (1+sigma)/2
+
+
+# --- the oracle itself ----------------------------------------------------------
+
+def test_oracle_one_tap_skip_equals_full_sum_on_finite_data():
+    """The CPU's one-tap skip and the full sum agree bit for bit on finite data
+    (test/ghost_tests.jl:562-634)."""
+    f, ops, kw, _ = W.config5(nvars=2, roots=2, N=16)
+    kw = dict(kw)
+    fs = FieldSet(f, kw.pop("nvars"), device="cpu", **kw)
+    W.fill_uniform(fs, 9)
+    s = GhostSchedule(fs, ops)
+    w1 = fs.work.numpy().copy()
+    w2 = w1.copy()
+    fac = fs.factors.numpy()
+    oracle.fill(s._descptr, fs.layout, w1, fac, nthreads=3, skip=True)
+    oracle.fill(s._descptr, fs.layout, w2, fac, nthreads=1, skip=False)
+    assert np.array_equal(w1.view(np.int64), w2.view(np.int64))
+
+
+def test_oracle_adjoint_dot_product():
+    """<fill(x), y> = <x, fill^T(y)> for the CPU transpose, with x own-only."""
+    f, ops, kw, _ = W.config1()
+    kw = dict(kw)
+    fs = FieldSet(f, kw.pop("nvars"), device="cpu", **kw)
+    s = GhostSchedule(fs, ops)
+    rng = np.random.default_rng(0)
+    own = np.zeros(fs.array().shape, bool)
+    own[(slice(None), slice(None)) + fs.own_slices()] = True
+    x = np.where(own, rng.uniform(-1, 1, own.shape), 0.0).ravel()
+    y = rng.uniform(-1, 1, own.shape).ravel()
+    fx = x.copy()
+    oracle.fill(s._descptr, fs.layout, fx, None)
+    aty = y.copy()
+    oracle.fill_adjoint(s._descptr, fs.layout, aty, None)
+    lhs = float(fx @ y)
+    rhs = float(x @ aty)
+    assert abs(lhs - rhs) <= 1e-13 * max(abs(lhs), 1.0)

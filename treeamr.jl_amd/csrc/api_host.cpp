@@ -106,6 +106,26 @@ tah_status tah_forest_isbalanced(const tah_forest* f, int32_t* balanced) {
   });
 }
 
+tah_status tah_lagrange_weights(int32_t p, int64_t num, int64_t den, double* out) {
+  return guard([&] {
+    if (!out || p < 1 || den == 0) fail("bad argument");
+    const auto& w = unit_lagrange_weights(p, Rational(num, den));
+    for (int i = 0; i < p; ++i) out[i] = w[i].to_double();
+  });
+}
+
+tah_status tah_conservative_weights(int32_t p, double* low, double* high) {
+  return guard([&] {
+    if (!low || !high || p < 1 || p % 2 == 0) fail("conservative order must be odd and positive");
+    std::vector<Rational> lo, hi;
+    conservative_prolong_weights(p, lo, hi);
+    for (int i = 0; i < p; ++i) {
+      low[i] = lo[i].to_double();
+      high[i] = hi[i].to_double();
+    }
+  });
+}
+
 tah_status tah_operators_check(const tah_operators* ops) {
   return guard([&] {
     if (!ops) fail("null operators");

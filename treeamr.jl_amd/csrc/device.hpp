@@ -7,13 +7,16 @@
 
 namespace tah {
 
+// Threads per workgroup of every tile kernel (four wave64s).
+constexpr int kThreads = 256;
+
 // Division of a 32-bit index by a run-time constant through a
 // multiply-high, as the reference's LinearShape does with multiplicative
 // inverses (src/threading.jl:240-267). Exact for every 32-bit numerator.
 struct FastDiv {
   uint32_t d = 1, m = 0, s = 0;
-  FastDiv() = default;
-  explicit FastDiv(uint32_t div) : d(div) {
+  __host__ __device__ FastDiv() {}
+  __host__ __device__ explicit FastDiv(uint32_t div) : d(div) {
     s = 0;
     while ((uint64_t(1) << s) < div) ++s;
     m = uint32_t(((uint64_t(1) << 32) * ((uint64_t(1) << s) - div)) / div + 1);
@@ -79,5 +82,26 @@ struct alignas(16) Tile {
   int32_t transfer;
   int32_t e0;
 };
+
+// What an interpolation tile stages in LDS: the source box lo..lo+ext-1
+// (stored indices) of variables v0..v0+nv-1, and how many of the tile's
+// flat elements it computes.
+struct alignas(16) TileBox {
+  int32_t lo[3];
+  int32_t ext[3];
+  int32_t v0, nv;
+  int32_t count;
+  int32_t pad[3];
+};
+
+// Workgroups are dealt round-robin to the eight XCDs, each with its own L2:
+// give XCD x the x-th contiguous share of the tiles, so that neighbouring
+// tiles (the same transfer, the same blocks) meet in one L2.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
+  constexpr uint32_t X = 8;
+  const uint32_t per = n / X, rem = n % X;
+  const uint32_t x = b % X, i = b / X;
+  return x * per + (x < rem ? x : rem) + i;
+}
 
 }  // namespace tah

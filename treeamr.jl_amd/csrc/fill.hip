@@ -33,39 +33,41 @@ int interp_index(int P, int shift) {
 namespace {
 
 template <int P, int S>
-void launch_interp_r(int rbits, dim3 grid, hipStream_t s, const Job* j, const Tile* t) {
+void launch_interp_r(int rbits, dim3 grid, hipStream_t s, const Job* j, const Tile* t,
+                     const TileBox* b, uint32_t lds) {
   dim3 blk(kThreads);
   switch (rbits) {
-    case 0: hipLaunchKernelGGL((interp_kernel<P, S, 1, 1, 1>), grid, blk, 0, s, j, t); break;
-    case 1: hipLaunchKernelGGL((interp_kernel<P, S, 2, 1, 1>), grid, blk, 0, s, j, t); break;
-    case 2: hipLaunchKernelGGL((interp_kernel<P, S, 1, 2, 1>), grid, blk, 0, s, j, t); break;
-    case 3: hipLaunchKernelGGL((interp_kernel<P, S, 2, 2, 1>), grid, blk, 0, s, j, t); break;
-    case 4: hipLaunchKernelGGL((interp_kernel<P, S, 1, 1, 2>), grid, blk, 0, s, j, t); break;
-    case 5: hipLaunchKernelGGL((interp_kernel<P, S, 2, 1, 2>), grid, blk, 0, s, j, t); break;
-    case 6: hipLaunchKernelGGL((interp_kernel<P, S, 1, 2, 2>), grid, blk, 0, s, j, t); break;
-    default: hipLaunchKernelGGL((interp_kernel<P, S, 2, 2, 2>), grid, blk, 0, s, j, t); break;
+    case 0: hipLaunchKernelGGL((interp_kernel<P, S, 1, 1, 1>), grid, blk, lds, s, j, t, b); break;
+    case 1: hipLaunchKernelGGL((interp_kernel<P, S, 2, 1, 1>), grid, blk, lds, s, j, t, b); break;
+    case 2: hipLaunchKernelGGL((interp_kernel<P, S, 1, 2, 1>), grid, blk, lds, s, j, t, b); break;
+    case 3: hipLaunchKernelGGL((interp_kernel<P, S, 2, 2, 1>), grid, blk, lds, s, j, t, b); break;
+    case 4: hipLaunchKernelGGL((interp_kernel<P, S, 1, 1, 2>), grid, blk, lds, s, j, t, b); break;
+    case 5: hipLaunchKernelGGL((interp_kernel<P, S, 2, 1, 2>), grid, blk, lds, s, j, t, b); break;
+    case 6: hipLaunchKernelGGL((interp_kernel<P, S, 1, 2, 2>), grid, blk, lds, s, j, t, b); break;
+    default: hipLaunchKernelGGL((interp_kernel<P, S, 2, 2, 2>), grid, blk, lds, s, j, t, b); break;
   }
 }
 
 }  // namespace
 
-void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles) {
+void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles,
+                  const TileBox* boxes, uint32_t lds) {
   dim3 grid(ntiles), blk(kThreads);
   if (cls == kClsGeneric) {
-    hipLaunchKernelGGL(generic_kernel, grid, blk, 0, s, jobs, tiles);
+    hipLaunchKernelGGL(generic_kernel, grid, blk, 0, s, jobs, tiles, boxes);
   } else if (cls == kClsCopy) {
-    hipLaunchKernelGGL(copy_kernel, grid, blk, 0, s, jobs, tiles);
+    hipLaunchKernelGGL(copy_kernel, grid, blk, 0, s, jobs, tiles, boxes);
   } else {
     int id = cls - kClsInterp, ps = id / 8, rbits = id % 8;
     switch (ps) {
-      case 0: launch_interp_r<2, 0>(rbits, grid, s, jobs, tiles); break;
-      case 1: launch_interp_r<3, 0>(rbits, grid, s, jobs, tiles); break;
-      case 2: launch_interp_r<4, 0>(rbits, grid, s, jobs, tiles); break;
-      case 3: launch_interp_r<5, 0>(rbits, grid, s, jobs, tiles); break;
-      case 4: launch_interp_r<6, 0>(rbits, grid, s, jobs, tiles); break;
-      case 5: launch_interp_r<2, 2>(rbits, grid, s, jobs, tiles); break;
-      case 6: launch_interp_r<4, 2>(rbits, grid, s, jobs, tiles); break;
-      default: launch_interp_r<6, 2>(rbits, grid, s, jobs, tiles); break;
+      case 0: launch_interp_r<2, 0>(rbits, grid, s, jobs, tiles, boxes, lds); break;
+      case 1: launch_interp_r<3, 0>(rbits, grid, s, jobs, tiles, boxes, lds); break;
+      case 2: launch_interp_r<4, 0>(rbits, grid, s, jobs, tiles, boxes, lds); break;
+      case 3: launch_interp_r<5, 0>(rbits, grid, s, jobs, tiles, boxes, lds); break;
+      case 4: launch_interp_r<6, 0>(rbits, grid, s, jobs, tiles, boxes, lds); break;
+      case 5: launch_interp_r<2, 2>(rbits, grid, s, jobs, tiles, boxes, lds); break;
+      case 6: launch_interp_r<4, 2>(rbits, grid, s, jobs, tiles, boxes, lds); break;
+      default: launch_interp_r<6, 2>(rbits, grid, s, jobs, tiles, boxes, lds); break;
     }
   }
   HIPCHECK(hipGetLastError());
@@ -295,7 +297,8 @@ tah_status tah_fill(tah_plan* p, double* work, const double* factors, void* stre
     }
     for (const auto& sg : p->launches)
       for (const Launch& L : sg)
-        launch_class(L.cls, uint32_t(L.count), s, p->djob, p->dtiles + L.first);
+        launch_class(L.cls, uint32_t(L.count), s, p->djob, p->dtiles + L.first, p->dboxes + L.first,
+                     L.lds);
   });
 }
 

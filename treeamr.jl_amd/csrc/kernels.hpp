@@ -16,8 +16,6 @@
 
 namespace tah {
 
-constexpr int kThreads = 256;
-
 __device__ __forceinline__ double scaled(double acc, const double* __restrict__ factors,
                                          int32_t factorcol, int32_t nvars, int32_t v) {
   if (factorcol == 0) return acc;
@@ -84,8 +82,9 @@ constexpr int kGenericPerThread = 4;
 constexpr int kGenericTile = kThreads * kGenericPerThread;
 
 __global__ void __launch_bounds__(kThreads) generic_kernel(const Job* __restrict__ jobs,
-                                                           const Tile* __restrict__ tiles) {
-  const Tile tile = tiles[blockIdx.x];
+                                                           const Tile* __restrict__ tiles,
+                                                           const TileBox* __restrict__) {
+  const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const Job& job = jobs[tile.job];
   const DevGroup& g = job.groups[tile.group];
   const int64_t s1 = job.stride[1], s2 = job.stride[2], s3 = job.stride[3], s4 = job.stride[4];
@@ -117,42 +116,49 @@ constexpr int kCopyPerThread = 8;
 constexpr int kCopyTile = kThreads * kCopyPerThread;
 
 __global__ void __launch_bounds__(kThreads) copy_kernel(const Job* __restrict__ jobs,
-                                                        const Tile* __restrict__ tiles) {
-  const Tile tile = tiles[blockIdx.x];
+                                                        const Tile* __restrict__ tiles,
+                                                        const TileBox* __restrict__) {
+  const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const Job& job = jobs[tile.job];
   const DevGroup& g = job.groups[tile.group];
-  const int64_t s1 = job.stride[1], s2 = job.stride[2], s3 = job.stride[3], s4 = job.stride[4];
+  // Offsets within one block fit 32 bits (plans refuse larger blocks).
+  const int32_t s1 = int32_t(job.stride[1]), s2 = int32_t(job.stride[2]), s3 = int32_t(job.stride[3]);
+  const int64_t s4 = job.stride[4];
   const int32_t nvars = job.nvars;
   const uint32_t total = uint32_t(g.points) * uint32_t(nvars);
   const double* __restrict__ sbase = job.work + s4 * g.sblocks[tile.transfer];
   double* __restrict__ tbase = job.work + s4 * g.tblocks[tile.transfer];
-  int64_t dst[kCopyPerThread];
-  int32_t var[kCopyPerThread];
+  const int32_t tb0 = g.tfirst[0] + s1 * g.tfirst[1] + s2 * g.tfirst[2];
+  const int32_t sb0 = g.ca[0] + s1 * g.ca[1] + s2 * g.ca[2];
+  const int32_t ds0 = g.cs[0], ds1 = s1 * g.cs[1], ds2 = s2 * g.cs[2];
+  int32_t dst[kCopyPerThread];
   double val[kCopyPerThread];
 #pragma unroll
   for (int r = 0; r < kCopyPerThread; ++r) {
     uint32_t e = uint32_t(tile.e0) + uint32_t(r * kThreads) + threadIdx.x;
-    bool ok = e < total;
+    const bool ok = e < total;
     e = ok ? e : 0u;
-    uint32_t q = g.dn[0].div(e);
-    int32_t i = int32_t(e - q * uint32_t(g.n[0]));
-    uint32_t q2 = g.dn[1].div(q);
-    int32_t j = int32_t(q - q2 * uint32_t(g.n[1]));
-    uint32_t v32 = g.dn[2].div(q2);
-    int32_t k = int32_t(q2 - v32 * uint32_t(g.n[2]));
-    int32_t v = int32_t(v32);
-    int64_t src = s3 * v + (g.ca[0] + g.cs[0] * i) + s1 * (g.ca[1] + g.cs[1] * j) +
-                  s2 * (g.ca[2] + g.cs[2] * k);
-    dst[r] = ok ? s3 * v + (g.tfirst[0] + i) + s1 * (g.tfirst[1] + j) + s2 * (g.tfirst[2] + k) : -1;
-    var[r] = v;
-    val[r] = sbase[src];
+    const uint32_t q = g.dn[0].div(e);
+    const int32_t i = int32_t(e - q * uint32_t(g.n[0]));
+    const uint32_t q2 = g.dn[1].div(q);
+    const int32_t j = int32_t(q - q2 * uint32_t(g.n[1]));
+    const uint32_t v32 = g.dn[2].div(q2);
+    const int32_t k = int32_t(q2 - v32 * uint32_t(g.n[2]));
+    const int32_t vo = s3 * int32_t(v32);
+    dst[r] = ok ? vo + tb0 + i + s1 * j + s2 * k : -1;
+    val[r] = sbase[vo + sb0 + ds0 * i + ds1 * j + ds2 * k];
   }
+  if (g.factorcol == 0) {
 #pragma unroll
-  for (int r = 0; r < kCopyPerThread; ++r) {
-    if (dst[r] < 0) continue;
-    double acc = __dadd_rn(0.0, val[r]);
-    acc = scaled(acc, job.factors, g.factorcol, nvars, var[r]);
-    tbase[dst[r]] = acc;
+    for (int r = 0; r < kCopyPerThread; ++r)
+      if (dst[r] >= 0) tbase[dst[r]] = __dadd_rn(0.0, val[r]);
+  } else {
+#pragma unroll
+    for (int r = 0; r < kCopyPerThread; ++r) {
+      if (dst[r] < 0) continue;
+      const int32_t v = (dst[r] - tb0) / s3;
+      tbase[dst[r]] = scaled(__dadd_rn(0.0, val[r]), job.factors, g.factorcol, nvars, v);
+    }
   }
 }
 
@@ -166,19 +172,43 @@ __global__ void __launch_bounds__(kThreads) copy_kernel(const Job* __restrict__ 
 // the reference's order (m3, m2, m1 ascending) with its own weight product.
 template <int P, int SHIFT, int R0, int R1, int R2>
 __global__ void __launch_bounds__(kThreads) interp_kernel(const Job* __restrict__ jobs,
-                                                          const Tile* __restrict__ tiles) {
+                                                          const Tile* __restrict__ tiles,
+                                                          const TileBox* __restrict__ boxes) {
+  extern __shared__ double lds[];
   constexpr int U0 = P + (R0 == 2 ? SHIFT : 0);
   constexpr int U1 = P + (R1 == 2 ? SHIFT : 0);
   constexpr int U2 = P + (R2 == 2 ? SHIFT : 0);
-  const Tile tile = tiles[blockIdx.x];
+  const uint32_t ti = xcd_tile(blockIdx.x, gridDim.x);
+  const Tile tile = tiles[ti];
+  const TileBox box = boxes[ti];
   const Job& job = jobs[tile.job];
   const DevSub& sub = job.subs[tile.group];
   const DevGroup& g = job.groups[sub.group];
-  const int64_t s1 = job.stride[1], s2 = job.stride[2], s3 = job.stride[3], s4 = job.stride[4];
+  const int32_t s1 = int32_t(job.stride[1]), s2 = int32_t(job.stride[2]), s3 = int32_t(job.stride[3]);
+  const int64_t s4 = job.stride[4];
   const int32_t nvars = job.nvars;
-  const uint32_t total = uint32_t(sub.units) * uint32_t(nvars);
-  uint32_t e = uint32_t(tile.e0) + threadIdx.x;
-  if (e >= total) return;
+
+  // Stage the tile's source box in LDS, x fastest, coalesced rows.
+  const int32_t e0 = box.ext[0], e1 = box.ext[1], e2 = box.ext[2];
+  {
+    const double* __restrict__ sblk = job.work + s4 * g.sblocks[tile.transfer] +
+                                      (s3 * box.v0 + box.lo[0] + s1 * box.lo[1] + s2 * box.lo[2]);
+    const FastDiv dx{uint32_t(e0)}, dy{uint32_t(e1)}, dz{uint32_t(e2)};
+    const uint32_t n = uint32_t(e0 * e1 * e2 * box.nv);
+    for (uint32_t idx = threadIdx.x; idx < n; idx += kThreads) {
+      const uint32_t r = dx.div(idx);
+      const int32_t x = int32_t(idx - r * uint32_t(e0));
+      const uint32_t r2 = dy.div(r);
+      const int32_t y = int32_t(r - r2 * uint32_t(e1));
+      const uint32_t vv = dz.div(r2);
+      const int32_t z = int32_t(r2 - vv * uint32_t(e2));
+      lds[idx] = sblk[s3 * int32_t(vv) + x + s1 * y + s2 * z];
+    }
+  }
+  __syncthreads();
+  if (int32_t(threadIdx.x) >= box.count) return;
+
+  const uint32_t e = uint32_t(tile.e0) + threadIdx.x;
   const uint32_t v32 = sub.du.div(e);
   const uint32_t u = e - v32 * uint32_t(sub.units);
   const uint32_t q = sub.dc[0].div(u);
@@ -209,9 +239,10 @@ __global__ void __launch_bounds__(kThreads) interp_kernel(const Job* __restrict_
 #pragma unroll
     for (int m = 0; m < P; ++m) w2[c][m] = g.weights[2][P * rc[c] + m];
 
-  const double* __restrict__ src = job.work + s4 * g.sblocks[tile.transfer] + s3 * v +
-                                   g.srcstart[0][ra[0]] + s1 * g.srcstart[1][rb[0]] +
-                                   s2 * g.srcstart[2][rc[0]];
+  const double* __restrict__ src =
+      lds + (g.srcstart[0][ra[0]] - box.lo[0]) +
+      e0 * ((g.srcstart[1][rb[0]] - box.lo[1]) +
+            e1 * ((g.srcstart[2][rc[0]] - box.lo[2]) + e2 * (v - box.v0)));
   double acc[R0][R1][R2];
 #pragma unroll
   for (int a = 0; a < R0; ++a)
@@ -222,19 +253,25 @@ __global__ void __launch_bounds__(kThreads) interp_kernel(const Job* __restrict_
 
 #pragma unroll 1
   for (int m2 = 0; m2 < U2; ++m2) {
+    double c2[R2];
+#pragma unroll
+    for (int c = 0; c < R2; ++c) {
+      const int k2 = m2 - c * SHIFT;
+      double x = 0.0;
+#pragma unroll
+      for (int m = 0; m < P; ++m) x = (m == k2) ? w2[c][m] : x;
+      c2[c] = x;
+    }
 #pragma unroll
     for (int m1 = 0; m1 < U1; ++m1) {
       double row[U0];
-      const double* __restrict__ p = src + s2 * m2 + s1 * m1;
+      const double* __restrict__ p = src + e0 * (m1 + e1 * m2);
 #pragma unroll
       for (int m0 = 0; m0 < U0; ++m0) row[m0] = p[m0];
 #pragma unroll
       for (int c = 0; c < R2; ++c) {
         const int k2 = m2 - c * SHIFT;
         if (k2 < 0 || k2 >= P) continue;
-        double c2 = 0.0;
-#pragma unroll
-        for (int m = 0; m < P; ++m) c2 = (m == k2) ? w2[c][m] : c2;
 #pragma unroll
         for (int b = 0; b < R1; ++b) {
           const int k1 = m1 - b * SHIFT;
@@ -245,7 +282,7 @@ __global__ void __launch_bounds__(kThreads) interp_kernel(const Job* __restrict_
             for (int m0 = 0; m0 < U0; ++m0) {
               const int k0 = m0 - a * SHIFT;
               if (k0 < 0 || k0 >= P) continue;
-              double w = __dmul_rn(__dmul_rn(w0[a][k0], w1[b][k1]), c2);
+              const double w = __dmul_rn(__dmul_rn(w0[a][k0], w1[b][k1]), c2[c]);
               acc[a][b][c] = __dadd_rn(acc[a][b][c], __dmul_rn(w, row[m0]));
             }
           }

@@ -95,6 +95,78 @@ void classify(tah_plan* p) {
   }
 }
 
+// The tiles of one interpolation sub-box (the same for every transfer of its
+// group): each a run of units with the source box it stages in LDS. The
+// tile's first flat element rides in pad[0].
+std::vector<TileBox> interp_tile_shapes(const tah_plan* p, const PlanSub& s, int32_t nvars) {
+  const PlanGroup& g = p->groups[s.group];
+  const int id = s.cls - kClsInterp, rbits = id % 8;
+  const int P = kInterpTable[id / 8][0], shift = kInterpTable[id / 8][1];
+  int R[3], W[3];
+  for (int d = 0; d < 3; ++d) {
+    R[d] = ((rbits >> d) & 1) ? 2 : 1;
+    W[d] = P + (R[d] == 2 ? shift : 0);
+  }
+  const int64_t U = int64_t(s.cnt[0]) * s.cnt[1] * s.cnt[2];
+  auto start = [&](int d, int64_t ud) { return g.srcstart[d][s.rows[d][ud * R[d]]]; };
+  auto box_of = [&](int64_t u0, int64_t u1) {
+    TileBox b{};
+    int32_t lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-1, -1, -1};
+    for (int64_t u = u0; u < u1; ++u) {
+      int64_t ud[3] = {u % s.cnt[0], (u / s.cnt[0]) % s.cnt[1], u / (int64_t(s.cnt[0]) * s.cnt[1])};
+      for (int d = 0; d < 3; ++d) {
+        int32_t a = start(d, ud[d]);
+        lo[d] = std::min(lo[d], a);
+        hi[d] = std::max(hi[d], a + W[d]);
+      }
+    }
+    for (int d = 0; d < 3; ++d) {
+      b.lo[d] = lo[d];
+      b.ext[d] = hi[d] - lo[d];
+    }
+    return b;
+  };
+  auto vol = [](const TileBox& b) { return int64_t(b.ext[0]) * b.ext[1] * b.ext[2]; };
+  std::vector<TileBox> out;
+  TileBox full = box_of(0, U);
+  if (U <= kThreads && vol(full) <= kMaxBox) {
+    int64_t nvt = std::max<int64_t>(1, std::min<int64_t>({int64_t(nvars), kThreads / U, kMaxBox / vol(full)}));
+    for (int64_t v0 = 0; v0 < nvars; v0 += nvt) {
+      TileBox b = full;
+      b.v0 = int32_t(v0);
+      b.nv = int32_t(std::min<int64_t>(nvt, nvars - v0));
+      b.count = int32_t(b.nv * U);
+      b.pad[0] = int32_t(v0 * U);
+      out.push_back(b);
+    }
+    return out;
+  }
+  int64_t cs = kThreads;
+  std::vector<TileBox> chunks;
+  while (true) {
+    chunks.clear();
+    bool fits = true;
+    for (int64_t u0 = 0; u0 < U; u0 += cs) {
+      int64_t u1 = std::min(U, u0 + cs);
+      TileBox b = box_of(u0, u1);
+      b.count = int32_t(u1 - u0);
+      b.pad[0] = int32_t(u0);
+      if (vol(b) > kMaxBox) fits = false;
+      chunks.push_back(b);
+    }
+    if (fits || cs == 1) break;
+    cs /= 2;
+  }
+  for (int32_t v = 0; v < nvars; ++v)
+    for (TileBox b : chunks) {
+      b.v0 = v;
+      b.nv = 1;
+      b.pad[0] = int32_t(v * U) + b.pad[0];
+      out.push_back(b);
+    }
+  return out;
+}
+
 void build_plan_device(tah_plan* p) {
   classify(p);
   Blob blob;
@@ -168,9 +240,11 @@ void build_plan_device(tah_plan* p) {
   p->dsubs = reinterpret_cast<DevSub*>(base + off_subs);
   HIPCHECK(hipMemcpy(p->dtables, blob.host.data(), blob.host.size(), hipMemcpyHostToDevice));
 
-  // Tiles: per stage group, per class, in group then transfer order.
+  // Tiles: per stage group, per class; within a launch ordered by target
+  // block, so that the tiles of one block run close together in time.
   const int32_t nvars = p->layout.nvars;
   p->tiles.clear();
+  p->boxes.clear();
   p->launches.clear();
   for (const auto& sg : p->stagegroups) {
     std::vector<Launch> ls;
@@ -186,7 +260,13 @@ void build_plan_device(tah_plan* p) {
     std::sort(classes.begin(), classes.end());
     classes.erase(std::unique(classes.begin(), classes.end()), classes.end());
     for (int cls : classes) {
-      Launch L{cls, int64_t(p->tiles.size()), 0};
+      struct Entry {
+        int32_t block;
+        Tile t;
+        TileBox b;
+      };
+      std::vector<Entry> tl;
+      uint32_t lds = 0;
       int64_t per = class_tile_elems(cls);
       for (size_t gi = 0; gi < p->groups.size(); ++gi) {
         const PlanGroup& g = p->groups[gi];
@@ -195,17 +275,29 @@ void build_plan_device(tah_plan* p) {
           for (int32_t si : g.subs) {
             const PlanSub& s = p->subs[si];
             if (s.cls != cls) continue;
-            int64_t elems = int64_t(s.cnt[0]) * s.cnt[1] * s.cnt[2] * nvars;
+            std::vector<TileBox> shapes = interp_tile_shapes(p, s, nvars);
+            for (const TileBox& b : shapes)
+              lds = std::max<uint32_t>(lds, uint32_t(8 * int64_t(b.ext[0]) * b.ext[1] * b.ext[2] * b.nv));
             for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t)
-              for (int64_t e = 0; e < elems; e += per) p->tiles.push_back(Tile{0, si, t, int32_t(e)});
+              for (const TileBox& b : shapes)
+                tl.push_back(Entry{g.tblocks[t], Tile{0, si, t, b.pad[0]}, b});
           }
         } else if (g.cls == cls) {
           int64_t elems = int64_t(g.points) * nvars;
           for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t)
-            for (int64_t e = 0; e < elems; e += per) p->tiles.push_back(Tile{0, int32_t(gi), t, int32_t(e)});
+            for (int64_t e = 0; e < elems; e += per)
+              tl.push_back(Entry{g.tblocks[t], Tile{0, int32_t(gi), t, int32_t(e)}, TileBox{}});
         }
       }
-      L.count = int64_t(p->tiles.size()) - L.first;
+      std::stable_sort(tl.begin(), tl.end(),
+                       [](const Entry& x, const Entry& y) { return x.block < y.block; });
+      Launch L{cls, int64_t(p->tiles.size()), int64_t(tl.size()), lds};
+      for (const Entry& e : tl) {
+        p->tiles.push_back(e.t);
+        TileBox b = e.b;
+        b.pad[0] = 0;
+        p->boxes.push_back(b);
+      }
       if (L.count > 0) ls.push_back(L);
     }
     p->launches.push_back(std::move(ls));
@@ -213,6 +305,8 @@ void build_plan_device(tah_plan* p) {
   if (!p->tiles.empty()) {
     HIPCHECK(hipMalloc(&p->dtiles, sizeof(Tile) * p->tiles.size()));
     HIPCHECK(hipMemcpy(p->dtiles, p->tiles.data(), sizeof(Tile) * p->tiles.size(), hipMemcpyHostToDevice));
+    HIPCHECK(hipMalloc(&p->dboxes, sizeof(TileBox) * p->boxes.size()));
+    HIPCHECK(hipMemcpy(p->dboxes, p->boxes.data(), sizeof(TileBox) * p->boxes.size(), hipMemcpyHostToDevice));
   }
   HIPCHECK(hipMalloc(&p->djob, sizeof(Job)));
   std::memset(&p->last_job, 0, sizeof(Job));
@@ -235,5 +329,6 @@ Job plan_job(const tah_plan* p, double* work, const double* factors) {
 tah_plan::~tah_plan() {
   if (dtables) (void)hipFree(dtables);
   if (dtiles) (void)hipFree(dtiles);
+  if (dboxes) (void)hipFree(dboxes);
   if (djob) (void)hipFree(djob);
 }

@@ -31,8 +31,11 @@ constexpr int kClsGeneric = 0;
 constexpr int kClsCopy = 1;
 constexpr int kClsInterp = 2;  // + 8*(index into kInterpTable) + rbits
 
+// (P, SHIFT) of each specialised interpolation kernel family.
+extern const std::array<std::array<int, 2>, 8> kInterpTable;
 int interp_index(int P, int shift);
-void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles);
+void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles,
+                  const TileBox* boxes, uint32_t lds_bytes);
 int class_tile_elems(int cls);
 
 struct PlanGroup {
@@ -57,7 +60,11 @@ struct PlanSub {
 struct Launch {
   int cls;
   int64_t first, count;
+  uint32_t lds;  // dynamic LDS bytes per workgroup
 };
+
+// Largest LDS box an interpolation tile may stage, in doubles (64 KiB).
+constexpr int64_t kMaxBox = 8192;
 
 }  // namespace tah
 
@@ -78,7 +85,9 @@ struct tah_plan {
   tah::DevGroup* dgroups = nullptr;
   tah::DevSub* dsubs = nullptr;
   std::vector<tah::Tile> tiles;          // host copy, job field 0
+  std::vector<tah::TileBox> boxes;       // parallel to tiles (interpolation)
   tah::Tile* dtiles = nullptr;
+  tah::TileBox* dboxes = nullptr;
   tah::Job* djob = nullptr;
   tah::Job last_job{};
   std::vector<std::vector<tah::Launch>> launches;  // per stage group

@@ -48,6 +48,8 @@ DP = C.POINTER(C.c_double)
 
 # (name, argtypes) for every entry point of include/tah.h.
 SIGNATURES = {
+    "tah_lagrange_weights": [C.c_int32, C.c_int64, C.c_int64, DP],
+    "tah_conservative_weights": [C.c_int32, DP, DP],
     "tah_forest_create": [C.c_int32, I32P, C.c_int32, I32P, I32P, I32P, DP, C.POINTER(P)],
     "tah_forest_destroy": [P],
     "tah_forest_info": [P, I32P, C.POINTER(C.c_int64)],
