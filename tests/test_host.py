@@ -383,3 +383,20 @@ def test_oracle_adjoint_dot_product():
     lhs = float(fx @ y)
     rhs = float(x @ aty)
     assert abs(lhs - rhs) <= 1e-13 * max(abs(lhs), 1.0)
+
+
+def test_arena_layout_gives_the_same_values_as_the_dense_one():
+    """The variable-innermost arena holds the same field; the oracle's fill
+    over it equals the dense fill value for value."""
+    f, ops, kw, _ = W.config5(nvars=3, roots=2, N=16)
+    kw = dict(kw)
+    nv = kw.pop("nvars")
+    dense = FieldSet(f, nv, device="cpu", **kw)
+    arena = FieldSet(f, nv, device="cpu", layout="arena", **kw)
+    assert arena.layout.stride[3] == 1 and arena.layout.stride[0] == nv
+    W.fill_uniform(dense, 5)
+    arena.array()[...] = dense.array()
+    s = GhostSchedule(dense, ops)
+    oracle.fill(s._descptr, dense.layout, dense.work.numpy(), dense.factors.numpy())
+    oracle.fill(s._descptr, arena.layout, arena.work.numpy(), arena.factors.numpy())
+    assert np.array_equal(dense.array().numpy().view(np.int64), arena.array().numpy().view(np.int64))

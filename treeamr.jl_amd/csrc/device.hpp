@@ -51,47 +51,42 @@ struct alignas(16) DevGroup {
   const double* weights[3];
 };
 
-// A sub-box of an interpolating group in which every thread computes the
-// same shape of targets: R_d rows along dimension d (R_d = 2: a pair of
-// rows whose windows start SHIFT apart), rows[d] listing them unit by unit.
-struct alignas(16) DevSub {
+// Interpolation units (up to 2x2x2 targets of one transfer that share a
+// source window) whose rows carry one and the same set of weights, so that
+// every tap's weight product ((w1*w2)*w3) is one table, W, for the whole
+// class, laid out in the order the kernel visits the taps.
+struct alignas(16) DevWClass {
   int32_t group;
-  int32_t units;   // cnt0*cnt1*cnt2
-  int32_t cnt[3];
-  int32_t pad;
-  FastDiv dc[2];   // division by cnt0, cnt1
-  FastDiv du;      // division by units
-  const int32_t* rows[3];
+  int32_t nunits;
+  const int32_t* units;  // 8 per unit: x rows (2), y rows (2), z rows (2), -, -
+  const double* W;
 };
 
-// One fill of one working array.
+// One fill of one working array. Element (i, j, k, v) of block b is at
+// stride[0]*i + stride[1]*j + stride[2]*k + stride[3]*v + stride[4]*b.
 struct alignas(16) Job {
   const DevGroup* groups;
-  const DevSub* subs;
+  const DevWClass* wclasses;
   double* work;
   const double* factors;
   int64_t stride[5];
   int32_t nvars;
-  int32_t pad;
+  int32_t vfast;  // stride[3] < stride[0]: variables innermost
+  FastDiv dnv;    // division by nvars
 };
 
-// A slice of work: job, group (or sub-box), transfer, first flat element.
+// A slice of work. Copy and generic tiles: group `item`, transfer, flat
+// elements from e0. Interpolation tiles: class `item`, transfer, units
+// e0 .. e0+nu-1 times variables v0 .. v0+nvt-1.
 struct alignas(16) Tile {
   int32_t job;
-  int32_t group;
+  int32_t item;
   int32_t transfer;
   int32_t e0;
-};
-
-// What an interpolation tile stages in LDS: the source box lo..lo+ext-1
-// (stored indices) of variables v0..v0+nv-1, and how many of the tile's
-// flat elements it computes.
-struct alignas(16) TileBox {
-  int32_t lo[3];
-  int32_t ext[3];
-  int32_t v0, nv;
-  int32_t count;
-  int32_t pad[3];
+  int32_t nu;
+  int32_t v0;
+  int32_t nvt;
+  int32_t pad;
 };
 
 // Workgroups are dealt round-robin to the eight XCDs, each with its own L2:

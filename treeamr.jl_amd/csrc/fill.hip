@@ -33,41 +33,39 @@ int interp_index(int P, int shift) {
 namespace {
 
 template <int P, int S>
-void launch_interp_r(int rbits, dim3 grid, hipStream_t s, const Job* j, const Tile* t,
-                     const TileBox* b, uint32_t lds) {
+void launch_interp_r(int rbits, dim3 grid, hipStream_t s, const Job* j, const Tile* t) {
   dim3 blk(kThreads);
   switch (rbits) {
-    case 0: hipLaunchKernelGGL((interp_kernel<P, S, 1, 1, 1>), grid, blk, lds, s, j, t, b); break;
-    case 1: hipLaunchKernelGGL((interp_kernel<P, S, 2, 1, 1>), grid, blk, lds, s, j, t, b); break;
-    case 2: hipLaunchKernelGGL((interp_kernel<P, S, 1, 2, 1>), grid, blk, lds, s, j, t, b); break;
-    case 3: hipLaunchKernelGGL((interp_kernel<P, S, 2, 2, 1>), grid, blk, lds, s, j, t, b); break;
-    case 4: hipLaunchKernelGGL((interp_kernel<P, S, 1, 1, 2>), grid, blk, lds, s, j, t, b); break;
-    case 5: hipLaunchKernelGGL((interp_kernel<P, S, 2, 1, 2>), grid, blk, lds, s, j, t, b); break;
-    case 6: hipLaunchKernelGGL((interp_kernel<P, S, 1, 2, 2>), grid, blk, lds, s, j, t, b); break;
-    default: hipLaunchKernelGGL((interp_kernel<P, S, 2, 2, 2>), grid, blk, lds, s, j, t, b); break;
+    case 0: hipLaunchKernelGGL((interp_kernel<P, S, 1, 1, 1>), grid, blk, 0, s, j, t); break;
+    case 1: hipLaunchKernelGGL((interp_kernel<P, S, 2, 1, 1>), grid, blk, 0, s, j, t); break;
+    case 2: hipLaunchKernelGGL((interp_kernel<P, S, 1, 2, 1>), grid, blk, 0, s, j, t); break;
+    case 3: hipLaunchKernelGGL((interp_kernel<P, S, 2, 2, 1>), grid, blk, 0, s, j, t); break;
+    case 4: hipLaunchKernelGGL((interp_kernel<P, S, 1, 1, 2>), grid, blk, 0, s, j, t); break;
+    case 5: hipLaunchKernelGGL((interp_kernel<P, S, 2, 1, 2>), grid, blk, 0, s, j, t); break;
+    case 6: hipLaunchKernelGGL((interp_kernel<P, S, 1, 2, 2>), grid, blk, 0, s, j, t); break;
+    default: hipLaunchKernelGGL((interp_kernel<P, S, 2, 2, 2>), grid, blk, 0, s, j, t); break;
   }
 }
 
 }  // namespace
 
-void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles,
-                  const TileBox* boxes, uint32_t lds) {
+void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles) {
   dim3 grid(ntiles), blk(kThreads);
   if (cls == kClsGeneric) {
-    hipLaunchKernelGGL(generic_kernel, grid, blk, 0, s, jobs, tiles, boxes);
+    hipLaunchKernelGGL(generic_kernel, grid, blk, 0, s, jobs, tiles);
   } else if (cls == kClsCopy) {
-    hipLaunchKernelGGL(copy_kernel, grid, blk, 0, s, jobs, tiles, boxes);
+    hipLaunchKernelGGL(copy_kernel, grid, blk, 0, s, jobs, tiles);
   } else {
     int id = cls - kClsInterp, ps = id / 8, rbits = id % 8;
     switch (ps) {
-      case 0: launch_interp_r<2, 0>(rbits, grid, s, jobs, tiles, boxes, lds); break;
-      case 1: launch_interp_r<3, 0>(rbits, grid, s, jobs, tiles, boxes, lds); break;
-      case 2: launch_interp_r<4, 0>(rbits, grid, s, jobs, tiles, boxes, lds); break;
-      case 3: launch_interp_r<5, 0>(rbits, grid, s, jobs, tiles, boxes, lds); break;
-      case 4: launch_interp_r<6, 0>(rbits, grid, s, jobs, tiles, boxes, lds); break;
-      case 5: launch_interp_r<2, 2>(rbits, grid, s, jobs, tiles, boxes, lds); break;
-      case 6: launch_interp_r<4, 2>(rbits, grid, s, jobs, tiles, boxes, lds); break;
-      default: launch_interp_r<6, 2>(rbits, grid, s, jobs, tiles, boxes, lds); break;
+      case 0: launch_interp_r<2, 0>(rbits, grid, s, jobs, tiles); break;
+      case 1: launch_interp_r<3, 0>(rbits, grid, s, jobs, tiles); break;
+      case 2: launch_interp_r<4, 0>(rbits, grid, s, jobs, tiles); break;
+      case 3: launch_interp_r<5, 0>(rbits, grid, s, jobs, tiles); break;
+      case 4: launch_interp_r<6, 0>(rbits, grid, s, jobs, tiles); break;
+      case 5: launch_interp_r<2, 2>(rbits, grid, s, jobs, tiles); break;
+      case 6: launch_interp_r<4, 2>(rbits, grid, s, jobs, tiles); break;
+      default: launch_interp_r<6, 2>(rbits, grid, s, jobs, tiles); break;
     }
   }
   HIPCHECK(hipGetLastError());
@@ -95,10 +93,20 @@ void validate_layout(const tah_schedule_desc* d, const tah_layout* l) {
     if (l->stored[k] != d->stored[k])
       fail("the layout's stored size differs from the schedule's in dimension " + std::to_string(k + 1),
            kLayout);
-  if (l->stride[0] != 1) fail("stride[0] must be 1", kLayout);
-  if (l->stride[1] < l->stored[0] || l->stride[2] < l->stride[1] * l->stored[1] ||
-      l->stride[3] < l->stride[2] * l->stored[2] || l->stride[4] < l->stride[3] * l->nvars)
-    fail("the layout's strides overlap", kLayout);
+  // Strides positive and the five axes nested without overlap: sorted by
+  // stride, each at least the span of the ones inside it.
+  int64_t ext[5] = {l->stored[0], l->stored[1], l->stored[2], l->nvars, l->nblocks};
+  int order[5] = {0, 1, 2, 3, 4};
+  for (int k = 0; k < 5; ++k)
+    if (l->stride[k] < 1) fail("layout strides must be positive", kLayout);
+  std::sort(order, order + 5, [&](int a, int b) { return l->stride[a] < l->stride[b]; });
+  int64_t span = 1;
+  for (int k = 0; k < 5; ++k) {
+    if (l->stride[order[k]] < span) fail("the layout's strides overlap", kLayout);
+    span = l->stride[order[k]] * ext[order[k]];
+  }
+  if (l->stride[4] < l->stride[0] * l->stored[0] || l->stride[4] < l->stride[3] * l->nvars)
+    fail("blocks must be the outermost axis of the layout", kLayout);
   if (int64_t(l->stored[0]) * l->stored[1] * l->stored[2] * l->nvars > (int64_t(1) << 31))
     fail("a block larger than 2^31 values is not supported", kUnsupported);
 }
@@ -132,13 +140,14 @@ tah_status tah_layout_arena(const tah_schedule_desc* d, int32_t nvars, tah_layou
     for (int k = 0; k < 3; ++k) l.stored[k] = d->stored[k];
     l.nvars = nvars;
     l.nblocks = d->nblocks;
-    l.stride[0] = 1;
-    // Rows padded to a multiple of four doubles (32 bytes), planes and
-    // blocks to 256 bytes, so that every row starts on a sector.
-    l.stride[1] = d->dim >= 2 ? align_up(l.stored[0], 4) : l.stored[0];
+    // Variables innermost: a point's nvars values are contiguous, so every
+    // ghost slab row, however short along x, is one run of at least nvars
+    // doubles; blocks start on 256-byte boundaries, in Morton order.
+    l.stride[3] = 1;
+    l.stride[0] = nvars;
+    l.stride[1] = l.stride[0] * l.stored[0];
     l.stride[2] = l.stride[1] * l.stored[1];
-    l.stride[3] = align_up(l.stride[2] * l.stored[2], 32);
-    l.stride[4] = l.stride[3] * nvars;
+    l.stride[4] = align_up(l.stride[2] * l.stored[2], 32);
     *out = l;
   });
 }
@@ -274,8 +283,8 @@ tah_status tah_fill_plain(tah_plan* p, double* work, const double* factors, void
         if (total >= (uint64_t(1) << 32)) fail("a group larger than 2^32 values", kUnsupported);
         uint32_t nb = uint32_t((total + 255) / 256);
         hipLaunchKernelGGL(plain_kernel, dim3(nb), dim3(256), 0, s, p->devgroups[gi], work, factors,
-                           L.stride[1], L.stride[2], L.stride[3], L.stride[4], L.nvars, dv,
-                           uint32_t(total));
+                           L.stride[0], L.stride[1], L.stride[2], L.stride[3], L.stride[4], L.nvars,
+                           dv, uint32_t(total));
         HIPCHECK(hipGetLastError());
       }
     }
@@ -297,8 +306,7 @@ tah_status tah_fill(tah_plan* p, double* work, const double* factors, void* stre
     }
     for (const auto& sg : p->launches)
       for (const Launch& L : sg)
-        launch_class(L.cls, uint32_t(L.count), s, p->djob, p->dtiles + L.first, p->dboxes + L.first,
-                     L.lds);
+        launch_class(L.cls, uint32_t(L.count), s, p->djob, p->dtiles + L.first);
   });
 }
 

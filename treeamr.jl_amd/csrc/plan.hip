@@ -1,8 +1,10 @@
-// Plan construction: kernel classes, device tables, tiles.
+// Plan construction: kernel classes, weight-product tables, device tables,
+// tiles.
 #include "plan.hpp"
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 
 namespace tah {
 
@@ -21,21 +23,19 @@ struct Blob {
 };
 
 // Pair the target rows of one dimension whose source windows start `shift`
-// apart (in either row order); the rest stay single.
-void pair_rows(const std::vector<int32_t>& ss, int shift, std::vector<int32_t>& pairs,
-               std::vector<int32_t>& singles) {
+// apart (the lower window first); the rest stay single.
+void pair_rows(const std::vector<int32_t>& ss, int shift, std::vector<std::array<int32_t, 2>>& pairs,
+               std::vector<std::array<int32_t, 2>>& singles) {
   int n = int(ss.size());
   for (int i = 0; i < n;) {
     if (i + 1 < n && ss[i + 1] - ss[i] == shift) {
-      pairs.push_back(i);
-      pairs.push_back(i + 1);
+      pairs.push_back({i, i + 1});
       i += 2;
     } else if (i + 1 < n && shift > 0 && ss[i] - ss[i + 1] == shift) {
-      pairs.push_back(i + 1);
-      pairs.push_back(i);
+      pairs.push_back({i + 1, i});
       i += 2;
     } else {
-      singles.push_back(i);
+      singles.push_back({i, -1});
       i += 1;
     }
   }
@@ -56,12 +56,23 @@ bool affine_copy(const PlanGroup& g, int32_t ca[3], int32_t cs[3]) {
   return true;
 }
 
+// Bitwise key of a vector of doubles.
+std::vector<uint64_t> bits_of(const double* w, size_t n) {
+  std::vector<uint64_t> k(n);
+  std::memcpy(k.data(), w, n * sizeof(double));
+  return k;
+}
+
 }  // namespace
 
 void classify(tah_plan* p) {
+  std::map<std::vector<uint64_t>, int64_t> tables;  // W content -> offset in wblob
+  p->wblob.clear();
+  p->wclasses.clear();
   for (size_t gi = 0; gi < p->groups.size(); ++gi) {
     PlanGroup& g = p->groups[gi];
     g.cls = kClsGeneric;
+    g.wclasses.clear();
     int32_t ca[3], cs[3];
     if (g.points == 0) continue;
     if (affine_copy(g, ca, cs)) {
@@ -69,102 +80,96 @@ void classify(tah_plan* p) {
       continue;
     }
     if (g.kind == 1) continue;
-    int P = g.order[0];
+    const int P = g.order[0];
     if (g.order[1] != P || g.order[2] != P) continue;
-    int shift = g.kind == 2 ? 2 : 0;
-    int idx = interp_index(P, shift);
+    const int shift = g.kind == 2 ? 2 : 0;
+    const int idx = interp_index(P, shift);
     if (idx < 0) continue;
-    std::array<std::vector<int32_t>, 3> pairs, singles;
+    g.cls = kClsInterp;
+    std::array<std::vector<std::array<int32_t, 2>>, 3> pairs, singles;
     for (int d = 0; d < 3; ++d) pair_rows(g.srcstart[d], shift, pairs[d], singles[d]);
-    g.cls = kClsInterp;  // marks "split into subs"
     for (int rbits = 0; rbits < 8; ++rbits) {
-      PlanSub s;
-      s.group = int32_t(gi);
-      s.cls = kClsInterp + 8 * idx + rbits;
+      int R[3];
+      std::array<const std::vector<std::array<int32_t, 2>>*, 3> E;
       bool empty = false;
       for (int d = 0; d < 3; ++d) {
-        bool pr = (rbits >> d) & 1;
-        s.rows[d] = pr ? pairs[d] : singles[d];
-        s.cnt[d] = int32_t(s.rows[d].size()) / (pr ? 2 : 1);
-        if (s.cnt[d] == 0) empty = true;
+        R[d] = ((rbits >> d) & 1) ? 2 : 1;
+        E[d] = R[d] == 2 ? &pairs[d] : &singles[d];
+        empty = empty || E[d]->empty();
       }
       if (empty) continue;
-      g.subs.push_back(int32_t(p->subs.size()));
-      p->subs.push_back(std::move(s));
-    }
-  }
-}
-
-// The tiles of one interpolation sub-box (the same for every transfer of its
-// group): each a run of units with the source box it stages in LDS. The
-// tile's first flat element rides in pad[0].
-std::vector<TileBox> interp_tile_shapes(const tah_plan* p, const PlanSub& s, int32_t nvars) {
-  const PlanGroup& g = p->groups[s.group];
-  const int id = s.cls - kClsInterp, rbits = id % 8;
-  const int P = kInterpTable[id / 8][0], shift = kInterpTable[id / 8][1];
-  int R[3], W[3];
-  for (int d = 0; d < 3; ++d) {
-    R[d] = ((rbits >> d) & 1) ? 2 : 1;
-    W[d] = P + (R[d] == 2 ? shift : 0);
-  }
-  const int64_t U = int64_t(s.cnt[0]) * s.cnt[1] * s.cnt[2];
-  auto start = [&](int d, int64_t ud) { return g.srcstart[d][s.rows[d][ud * R[d]]]; };
-  auto box_of = [&](int64_t u0, int64_t u1) {
-    TileBox b{};
-    int32_t lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-1, -1, -1};
-    for (int64_t u = u0; u < u1; ++u) {
-      int64_t ud[3] = {u % s.cnt[0], (u / s.cnt[0]) % s.cnt[1], u / (int64_t(s.cnt[0]) * s.cnt[1])};
+      // Signature of every entry: the weights of its rows.
+      std::array<std::vector<int>, 3> sig;
       for (int d = 0; d < 3; ++d) {
-        int32_t a = start(d, ud[d]);
-        lo[d] = std::min(lo[d], a);
-        hi[d] = std::max(hi[d], a + W[d]);
+        std::map<std::vector<uint64_t>, int> ids;
+        for (const auto& e : *E[d]) {
+          std::vector<uint64_t> key;
+          for (int r = 0; r < R[d]; ++r) {
+            auto k = bits_of(&g.weights[d][size_t(P) * e[r]], P);
+            key.insert(key.end(), k.begin(), k.end());
+          }
+          auto it = ids.emplace(key, int(ids.size())).first;
+          sig[d].push_back(it->second);
+        }
       }
+      std::map<std::array<int, 3>, int32_t> byclass;
+      for (size_t ez = 0; ez < E[2]->size(); ++ez)
+        for (size_t ey = 0; ey < E[1]->size(); ++ey)
+          for (size_t ex = 0; ex < E[0]->size(); ++ex) {
+            std::array<int, 3> key{sig[0][ex], sig[1][ey], sig[2][ez]};
+            auto it = byclass.find(key);
+            if (it == byclass.end()) {
+              PlanWClass wc;
+              wc.group = int32_t(gi);
+              wc.cls = kClsInterp + 8 * idx + rbits;
+              // The weight products of this class, in the kernel's visit order.
+              const auto& ex0 = (*E[0])[ex];
+              const auto& ey0 = (*E[1])[ey];
+              const auto& ez0 = (*E[2])[ez];
+              const int U0 = P + (R[0] == 2 ? shift : 0), U1 = P + (R[1] == 2 ? shift : 0),
+                        U2 = P + (R[2] == 2 ? shift : 0);
+              auto w = [&](int d, int row, int k) { return g.weights[d][size_t(P) * row + k]; };
+              std::vector<double> W;
+              for (int m2 = 0; m2 < U2; ++m2)
+                for (int m1 = 0; m1 < U1; ++m1)
+                  for (int c = 0; c < R[2]; ++c) {
+                    int k2 = m2 - c * shift;
+                    if (k2 < 0 || k2 >= P) continue;
+                    for (int b = 0; b < R[1]; ++b) {
+                      int k1 = m1 - b * shift;
+                      if (k1 < 0 || k1 >= P) continue;
+                      for (int a = 0; a < R[0]; ++a)
+                        for (int m0 = 0; m0 < U0; ++m0) {
+                          int k0 = m0 - a * shift;
+                          if (k0 < 0 || k0 >= P) continue;
+                          double p01 = w(0, ex0[a], k0) * w(1, ey0[b], k1);
+                          W.push_back(p01 * w(2, ez0[c], k2));
+                        }
+                    }
+                  }
+              auto key2 = bits_of(W.data(), W.size());
+              auto t = tables.find(key2);
+              if (t == tables.end()) {
+                int64_t off = int64_t(p->wblob.size());
+                p->wblob.insert(p->wblob.end(), W.begin(), W.end());
+                // keep tables 256-byte aligned
+                while (p->wblob.size() % 32) p->wblob.push_back(0.0);
+                t = tables.emplace(key2, off).first;
+              }
+              wc.wtable = t->second;
+              it = byclass.emplace(key, int32_t(p->wclasses.size())).first;
+              g.wclasses.push_back(it->second);
+              p->wclasses.push_back(std::move(wc));
+            }
+            PlanWClass& wc = p->wclasses[it->second];
+            const auto& a = (*E[0])[ex];
+            const auto& b = (*E[1])[ey];
+            const auto& c = (*E[2])[ez];
+            int32_t u[8] = {a[0], a[1], b[0], b[1], c[0], c[1], 0, 0};
+            wc.units.insert(wc.units.end(), u, u + 8);
+          }
     }
-    for (int d = 0; d < 3; ++d) {
-      b.lo[d] = lo[d];
-      b.ext[d] = hi[d] - lo[d];
-    }
-    return b;
-  };
-  auto vol = [](const TileBox& b) { return int64_t(b.ext[0]) * b.ext[1] * b.ext[2]; };
-  std::vector<TileBox> out;
-  TileBox full = box_of(0, U);
-  if (U <= kThreads && vol(full) <= kMaxBox) {
-    int64_t nvt = std::max<int64_t>(1, std::min<int64_t>({int64_t(nvars), kThreads / U, kMaxBox / vol(full)}));
-    for (int64_t v0 = 0; v0 < nvars; v0 += nvt) {
-      TileBox b = full;
-      b.v0 = int32_t(v0);
-      b.nv = int32_t(std::min<int64_t>(nvt, nvars - v0));
-      b.count = int32_t(b.nv * U);
-      b.pad[0] = int32_t(v0 * U);
-      out.push_back(b);
-    }
-    return out;
   }
-  int64_t cs = kThreads;
-  std::vector<TileBox> chunks;
-  while (true) {
-    chunks.clear();
-    bool fits = true;
-    for (int64_t u0 = 0; u0 < U; u0 += cs) {
-      int64_t u1 = std::min(U, u0 + cs);
-      TileBox b = box_of(u0, u1);
-      b.count = int32_t(u1 - u0);
-      b.pad[0] = int32_t(u0);
-      if (vol(b) > kMaxBox) fits = false;
-      chunks.push_back(b);
-    }
-    if (fits || cs == 1) break;
-    cs /= 2;
-  }
-  for (int32_t v = 0; v < nvars; ++v)
-    for (TileBox b : chunks) {
-      b.v0 = v;
-      b.nv = 1;
-      b.pad[0] = int32_t(v * U) + b.pad[0];
-      out.push_back(b);
-    }
-  return out;
 }
 
 void build_plan_device(tah_plan* p) {
@@ -184,12 +189,12 @@ void build_plan_device(tah_plan* p) {
       o.w[d] = blob.put(g.weights[d].data(), 8 * int64_t(g.weights[d].size()));
     }
   }
-  std::vector<std::array<int64_t, 3>> subrows(p->subs.size());
-  for (size_t si = 0; si < p->subs.size(); ++si)
-    for (int d = 0; d < 3; ++d)
-      subrows[si][d] = blob.put(p->subs[si].rows[d].data(), 4 * int64_t(p->subs[si].rows[d].size()));
+  std::vector<int64_t> unit_off(p->wclasses.size());
+  for (size_t ci = 0; ci < p->wclasses.size(); ++ci)
+    unit_off[ci] = blob.put(p->wclasses[ci].units.data(), 4 * int64_t(p->wclasses[ci].units.size()));
+  int64_t off_w = blob.put(p->wblob.data(), 8 * int64_t(p->wblob.size()));
   int64_t off_groups = blob.put(nullptr, int64_t(sizeof(DevGroup)) * std::max<size_t>(1, p->groups.size()));
-  int64_t off_subs = blob.put(nullptr, int64_t(sizeof(DevSub)) * std::max<size_t>(1, p->subs.size()));
+  int64_t off_wc = blob.put(nullptr, int64_t(sizeof(DevWClass)) * std::max<size_t>(1, p->wclasses.size()));
   HIPCHECK(hipMalloc(&p->dtables, blob.host.size()));
   char* base = p->dtables;
   p->devgroups.clear();
@@ -220,31 +225,26 @@ void build_plan_device(tah_plan* p) {
   }
   if (!p->devgroups.empty())
     std::memcpy(blob.host.data() + off_groups, p->devgroups.data(), sizeof(DevGroup) * p->devgroups.size());
-  std::vector<DevSub> ds;
-  for (size_t si = 0; si < p->subs.size(); ++si) {
-    const PlanSub& s = p->subs[si];
-    DevSub d{};
-    d.group = s.group;
-    d.units = s.cnt[0] * s.cnt[1] * s.cnt[2];
-    for (int k = 0; k < 3; ++k) {
-      d.cnt[k] = s.cnt[k];
-      d.rows[k] = reinterpret_cast<const int32_t*>(base + subrows[si][k]);
-    }
-    d.dc[0] = FastDiv(uint32_t(s.cnt[0]));
-    d.dc[1] = FastDiv(uint32_t(s.cnt[1]));
-    d.du = FastDiv(uint32_t(d.units));
-    ds.push_back(d);
+  std::vector<DevWClass> dw;
+  for (size_t ci = 0; ci < p->wclasses.size(); ++ci) {
+    const PlanWClass& c = p->wclasses[ci];
+    DevWClass d{};
+    d.group = c.group;
+    d.nunits = c.nunits();
+    d.units = reinterpret_cast<const int32_t*>(base + unit_off[ci]);
+    d.W = reinterpret_cast<const double*>(base + off_w) + c.wtable;
+    dw.push_back(d);
   }
-  if (!ds.empty()) std::memcpy(blob.host.data() + off_subs, ds.data(), sizeof(DevSub) * ds.size());
+  if (!dw.empty()) std::memcpy(blob.host.data() + off_wc, dw.data(), sizeof(DevWClass) * dw.size());
   p->dgroups = reinterpret_cast<DevGroup*>(base + off_groups);
-  p->dsubs = reinterpret_cast<DevSub*>(base + off_subs);
+  p->dwclasses = reinterpret_cast<DevWClass*>(base + off_wc);
   HIPCHECK(hipMemcpy(p->dtables, blob.host.data(), blob.host.size(), hipMemcpyHostToDevice));
 
   // Tiles: per stage group, per class; within a launch ordered by target
   // block, so that the tiles of one block run close together in time.
   const int32_t nvars = p->layout.nvars;
+  const bool vfast = p->layout.stride[3] < p->layout.stride[0];
   p->tiles.clear();
-  p->boxes.clear();
   p->launches.clear();
   for (const auto& sg : p->stagegroups) {
     std::vector<Launch> ls;
@@ -252,7 +252,7 @@ void build_plan_device(tah_plan* p) {
     for (const PlanGroup& g : p->groups) {
       if (g.stage < sg[0] || g.stage > sg[1] || g.points == 0 || g.tblocks.empty()) continue;
       if (g.cls == kClsInterp) {
-        for (int32_t si : g.subs) classes.push_back(p->subs[si].cls);
+        for (int32_t ci : g.wclasses) classes.push_back(p->wclasses[ci].cls);
       } else {
         classes.push_back(g.cls);
       }
@@ -260,44 +260,43 @@ void build_plan_device(tah_plan* p) {
     std::sort(classes.begin(), classes.end());
     classes.erase(std::unique(classes.begin(), classes.end()), classes.end());
     for (int cls : classes) {
-      struct Entry {
-        int32_t block;
-        Tile t;
-        TileBox b;
-      };
-      std::vector<Entry> tl;
-      uint32_t lds = 0;
+      std::vector<std::pair<int32_t, Tile>> tl;
       int64_t per = class_tile_elems(cls);
       for (size_t gi = 0; gi < p->groups.size(); ++gi) {
         const PlanGroup& g = p->groups[gi];
         if (g.stage < sg[0] || g.stage > sg[1] || g.points == 0) continue;
         if (g.cls == kClsInterp) {
-          for (int32_t si : g.subs) {
-            const PlanSub& s = p->subs[si];
-            if (s.cls != cls) continue;
-            std::vector<TileBox> shapes = interp_tile_shapes(p, s, nvars);
-            for (const TileBox& b : shapes)
-              lds = std::max<uint32_t>(lds, uint32_t(8 * int64_t(b.ext[0]) * b.ext[1] * b.ext[2] * b.nv));
+          for (int32_t ci : g.wclasses) {
+            const PlanWClass& c = p->wclasses[ci];
+            if (c.cls != cls) continue;
+            const int32_t nun = c.nunits();
+            int32_t nu, nvt;
+            if (vfast) {
+              nvt = std::min(nvars, kThreads);
+              nu = std::max(1, kThreads / nvt);
+            } else {
+              nu = std::min(nun, kThreads);
+              nvt = std::max(1, std::min(nvars, kThreads / nu));
+            }
             for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t)
-              for (const TileBox& b : shapes)
-                tl.push_back(Entry{g.tblocks[t], Tile{0, si, t, b.pad[0]}, b});
+              for (int32_t v0 = 0; v0 < nvars; v0 += nvt)
+                for (int32_t u0 = 0; u0 < nun; u0 += nu)
+                  tl.push_back({g.tblocks[t], Tile{0, ci, t, u0, std::min(nu, nun - u0), v0,
+                                                   std::min(nvt, nvars - v0), 0}});
           }
         } else if (g.cls == cls) {
           int64_t elems = int64_t(g.points) * nvars;
           for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t)
             for (int64_t e = 0; e < elems; e += per)
-              tl.push_back(Entry{g.tblocks[t], Tile{0, int32_t(gi), t, int32_t(e)}, TileBox{}});
+              tl.push_back({g.tblocks[t], Tile{0, int32_t(gi), t, int32_t(e), 0, 0, 0, 0}});
         }
       }
       std::stable_sort(tl.begin(), tl.end(),
-                       [](const Entry& x, const Entry& y) { return x.block < y.block; });
-      Launch L{cls, int64_t(p->tiles.size()), int64_t(tl.size()), lds};
-      for (const Entry& e : tl) {
-        p->tiles.push_back(e.t);
-        TileBox b = e.b;
-        b.pad[0] = 0;
-        p->boxes.push_back(b);
-      }
+                       [](const std::pair<int32_t, Tile>& x, const std::pair<int32_t, Tile>& y) {
+                         return x.first < y.first;
+                       });
+      Launch L{cls, int64_t(p->tiles.size()), int64_t(tl.size())};
+      for (const auto& e : tl) p->tiles.push_back(e.second);
       if (L.count > 0) ls.push_back(L);
     }
     p->launches.push_back(std::move(ls));
@@ -305,8 +304,6 @@ void build_plan_device(tah_plan* p) {
   if (!p->tiles.empty()) {
     HIPCHECK(hipMalloc(&p->dtiles, sizeof(Tile) * p->tiles.size()));
     HIPCHECK(hipMemcpy(p->dtiles, p->tiles.data(), sizeof(Tile) * p->tiles.size(), hipMemcpyHostToDevice));
-    HIPCHECK(hipMalloc(&p->dboxes, sizeof(TileBox) * p->boxes.size()));
-    HIPCHECK(hipMemcpy(p->dboxes, p->boxes.data(), sizeof(TileBox) * p->boxes.size(), hipMemcpyHostToDevice));
   }
   HIPCHECK(hipMalloc(&p->djob, sizeof(Job)));
   std::memset(&p->last_job, 0, sizeof(Job));
@@ -316,11 +313,13 @@ Job plan_job(const tah_plan* p, double* work, const double* factors) {
   Job job;
   std::memset(&job, 0, sizeof(Job));
   job.groups = p->dgroups;
-  job.subs = p->dsubs;
+  job.wclasses = p->dwclasses;
   job.work = work;
   job.factors = factors;
   for (int k = 0; k < 5; ++k) job.stride[k] = p->layout.stride[k];
   job.nvars = p->layout.nvars;
+  job.vfast = p->layout.stride[3] < p->layout.stride[0] ? 1 : 0;
+  job.dnv = FastDiv(uint32_t(p->layout.nvars));
   return job;
 }
 
@@ -329,6 +328,5 @@ Job plan_job(const tah_plan* p, double* work, const double* factors) {
 tah_plan::~tah_plan() {
   if (dtables) (void)hipFree(dtables);
   if (dtiles) (void)hipFree(dtiles);
-  if (dboxes) (void)hipFree(dboxes);
   if (djob) (void)hipFree(djob);
 }

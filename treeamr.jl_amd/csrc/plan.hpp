@@ -34,8 +34,7 @@ constexpr int kClsInterp = 2;  // + 8*(index into kInterpTable) + rbits
 // (P, SHIFT) of each specialised interpolation kernel family.
 extern const std::array<std::array<int, 2>, 8> kInterpTable;
 int interp_index(int P, int shift);
-void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles,
-                  const TileBox* boxes, uint32_t lds_bytes);
+void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles);
 int class_tile_elems(int cls);
 
 struct PlanGroup {
@@ -46,25 +45,23 @@ struct PlanGroup {
   std::array<std::vector<int32_t>, 3> srcstart;
   std::array<std::vector<double>, 3> weights;
   int cls = kClsGeneric;
-  std::vector<int32_t> subs;  // indices into tah_plan::subs (interp classes)
+  std::vector<int32_t> wclasses;  // indices into tah_plan::wclasses
 };
 
-struct PlanSub {
+// Interpolation units of one group sharing one weight-product table.
+struct PlanWClass {
   int32_t group = 0;
-  int cls = 0;
-  int32_t cnt[3] = {0, 0, 0};
-  std::array<std::vector<int32_t>, 3> rows;
+  int cls = 0;                  // kernel class (P, SHIFT, rbits)
+  std::vector<int32_t> units;   // 8 ints per unit (see DevWClass)
+  int64_t wtable = 0;           // offset of its W in tah_plan::wblob (doubles)
+  int32_t nunits() const { return int32_t(units.size() / 8); }
 };
 
 // One launch: a class over a contiguous range of the plan's tiles.
 struct Launch {
   int cls;
   int64_t first, count;
-  uint32_t lds;  // dynamic LDS bytes per workgroup
 };
-
-// Largest LDS box an interpolation tile may stage, in doubles (64 KiB).
-constexpr int64_t kMaxBox = 8192;
 
 }  // namespace tah
 
@@ -76,18 +73,17 @@ struct tah_plan {
   std::vector<int32_t> tags;
   std::vector<std::array<int32_t, 2>> stagegroups;
   std::vector<tah::PlanGroup> groups;
-  std::vector<tah::PlanSub> subs;
+  std::vector<tah::PlanWClass> wclasses;
+  std::vector<double> wblob;  // every distinct weight-product table
   bool needs_factors = false;
   int64_t values = 0;
-  // Device memory: tables, DevGroup and DevSub arrays, tiles, one Job.
+  // Device memory: tables, DevGroup and DevWClass arrays, tiles, one Job.
   char* dtables = nullptr;
   std::vector<tah::DevGroup> devgroups;  // host copies (pointers into dtables)
   tah::DevGroup* dgroups = nullptr;
-  tah::DevSub* dsubs = nullptr;
+  tah::DevWClass* dwclasses = nullptr;
   std::vector<tah::Tile> tiles;          // host copy, job field 0
-  std::vector<tah::TileBox> boxes;       // parallel to tiles (interpolation)
   tah::Tile* dtiles = nullptr;
-  tah::TileBox* dboxes = nullptr;
   tah::Job* djob = nullptr;
   tah::Job last_job{};
   std::vector<std::vector<tah::Launch>> launches;  // per stage group

@@ -392,7 +392,7 @@ class FieldSet:
         s = self.layout.stride
         S = self.stored
         return self.work.as_strided((self.nblocks, self.nvars, S[2], S[1], S[0]),
-                                    (s[4], s[3], s[2], s[1], 1))
+                                    (s[4], s[3], s[2], s[1], s[0]))
 
     def own_slices(self):
         """Slices of the own range along (k, j, i)."""
