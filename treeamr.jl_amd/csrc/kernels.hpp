@@ -18,6 +18,17 @@
 
 namespace tah {
 
+// A value every lane holds alike, moved into scalar registers, so that what
+// is loaded through it is loaded by the scalar unit.
+__device__ __forceinline__ int32_t uniform(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+template <class T>
+__device__ __forceinline__ T* uniform(T* p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint64_t lo = uint32_t(__builtin_amdgcn_readfirstlane(uint32_t(v)));
+  const uint64_t hi = uint32_t(__builtin_amdgcn_readfirstlane(uint32_t(v >> 32)));
+  return reinterpret_cast<T*>((hi << 32) | lo);
+}
+
 __device__ __forceinline__ double scaled(double acc, const double* __restrict__ factors,
                                          int32_t factorcol, int32_t nvars, int32_t v) {
   if (factorcol == 0) return acc;
@@ -211,7 +222,9 @@ __global__ void __launch_bounds__(kThreads) interp_kernel(const Job* __restrict_
   const double* __restrict__ src = job.work + s4 * g.sblocks[tile.transfer] +
                                    (s3 * v + s0 * g.srcstart[0][ra[0]] + s1 * g.srcstart[1][rb[0]] +
                                     s2 * g.srcstart[2][rc[0]]);
-  const double* __restrict__ W = wc.W;
+  // The table is the same for every lane: read it through the constant
+  // address space, so that it arrives by scalar loads into SGPRs.
+  uint64_t wbase = reinterpret_cast<uint64_t>(uniform(wc.W));
   double acc[R0][R1][R2];
 #pragma unroll
   for (int a = 0; a < R0; ++a)
@@ -224,8 +237,16 @@ __global__ void __launch_bounds__(kThreads) interp_kernel(const Job* __restrict_
   for (int m2 = 0; m2 < U2; ++m2) {
 #pragma unroll
     for (int m1 = 0; m1 < U1; ++m1) {
+      // The table pointer passes through an empty asm per row, so that each
+      // row's weights are loaded (by the scalar unit) when the row is summed,
+      // not all hoisted to the top of the kernel.
+      asm volatile("" : "+s"(wbase));
+      const __attribute__((address_space(4))) double* __restrict__ W =
+          (const __attribute__((address_space(4))) double*)(wbase) + idx;
+      int widx = 0;
       double row[U0];
-      const double* __restrict__ p = src + (s2 * m2 + s1 * m1);
+      const __attribute__((address_space(1))) double* __restrict__ p =
+          (const __attribute__((address_space(1))) double*)(src + (s2 * m2 + s1 * m1));
 #pragma unroll
       for (int m0 = 0; m0 < U0; ++m0) row[m0] = p[s0 * m0];
 #pragma unroll
@@ -242,12 +263,22 @@ __global__ void __launch_bounds__(kThreads) interp_kernel(const Job* __restrict_
             for (int m0 = 0; m0 < U0; ++m0) {
               const int k0 = m0 - a * SHIFT;
               if (k0 < 0 || k0 >= P) continue;
-              acc[a][b][c] = __dadd_rn(acc[a][b][c], __dmul_rn(W[idx], row[m0]));
-              ++idx;
+              acc[a][b][c] = __dadd_rn(acc[a][b][c], __dmul_rn(W[widx], row[m0]));
+              ++widx;
             }
           }
         }
       }
+      idx += widx;
+      // Every accumulator is brought up to date row by row: without this
+      // the compiler defers whole chains to the end and keeps every row's
+      // source values and weights live until then, spilling them.
+#pragma unroll
+      for (int a = 0; a < R0; ++a)
+#pragma unroll
+        for (int b = 0; b < R1; ++b)
+#pragma unroll
+          for (int c = 0; c < R2; ++c) asm volatile("" : "+v"(acc[a][b][c]));
     }
   }
   double* __restrict__ dst = job.work + s4 * g.tblocks[tile.transfer] + s3 * v;
