@@ -304,9 +304,13 @@ tah_status tah_fill(tah_plan* p, double* work, const double* factors, void* stre
       p->last_job = job;
       HIPCHECK(hipMemcpyAsync(p->djob, &p->last_job, sizeof(Job), hipMemcpyHostToDevice, s));
     }
-    for (const auto& sg : p->launches)
-      for (const Launch& L : sg)
-        launch_class(L.cls, uint32_t(L.count), s, p->djob, p->dtiles + L.first);
+    if (p->events.empty())
+      for (int i = 0; i < 8; ++i) {
+        hipEvent_t e;
+        HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        p->events.push_back(e);
+      }
+    for (const auto& sg : p->launches) run_launches(p->ctx, sg, s, p->djob, p->dtiles, p->events.data());
   });
 }
 

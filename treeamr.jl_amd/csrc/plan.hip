@@ -240,17 +240,90 @@ void build_plan_device(tah_plan* p) {
   p->dwclasses = reinterpret_cast<DevWClass*>(base + off_wc);
   HIPCHECK(hipMemcpy(p->dtables, blob.host.data(), blob.host.size(), hipMemcpyHostToDevice));
 
+  // Which stage group each transfer runs in: its own, or the last one when
+  // no later stage reads what it writes nor writes what it reads, so that it
+  // can run beside the last group's work (most of phase 1 beside the
+  // prolongations).
+  const int nsg = int(p->stagegroups.size());
+  std::vector<int> sg_of_stage(std::max(1, p->nstages), 0);
+  for (int k = 0; k < nsg; ++k)
+    for (int st = p->stagegroups[k][0]; st <= p->stagegroups[k][1]; ++st) sg_of_stage[st] = k;
+  struct Box3 {
+    int lo[3], hi[3];
+  };
+  auto overlaps = [](const Box3& x, const Box3& y) {
+    for (int d = 0; d < 3; ++d)
+      if (std::max(x.lo[d], y.lo[d]) > std::min(x.hi[d], y.hi[d])) return false;
+    return true;
+  };
+  auto wbox = [](const PlanGroup& g) {
+    Box3 b;
+    for (int d = 0; d < 3; ++d) {
+      b.lo[d] = g.tfirst[d];
+      b.hi[d] = g.tfirst[d] + g.n[d] - 1;
+    }
+    return b;
+  };
+  auto rbox = [](const PlanGroup& g) {
+    Box3 b;
+    for (int d = 0; d < 3; ++d) {
+      b.lo[d] = *std::min_element(g.srcstart[d].begin(), g.srcstart[d].end());
+      b.hi[d] = *std::max_element(g.srcstart[d].begin(), g.srcstart[d].end()) + g.order[d] - 1;
+    }
+    return b;
+  };
+  std::vector<std::vector<int>> run_in(p->groups.size());
+  for (size_t gi = 0; gi < p->groups.size(); ++gi)
+    run_in[gi].assign(p->groups[gi].tblocks.size(), sg_of_stage[p->groups[gi].stage]);
+  p->deferred = 0;
+  {
+    std::map<int32_t, std::vector<Box3>> later_reads, later_writes;
+    for (int k = nsg - 1; k >= 0; --k) {
+      if (k < nsg - 1) {
+        for (size_t gi = 0; gi < p->groups.size(); ++gi) {
+          const PlanGroup& g = p->groups[gi];
+          if (g.points == 0 || sg_of_stage[g.stage] != k) continue;
+          const Box3 wb = wbox(g), rb = rbox(g);
+          for (size_t t = 0; t < g.tblocks.size(); ++t) {
+            bool ok = true;
+            auto ir = later_reads.find(g.tblocks[t]);
+            if (ir != later_reads.end())
+              for (const Box3& x : ir->second) ok = ok && !overlaps(x, wb);
+            auto iw = later_writes.find(g.sblocks[t]);
+            if (ok && iw != later_writes.end())
+              for (const Box3& x : iw->second) ok = ok && !overlaps(x, rb);
+            if (ok) {
+              run_in[gi][t] = nsg - 1;
+              ++p->deferred;
+            }
+          }
+        }
+      }
+      // Then this group's own accesses join the later ones.
+      for (size_t gi = 0; gi < p->groups.size(); ++gi) {
+        const PlanGroup& g = p->groups[gi];
+        if (g.points == 0 || sg_of_stage[g.stage] != k) continue;
+        const Box3 wb = wbox(g), rb = rbox(g);
+        for (size_t t = 0; t < g.tblocks.size(); ++t) {
+          later_writes[g.tblocks[t]].push_back(wb);
+          later_reads[g.sblocks[t]].push_back(rb);
+        }
+      }
+    }
+  }
+
   // Tiles: per stage group, per class; within a launch ordered by target
   // block, so that the tiles of one block run close together in time.
   const int32_t nvars = p->layout.nvars;
   const bool vfast = p->layout.stride[3] < p->layout.stride[0];
   p->tiles.clear();
   p->launches.clear();
-  for (const auto& sg : p->stagegroups) {
+  for (int k = 0; k < nsg; ++k) {
     std::vector<Launch> ls;
     std::vector<int> classes;
-    for (const PlanGroup& g : p->groups) {
-      if (g.stage < sg[0] || g.stage > sg[1] || g.points == 0 || g.tblocks.empty()) continue;
+    for (size_t gi = 0; gi < p->groups.size(); ++gi) {
+      const PlanGroup& g = p->groups[gi];
+      if (g.points == 0 || std::find(run_in[gi].begin(), run_in[gi].end(), k) == run_in[gi].end()) continue;
       if (g.cls == kClsInterp) {
         for (int32_t ci : g.wclasses) classes.push_back(p->wclasses[ci].cls);
       } else {
@@ -264,7 +337,7 @@ void build_plan_device(tah_plan* p) {
       int64_t per = class_tile_elems(cls);
       for (size_t gi = 0; gi < p->groups.size(); ++gi) {
         const PlanGroup& g = p->groups[gi];
-        if (g.stage < sg[0] || g.stage > sg[1] || g.points == 0) continue;
+        if (g.points == 0) continue;
         if (g.cls == kClsInterp) {
           for (int32_t ci : g.wclasses) {
             const PlanWClass& c = p->wclasses[ci];
@@ -278,17 +351,21 @@ void build_plan_device(tah_plan* p) {
               nu = std::min(nun, kThreads);
               nvt = std::max(1, std::min(nvars, kThreads / nu));
             }
-            for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t)
+            for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t) {
+              if (run_in[gi][t] != k) continue;
               for (int32_t v0 = 0; v0 < nvars; v0 += nvt)
                 for (int32_t u0 = 0; u0 < nun; u0 += nu)
                   tl.push_back({g.tblocks[t], Tile{0, ci, t, u0, std::min(nu, nun - u0), v0,
                                                    std::min(nvt, nvars - v0), 0}});
+            }
           }
         } else if (g.cls == cls) {
           int64_t elems = int64_t(g.points) * nvars;
-          for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t)
+          for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t) {
+            if (run_in[gi][t] != k) continue;
             for (int64_t e = 0; e < elems; e += per)
               tl.push_back({g.tblocks[t], Tile{0, int32_t(gi), t, int32_t(e), 0, 0, 0, 0}});
+          }
         }
       }
       std::stable_sort(tl.begin(), tl.end(),
@@ -299,6 +376,8 @@ void build_plan_device(tah_plan* p) {
       for (const auto& e : tl) p->tiles.push_back(e.second);
       if (L.count > 0) ls.push_back(L);
     }
+    // Largest launches first, so that they start first on their streams.
+    std::stable_sort(ls.begin(), ls.end(), [](const Launch& x, const Launch& y) { return x.count > y.count; });
     p->launches.push_back(std::move(ls));
   }
   if (!p->tiles.empty()) {
@@ -323,9 +402,45 @@ Job plan_job(const tah_plan* p, double* work, const double* factors) {
   return job;
 }
 
+const std::vector<hipStream_t>& ctx_streams(tah_ctx* c) {
+  if (c->streams.empty()) {
+    HIPCHECK(hipSetDevice(c->device));
+    for (int i = 0; i < 4; ++i) {
+      hipStream_t st;
+      HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      c->streams.push_back(st);
+    }
+  }
+  return c->streams;
+}
+
+void run_launches(tah_ctx* c, const std::vector<Launch>& ls, hipStream_t s, const Job* jobs,
+                  const Tile* tiles, hipEvent_t* ev) {
+  if (ls.size() <= 1) {
+    for (const Launch& L : ls) launch_class(L.cls, uint32_t(L.count), s, jobs, tiles + L.first);
+    return;
+  }
+  const auto& streams = ctx_streams(c);
+  HIPCHECK(hipEventRecord(ev[0], s));
+  for (size_t i = 0; i < ls.size(); ++i) {
+    hipStream_t si = streams[i % streams.size()];
+    if (i < streams.size()) HIPCHECK(hipStreamWaitEvent(si, ev[0], 0));
+    launch_class(ls[i].cls, uint32_t(ls[i].count), si, jobs, tiles + ls[i].first);
+  }
+  for (size_t i = 0; i < std::min(ls.size(), streams.size()); ++i) {
+    HIPCHECK(hipEventRecord(ev[1 + i], streams[i]));
+    HIPCHECK(hipStreamWaitEvent(s, ev[1 + i], 0));
+  }
+}
+
 }  // namespace tah
 
+tah_ctx::~tah_ctx() {
+  for (hipStream_t st : streams) (void)hipStreamDestroy(st);
+}
+
 tah_plan::~tah_plan() {
+  for (hipEvent_t e : events) (void)hipEventDestroy(e);
   if (dtables) (void)hipFree(dtables);
   if (dtiles) (void)hipFree(dtiles);
   if (djob) (void)hipFree(djob);

@@ -22,6 +22,11 @@
 struct tah_ctx {
   int device = 0;
   hipDeviceProp_t prop{};
+  // Streams the launches of one stage group are spread over, so that
+  // memory-bound copies and compute-bound interpolation overlap. Created
+  // on first use, default priority.
+  std::vector<hipStream_t> streams;
+  ~tah_ctx();
 };
 
 namespace tah {
@@ -87,6 +92,8 @@ struct tah_plan {
   tah::Job* djob = nullptr;
   tah::Job last_job{};
   std::vector<std::vector<tah::Launch>> launches;  // per stage group
+  int64_t deferred = 0;            // transfers moved to run with the last stage group
+  std::vector<hipEvent_t> events;  // fork/join events, created on first use
   ~tah_plan();
 };
 
@@ -95,4 +102,10 @@ namespace tah {
 void build_plan_device(tah_plan* p);
 // The Job of one fill of `work` by plan p.
 Job plan_job(const tah_plan* p, double* work, const double* factors);
+// Streams of the context (created on first use).
+const std::vector<hipStream_t>& ctx_streams(tah_ctx* c);
+// Run a stage group's launches forked over the context's streams and
+// joined back into `s`. ev must hold launches.size() + 1 events.
+void run_launches(tah_ctx* c, const std::vector<Launch>& ls, hipStream_t s, const Job* jobs,
+                  const Tile* tiles, hipEvent_t* ev);
 }  // namespace tah
