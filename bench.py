@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nvars", type=int, default=20)
-    ap.add_argument("--layout", default="dense", choices=["dense", "arena"])
+    ap.add_argument("--layout", default="arena", choices=["dense", "arena"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()

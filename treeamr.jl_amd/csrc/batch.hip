@@ -1,40 +1,193 @@
-// Batched fills and graph capture (stubs until implemented).
+// Many independent fills in one launch per kernel class and stage position,
+// and the same captured once in a hipGraph.
+//
+// Every plan brings its tiles with its own job (working array, factor table,
+// strides); position k runs the k-th stage group of every plan -- the plans
+// share nothing, so any alignment of their stages is a valid order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+
 #include "api_common.hpp"
 #include "plan.hpp"
 
 using namespace tah;
 
+namespace tah {
+
+struct Batch {
+  std::vector<tah_plan*> plans;
+  std::vector<double*> works;
+  std::vector<const double*> factors;
+  std::vector<Job> jobs;
+  Job* djobs = nullptr;
+  Tile* dtiles = nullptr;
+  std::vector<std::vector<Launch>> launches;  // per stage position
+  std::vector<hipEvent_t> events;
+  int device = 0;
+  ~Batch() {
+    if (djobs) (void)hipFree(djobs);
+    if (dtiles) (void)hipFree(dtiles);
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+  }
+};
+
+namespace {
+
+std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plans, double* const* works,
+                                  const double* const* factors) {
+  if (!ctx || n < 1 || !plans || !works) fail("null argument");
+  auto b = std::make_unique<Batch>();
+  b->device = ctx->device;
+  size_t npos = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    tah_plan* p = plans[i];
+    if (!p || !works[i]) fail("null plan or working array in the batch");
+    if (p->ctx->device != ctx->device) fail("every plan of a batch must be on the context's device");
+    const double* f = factors ? factors[i] : nullptr;
+    if (p->needs_factors && !f)
+      fail("plan " + std::to_string(i) + " mirrors ghosts across a reflecting face, but no factor table was given");
+    b->plans.push_back(p);
+    b->works.push_back(works[i]);
+    b->factors.push_back(f);
+    Job j = plan_job(p, works[i], f);
+    b->jobs.push_back(j);
+    npos = std::max(npos, p->launches.size());
+  }
+  for (size_t i = 0; i < b->works.size(); ++i)
+    for (size_t k = i + 1; k < b->works.size(); ++k)
+      if (b->works[i] == b->works[k]) fail("two fills of one batch share a working array");
+  std::vector<Tile> tiles;
+  for (size_t pos = 0; pos < npos; ++pos) {
+    std::map<int, std::vector<Tile>> bycls;
+    for (int32_t i = 0; i < n; ++i) {
+      tah_plan* p = b->plans[i];
+      if (pos >= p->launches.size()) continue;
+      for (const Launch& L : p->launches[pos])
+        for (int64_t t = L.first; t < L.first + L.count; ++t) {
+          Tile tl = p->tiles[t];
+          tl.job = i;
+          bycls[L.cls].push_back(tl);
+        }
+    }
+    std::vector<Launch> ls;
+    for (auto& kv : bycls) {
+      ls.push_back(Launch{kv.first, int64_t(tiles.size()), int64_t(kv.second.size())});
+      tiles.insert(tiles.end(), kv.second.begin(), kv.second.end());
+    }
+    std::stable_sort(ls.begin(), ls.end(), [](const Launch& x, const Launch& y) { return x.count > y.count; });
+    b->launches.push_back(std::move(ls));
+  }
+  HIPCHECK(hipSetDevice(ctx->device));
+  HIPCHECK(hipMalloc(&b->djobs, sizeof(Job) * b->jobs.size()));
+  HIPCHECK(hipMemcpy(b->djobs, b->jobs.data(), sizeof(Job) * b->jobs.size(), hipMemcpyHostToDevice));
+  if (!tiles.empty()) {
+    HIPCHECK(hipMalloc(&b->dtiles, sizeof(Tile) * tiles.size()));
+    HIPCHECK(hipMemcpy(b->dtiles, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice));
+  }
+  for (int i = 0; i < 8; ++i) {
+    hipEvent_t e;
+    HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b->events.push_back(e);
+  }
+  return b;
+}
+
+void run_batch(tah_ctx* ctx, Batch& b, hipStream_t s) {
+  for (const auto& ls : b.launches) run_launches(ctx, ls, s, b.djobs, b.dtiles, b.events.data());
+}
+
+bool same_batch(const Batch& b, int32_t n, tah_plan* const* plans, double* const* works,
+                const double* const* factors) {
+  if (int32_t(b.plans.size()) != n) return false;
+  for (int32_t i = 0; i < n; ++i)
+    if (b.plans[i] != plans[i] || b.works[i] != works[i] || b.factors[i] != (factors ? factors[i] : nullptr))
+      return false;
+  return true;
+}
+
+}  // namespace
+}  // namespace tah
+
+struct tah_graph {
+  tah_ctx* ctx = nullptr;
+  std::unique_ptr<Batch> batch;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~tah_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+namespace {
+// The last batch of each context, reused while the same fills are asked for.
+std::map<tah_ctx*, std::unique_ptr<Batch>> last_batch;
+}  // namespace
+
 extern "C" {
 
 tah_status tah_fill_batched(tah_ctx* ctx, int32_t n, tah_plan* const* plans, double* const* works,
                             const double* const* factors, void* stream) {
-  (void)ctx; (void)n; (void)plans; (void)works; (void)factors; (void)stream;
-  set_last_error("tah_fill_batched is not implemented yet");
-  return TAH_EUNSUPPORTED;
+  return guard([&] {
+    if (!ctx) fail("null context");
+    HIPCHECK(hipSetDevice(ctx->device));
+    auto& slot = last_batch[ctx];
+    if (!slot || !same_batch(*slot, n, plans, works, factors)) {
+      HIPCHECK(hipDeviceSynchronize());  // the previous batch's tables may still be in use
+      slot = make_batch(ctx, n, plans, works, factors);
+    }
+    run_batch(ctx, *slot, reinterpret_cast<hipStream_t>(stream));
+  });
 }
 
 tah_status tah_graph_create(tah_ctx* ctx, int32_t n, tah_plan* const* plans, double* const* works,
                             const double* const* factors, tah_graph** out) {
-  (void)ctx; (void)n; (void)plans; (void)works; (void)factors; (void)out;
-  set_last_error("tah_graph_create is not implemented yet");
-  return TAH_EUNSUPPORTED;
+  return guard([&] {
+    if (!out) fail("null argument");
+    auto g = std::make_unique<tah_graph>();
+    g->ctx = ctx;
+    g->batch = make_batch(ctx, n, plans, works, factors);
+    ctx_streams(ctx);  // created outside the capture
+    hipStream_t cs;
+    HIPCHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      (void)hipStreamDestroy(cs);
+      HIPCHECK(e);
+    }
+    try {
+      run_batch(ctx, *g->batch, cs);
+    } catch (...) {
+      hipGraph_t dropped = nullptr;
+      (void)hipStreamEndCapture(cs, &dropped);
+      if (dropped) (void)hipGraphDestroy(dropped);
+      (void)hipStreamDestroy(cs);
+      throw;
+    }
+    e = hipStreamEndCapture(cs, &g->graph);
+    (void)hipStreamDestroy(cs);
+    HIPCHECK(e);
+    HIPCHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    *out = g.release();
+  });
 }
 
 tah_status tah_graph_launch(tah_graph* g, void* stream) {
-  (void)g; (void)stream;
-  set_last_error("tah_graph_launch is not implemented yet");
-  return TAH_EUNSUPPORTED;
+  return guard([&] {
+    if (!g) fail("null graph");
+    HIPCHECK(hipSetDevice(g->ctx->device));
+    HIPCHECK(hipGraphLaunch(g->exec, reinterpret_cast<hipStream_t>(stream)));
+  });
 }
 
 tah_status tah_graph_destroy(tah_graph* g) {
-  (void)g;
+  delete g;
   return TAH_OK;
-}
-
-tah_status tah_fill_adjoint(tah_plan* p, double* adj, const double* factors, void* stream) {
-  (void)p; (void)adj; (void)factors; (void)stream;
-  set_last_error("tah_fill_adjoint is not implemented yet");
-  return TAH_EUNSUPPORTED;
 }
 
 tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char* cache_path) {

@@ -12,6 +12,7 @@
 #include "api_common.hpp"
 #include "device.hpp"
 #include "forest.hpp"
+#define TAH_DEFINE_KERNELS
 #include "kernels.hpp"
 #include "plan.hpp"
 

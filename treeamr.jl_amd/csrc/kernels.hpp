@@ -67,6 +67,7 @@ __device__ __forceinline__ double stencil_value(const DevGroup& g, const double*
 // The plain fill: one launch per group, one thread per written value, the
 // flat index split into (i, j, k, v, t) with i fastest (launch_transfer!'s
 // flat form, src/ghosts.jl:507-521).
+#ifdef TAH_DEFINE_KERNELS
 __global__ void __launch_bounds__(256) plain_kernel(DevGroup g, double* __restrict__ work,
                                                     const double* __restrict__ factors, int64_t s0,
                                                     int64_t s1, int64_t s2, int64_t s3, int64_t s4,
@@ -87,6 +88,8 @@ __global__ void __launch_bounds__(256) plain_kernel(DevGroup g, double* __restri
   double* tb = work + s4 * g.tblocks[t] + s3 * v;
   tb[s0 * (g.tfirst[0] + i) + s1 * (g.tfirst[1] + j) + s2 * (g.tfirst[2] + k)] = acc;
 }
+
+#endif  // TAH_DEFINE_KERNELS
 
 // Split a flat element of a (box x nvars) space: variables fastest when
 // they are innermost in memory, else x fastest.
@@ -117,6 +120,7 @@ __device__ __forceinline__ void split_element(const DevGroup& g, const Job& job,
 constexpr int kGenericPerThread = 4;
 constexpr int kGenericTile = kThreads * kGenericPerThread;
 
+#ifdef TAH_DEFINE_KERNELS
 __global__ void __launch_bounds__(kThreads) generic_kernel(const Job* __restrict__ jobs,
                                                            const Tile* __restrict__ tiles) {
   const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
@@ -139,6 +143,8 @@ __global__ void __launch_bounds__(kThreads) generic_kernel(const Job* __restrict
   }
 }
 
+#endif  // TAH_DEFINE_KERNELS
+
 // ---------------------------------------------------------------------------
 // Copies (unit stencils: same-level copies, mirrored and clamped rows): the
 // source row of target row r along d is ca[d] + cs[d]*r, so no index table
@@ -146,6 +152,7 @@ __global__ void __launch_bounds__(kThreads) generic_kernel(const Job* __restrict
 constexpr int kCopyPerThread = 8;
 constexpr int kCopyTile = kThreads * kCopyPerThread;
 
+#ifdef TAH_DEFINE_KERNELS
 __global__ void __launch_bounds__(kThreads) copy_kernel(const Job* __restrict__ jobs,
                                                         const Tile* __restrict__ tiles) {
   const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
@@ -182,6 +189,8 @@ __global__ void __launch_bounds__(kThreads) copy_kernel(const Job* __restrict__ 
     tbase[dst[r]] = scaled(__dadd_rn(0.0, val[r]), job.factors, g.factorcol, nvars, var[r]);
   }
 }
+
+#endif  // TAH_DEFINE_KERNELS
 
 // ---------------------------------------------------------------------------
 // Interpolation (restriction and prolongation) of order P in every

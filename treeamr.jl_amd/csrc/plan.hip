@@ -439,7 +439,12 @@ tah_ctx::~tah_ctx() {
   for (hipStream_t st : streams) (void)hipStreamDestroy(st);
 }
 
+namespace tah {
+void free_adjoint(tah_plan* p);
+}
+
 tah_plan::~tah_plan() {
+  tah::free_adjoint(this);
   for (hipEvent_t e : events) (void)hipEventDestroy(e);
   if (dtables) (void)hipFree(dtables);
   if (dtiles) (void)hipFree(dtiles);

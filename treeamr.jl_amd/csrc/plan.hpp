@@ -68,6 +68,8 @@ struct Launch {
   int64_t first, count;
 };
 
+struct AdjointTables;
+
 }  // namespace tah
 
 struct tah_plan {
@@ -94,6 +96,8 @@ struct tah_plan {
   std::vector<std::vector<tah::Launch>> launches;  // per stage group
   int64_t deferred = 0;            // transfers moved to run with the last stage group
   std::vector<hipEvent_t> events;  // fork/join events, created on first use
+  // The adjoint's inverted schedule, built on first use (adjoint.hip).
+  tah::AdjointTables* adj = nullptr;
   ~tah_plan();
 };
 

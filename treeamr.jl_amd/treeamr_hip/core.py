@@ -472,3 +472,51 @@ def fill_ghosts_adjoint(fs, schedule, adj, *, ctx=None):
     fptr = C.c_void_p(fs.factors.data_ptr()) if fs.factors is not None else None
     check(lib().tah_fill_adjoint(plan._h, C.c_void_p(adj.data_ptr()), fptr, _stream_ptr(fs)))
     return adj
+
+
+def _batch_args(pairs, ctx):
+    ctx = ctx or default_context()
+    n = len(pairs)
+    plans = (C.c_void_p * n)()
+    works = (C.c_void_p * n)()
+    facs = (C.c_void_p * n)()
+    keep = []
+    for i, (fs, sched) in enumerate(pairs):
+        _check_fill(fs, sched)
+        if fs.work.device.type != "cuda":
+            raise RuntimeError("batched fills run on the GPU only")
+        pl = sched.plan(fs.layout, ctx)
+        keep.append(pl)
+        plans[i] = pl._h
+        works[i] = fs.work.data_ptr()
+        facs[i] = fs.factors.data_ptr() if fs.factors is not None else None
+    return ctx, n, plans, works, facs, keep
+
+
+def fill_ghosts_batched(pairs, *, ctx=None):
+    """Fill many independent field sets ((fs, schedule) pairs, distinct working
+    arrays) with one launch per kernel class and stage position."""
+    ctx, n, plans, works, facs, keep = _batch_args(pairs, ctx)
+    check(lib().tah_fill_batched(ctx._h, n, plans, works, facs, _stream_ptr(pairs[0][0])))
+    return pairs
+
+
+class FillGraph:
+    """A batched fill captured once in a hipGraph and replayed by launch().
+    The field sets' working arrays must stay where they are."""
+
+    def __init__(self, pairs, *, ctx=None):
+        ctx, n, plans, works, facs, keep = _batch_args(pairs, ctx)
+        self._keep = (pairs, keep)
+        h = C.c_void_p()
+        check(lib().tah_graph_create(ctx._h, n, plans, works, facs, C.byref(h)))
+        self._h = h
+        self._stream_of = pairs[0][0]
+
+    def launch(self):
+        check(lib().tah_graph_launch(self._h, _stream_ptr(self._stream_of)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _abi is not None and _abi._lib is not None:
+            _abi._lib.tah_graph_destroy(h)
