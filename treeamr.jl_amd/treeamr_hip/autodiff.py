@@ -58,10 +58,20 @@ def fill(fs, schedule, work):
 
 
 def view(fs, work):
-    """work as [block, var, k, j, i]."""
+    """work as [block, var, k, j, i], through reshape and permute where the
+    layout allows (cheap under autograd), else through as_strided."""
     s = fs.layout.stride
     S = fs.stored
-    return work.as_strided((fs.nblocks, fs.nvars, S[2], S[1], S[0]), (s[4], s[3], s[2], s[1], s[0]))
+    nb, nv = fs.nblocks, fs.nvars
+    dense = (s[0] == 1 and s[1] == S[0] and s[2] == S[0] * S[1] and s[3] == S[0] * S[1] * S[2]
+             and s[4] == s[3] * nv)
+    vin = (s[3] == 1 and s[0] == nv and s[1] == nv * S[0] and s[2] == nv * S[0] * S[1]
+           and s[4] == s[2] * S[2])
+    if dense:
+        return work.reshape(nb, nv, S[2], S[1], S[0])
+    if vin:
+        return work.reshape(nb, S[2], S[1], S[0], nv).permute(0, 4, 1, 2, 3)
+    return work.as_strided((nb, nv, S[2], S[1], S[0]), (s[4], s[3], s[2], s[1], s[0]))
 
 
 def spacings(fs):
@@ -93,9 +103,17 @@ def own_mask(fs):
 def scatter_own(fs, own_values):
     """A working array holding own_values [b, v, N, N, N] at own points, 0 elsewhere
     (scatter!, src/state.jl:136-140)."""
+    G, N, S = fs.G, fs.forest.N, fs.stored
+    pad = []
+    for d in range(3):  # F.pad takes the last dimension first: i, j, k
+        pad += [G[d], S[d] - G[d] - N - fs.centering[d]]
+    full = torch.nn.functional.pad(own_values, pad)  # [b, v, k, j, i]
+    s = fs.layout.stride
+    dense = s[0] == 1 and s[4] == s[3] * fs.nvars and s[3] == S[0] * S[1] * S[2]
+    if dense:
+        return full.reshape(-1)
     w = torch.zeros_like(fs.work)
-    vw = view(fs, w)
-    vw[(slice(None), slice(None)) + fs.own_slices()] = own_values
+    view(fs, w)[...] = full
     return w
 
 
