@@ -146,10 +146,12 @@ def wave_rk4(fs, schedule, u, v, dt, steps, checkpoint_every=5):
             return uu, vv
         return run
 
+    if not checkpoint_every or checkpoint_every <= 0:
+        return segment(steps)(u, v)
     done = 0
     while done < steps:
         n = min(checkpoint_every, steps - done)
-        if checkpoint_every and torch.is_grad_enabled():
+        if torch.is_grad_enabled():
             u, v = cp.checkpoint(segment(n), u, v, use_reentrant=False)
         else:
             u, v = segment(n)(u, v)
