@@ -10,9 +10,11 @@ A step is one full ghost fill of that field set, inputs resident in HBM.
 Effective bytes per fill = 16 x (values written), the reference's accounting
 (bench/copies.jl:115-125): one 8-byte read and one 8-byte write per value.
 
-Reported: the staged fill (headline), the plain per-group fill beside it,
-the roofline figure of the staged kernel, and the restated reference CPU fill
-(the oracle) on a bounded sample, on this host's cores.
+Reported: the graph-captured staged fill (headline), the staged fill launched
+eagerly, the plain per-group fill, the adjoint fill, the 64-forest batch
+(sequential plain fills against one graph-captured batched fill), the
+roofline figure of the fill, and the restated reference CPU fill (the oracle)
+on a bounded sample, on this host's cores.
 """
 import argparse
 import ctypes as C
@@ -94,6 +96,9 @@ def main():
     ap.add_argument("--layout", default="arena", choices=["dense", "arena"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path "
@@ -101,7 +106,7 @@ def main():
         sys.exit(2)
 
     import torch
-    from treeamr_hip import FieldSet, GhostSchedule, fill_ghosts
+    from treeamr_hip import FieldSet, FillGraph, GhostSchedule, fill_ghosts, fill_ghosts_adjoint
     from treeamr_hip import workloads as W
 
     def builder(nvars):
@@ -124,23 +129,70 @@ def main():
         f"setup {time.perf_counter() - t0:.1f}s")
     stream = torch.cuda.current_stream()
 
+    def gbs_of(ms):
+        return bytes_eff / (ms * 1e-3) / 1e9
+
     plain = None
     if not args.no_plain:
-        log("[bench] timing plain per-group fill")
+        log("[bench] timing the plain per-group fill")
         plain = time_fill(lambda: fill_ghosts(fs, sched, method="plain"), args.steps, args.warmup, stream)
         log(f"[bench] plain: {plain}")
-    log("[bench] timing staged fill")
+    log("[bench] timing the staged fill")
     staged = time_fill(lambda: fill_ghosts(fs, sched, method="staged"), args.steps, args.warmup, stream)
     log(f"[bench] staged: {staged}")
-    head = staged
+    log("[bench] timing the graph-captured fill")
+    graph = FillGraph([(fs, sched)])
+    graphed = time_fill(graph.launch, args.steps, args.warmup, stream)
+    log(f"[bench] graph: {graphed}")
+    adjoint = None
+    if not args.no_adjoint:
+        log("[bench] timing the adjoint fill")
+        adj = torch.rand_like(fs.work)
+        adjoint = time_fill(lambda: fill_ghosts_adjoint(fs, sched, adj), args.steps, args.warmup, stream)
+        log(f"[bench] adjoint: {adjoint}")
+        del adj
+    head = graphed
     t_ms = head["mean_ms"]
-    gbs = bytes_eff / (t_ms * 1e-3) / 1e9
+    gbs = gbs_of(t_ms)
+
+    batched = None
+    if not args.no_batched:
+        log("[bench] building the 64-forest batch")
+        tb = time.perf_counter()
+        pairs = W.config3(count=64, layout=args.layout)
+        bvalues = sum(s.points() * f2.nvars for f2, s in pairs)
+        bblocks = sum({id(f2.forest): f2.nblocks for f2, _ in pairs}.values())
+        log(f"[bench] batch: {len(pairs)} field sets, {bblocks} blocks, {bvalues} values; "
+            f"{time.perf_counter() - tb:.1f}s")
+
+        def seq_plain():
+            for f2, s2 in pairs:
+                fill_ghosts(f2, s2, method="plain")
+
+        bplain = time_fill(seq_plain, args.batched_steps, 1, stream)
+        log(f"[bench] batch, 64 sequential plain fills: {bplain}")
+        bgraph_obj = FillGraph(pairs)
+        bgraph = time_fill(bgraph_obj.launch, args.batched_steps * 3, 1, stream)
+        log(f"[bench] batch, graph-captured batched fill: {bgraph}")
+        batched = {
+            "workload": "64 independent forests, seeds 2..65 (1^3-4^3 roots, N in {8,16,32}, 1-4 levels, "
+                        "G in {2,3}, 1-3 field sets of 2-10 vars, PointValue(2G), periodic, <=2 GiB each)",
+            "field_sets": len(pairs), "blocks": bblocks, "values_per_fill": bvalues,
+            "plain_sequential_ms": round(bplain["mean_ms"], 3),
+            "graph_batched_ms": round(bgraph["mean_ms"], 3),
+            "speedup": round(bplain["mean_ms"] / bgraph["mean_ms"], 2),
+            "graph_effective_GBs": round(16.0 * bvalues / (bgraph["mean_ms"] * 1e-3) / 1e9, 1),
+        }
+        del bgraph_obj, pairs
 
     cpu = None
     if not args.no_cpu:
         log("[bench] timing the CPU baseline (oracle)")
         cpu = cpu_baseline(builder)
         log(f"[bench] cpu: {cpu}")
+
+    def r(d):
+        return {k: round(v, 4) for k, v in d.items()} if d else None
 
     out = {
         "metric": METRIC,
@@ -157,16 +209,20 @@ def main():
         "data": "synthetic (seeded Gaussian bumps + 1e-4 noise, seed 5)",
         "config": {
             "workload": "large single forest: 8^3 roots, N=32, cell, G=3, 20 vars, PointValue(6,6), "
-                        "2 levels over 4 spheres, reflecting z-lo + outflow",
+                        "2 levels over 4 spheres (~12% refined), reflecting z-lo + outflow",
             "blocks": f.nleaves,
             "values_per_fill": values,
             "layout": args.layout,
-            "fill": "staged",
+            "fill": "graph-captured staged fill",
         },
         "pct_of_peak": round(100.0 * gbs / HBM_PEAK_GBS, 2),
-        "staged_ms": {k: round(v, 4) for k, v in staged.items()},
-        "plain_ms": {k: round(v, 4) for k, v in plain.items()} if plain else None,
-        "speedup_vs_plain": round(plain["mean_ms"] / staged["mean_ms"], 3) if plain else None,
+        "graph_ms": r(graphed),
+        "staged_ms": r(staged),
+        "plain_ms": r(plain),
+        "speedup_vs_plain": round(plain["mean_ms"] / t_ms, 3) if plain else None,
+        "adjoint_ms": r(adjoint),
+        "adjoint_over_forward": round(adjoint["mean_ms"] / t_ms, 3) if adjoint else None,
+        "batched": batched,
         "schedule_build_s": round(build_s, 3),
         "roofline": {
             "bound": "hbm",

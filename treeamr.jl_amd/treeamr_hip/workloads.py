@@ -112,17 +112,34 @@ def fill_gaussians(fs, seed, noise=1e-4):
     return fs
 
 
+def config3(count=64, first_seed=2, max_bytes=2 << 30, layout="arena", device="cuda", seed_data=True):
+    """The batched configuration: `count` independent forests from seeds
+    first_seed.., each with its 1-3 field sets; returns (fs, schedule) pairs."""
+    from .core import GhostSchedule
+    pairs = []
+    for i in range(count):
+        sd = first_seed + i
+        f, ops, G, nvars = random_forest(sd, max_blocks=4096, max_bytes=max_bytes)
+        for k, nv in enumerate(nvars):
+            fs = FieldSet(f, nv, G=G, device=device, layout=layout, fill=0.0)
+            if seed_data:
+                fill_uniform(fs, 1000 * sd + k)
+            pairs.append((fs, GhostSchedule(fs, ops)))
+    return pairs
+
+
 def make_fieldset(forest, device="cuda", layout="dense", fill=float("nan"), **kw):
     return FieldSet(forest, kw.pop("nvars"), device=device, layout=layout, fill=fill, **kw)
 
 
-def random_forest(seed, max_blocks=4096):
+def random_forest(seed, max_blocks=4096, max_bytes=None):
     """One forest of the batched configuration, drawn from `seed`: brick of
     r^3 roots (r in 1..4), N in {8,16,32}, 1-4 refinement levels around 1-3
     spheres of radius 0.05-0.3, G in {2,3}, 1-3 field sets of 2-10 variables.
     All faces periodic; PointValue of order 2G. Cell centering (vertex
     centering is not yet supported by this library). Draws whose blocks
-    exceed max_blocks are redrawn with fewer levels."""
+    exceed max_blocks, or whose field sets together exceed max_bytes, are
+    redrawn with fewer levels."""
     rng = np.random.default_rng(seed)
     r = int(rng.integers(1, 5))
     N = int(rng.choice([8, 16, 32]))
@@ -133,6 +150,14 @@ def random_forest(seed, max_blocks=4096):
     G = int(rng.choice([2, 3]))
     nsets = int(rng.integers(1, 4))
     nvars = [int(rng.integers(2, 11)) for _ in range(nsets)]
+    def too_big(f):
+        if f.nleaves > max_blocks:
+            return True
+        if max_bytes is not None:
+            per_block = (N + 2 * G) ** 3 * 8 * sum(nvars)
+            return f.nleaves * per_block > max_bytes
+        return False
+
     while True:
         f = Forest((r,) * 3, N=N, periodic=(True,) * 3, extents=((0.0, 1.0),) * 3)
         for lvl in range(levels):
@@ -144,9 +169,9 @@ def random_forest(seed, max_blocks=4096):
             hit &= level == lvl
             f.refine(np.nonzero(hit)[0])
             f.balance()
-            if f.nleaves > max_blocks:
+            if too_big(f):
                 break
-        if f.nleaves <= max_blocks or levels == 1:
+        if not too_big(f) or levels == 1:
             break
         levels -= 1
     ops = Operators(prolongation=2 * G, restriction=2 * G)
