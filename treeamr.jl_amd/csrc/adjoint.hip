@@ -90,6 +90,31 @@ __global__ void __launch_bounds__(kThreads) adjoint_gather_kernel(const Job* __r
       const int32_t y = int32_t(q - zq * uint32_t(T.ext[1]));
       const int32_t z = int32_t(zq);
       double acc = 0.0;
+      if (g.unit == 2) {
+        // A copy: along a dimension with step +-1 one target row reads this
+        // point, along a clamped one (step 0) every row does; rows in order.
+        const int32_t p0 = T.lo[0] + x, p1 = T.lo[1] + y, p2 = T.lo[2] + z;
+        int32_t a[3], b[3];
+        const int32_t pd[3] = {p0, p1, p2};
+        bool any = true;
+        for (int d = 0; d < 3; ++d) {
+          if (g.cs[d] == 0) {
+            a[d] = 0;
+            b[d] = g.n[d];
+          } else {
+            const int32_t r = (pd[d] - g.ca[d]) * g.cs[d];
+            a[d] = r;
+            b[d] = r + 1;
+            any = any && r >= 0 && r < g.n[d];
+          }
+        }
+        if (any)
+          for (int32_t r2 = a[2]; r2 < b[2]; ++r2)
+            for (int32_t r1 = a[1]; r1 < b[1]; ++r1)
+              for (int32_t r0 = a[0]; r0 < b[0]; ++r0)
+                acc = __dadd_rn(acc, tb[s3 * v + s0 * (g.tfirst[0] + r0) + s1 * (g.tfirst[1] + r1) +
+                                       s2 * (g.tfirst[2] + r2)]);
+      } else {
       for (int32_t j2 = T.start[2][z]; j2 < T.start[2][z + 1]; ++j2) {
         const int32_t p2 = T.list[2][j2], r2 = p2 >> 4, m2 = p2 & 15;
         const double w2 = g.weights[2][g.order[2] * r2 + m2];
@@ -103,6 +128,7 @@ __global__ void __launch_bounds__(kThreads) adjoint_gather_kernel(const Job* __r
                                                   s1 * (g.tfirst[1] + r1) + s2 * (g.tfirst[2] + r2)]));
           }
         }
+      }
       }
       if (g.factorcol) acc = __dmul_rn(acc, job.factors[v + int64_t(nvars) * (g.factorcol - 1)]);
       double* dst = sb + (s3 * v + s0 * (T.lo[0] + x) + s1 * (T.lo[1] + y) + s2 * (T.lo[2] + z));

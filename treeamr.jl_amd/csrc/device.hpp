@@ -37,7 +37,7 @@ struct alignas(16) DevGroup {
   int32_t kind;
   int32_t factorcol;
   int32_t ntransfers;
-  int32_t unit;
+  int32_t unit;       // 1: every stencil one point of weight one; 2: and affine rows
   int32_t n[3];       // box extents
   int32_t order[3];   // stencil widths
   int32_t tfirst[3];  // first target row, 0-based

@@ -205,7 +205,8 @@ void build_plan_device(tah_plan* p) {
     g.kind = h.kind;
     g.factorcol = h.factorcol;
     g.ntransfers = int32_t(h.tblocks.size());
-    g.unit = h.unit ? 1 : 0;
+    // 2: a unit copy whose rows are affine (ca, cs below hold its map).
+    g.unit = h.cls == kClsCopy ? 2 : h.unit ? 1 : 0;
     g.points = h.points;
     g.tblocks = reinterpret_cast<const int32_t*>(base + o.tb);
     g.sblocks = reinterpret_cast<const int32_t*>(base + o.sb);
