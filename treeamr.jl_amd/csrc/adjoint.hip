@@ -4,18 +4,32 @@
 // sources it never writes. Its transpose therefore runs the stages in
 // reverse: in each, every source point gathers w * f * adj[target] over the
 // transfers that read it, and then every target of the stage is zeroed.
-// The gather is deterministic and uses no float atomics: one workgroup owns
-// a source block and visits its incoming transfers in a fixed order (group,
-// then transfer), with a barrier between transfers, so that no two threads
-// ever update one point at once; within a transfer a point sums its
-// (target, tap) pairs in a fixed order from a host-built inverted stencil.
+//
+// An interpolating transfer is a tensor product of one-dimensional stencils
+// (weight (w1*w2)*w3 per tap), so its transpose is one too: three passes,
+// each applying one dimension's transposed stencil, take the transfer's
+// target box to its contribution over its source box, staged in scratch
+// memory (z first, then y, then x). Each output value of a pass sums its
+// (target row, tap) pairs in a fixed order from a host-built inverted stencil.
+//
+// The gather that follows is deterministic and uses no float atomics. Each
+// source block's read region is cut, along the faces of the boxes its
+// incoming transfers read, into cells; every point of a cell is read by one
+// and the same list of transfers (group, then transfer order). One thread
+// owns one source value and adds each transfer's contribution to it in that
+// order -- a copy's target value(s) or an interpolation's staged value, both
+// at an affine offset of the point -- so no two threads ever update one point
+// and no barrier is needed. A copy's target is read by exactly one thread
+// (its source point's), which zeroes it on the spot; the targets of the
+// interpolations are zeroed after the gather.
+//
 // The reference has no adjoint: this is new, pinned by the dot-product
 // identity and by the CPU transpose of the oracle.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstring>
-#include <map>
 #include <vector>
 
 #include "api_common.hpp"
@@ -25,117 +39,218 @@
 namespace tah {
 
 // One dimension of a group's transposed stencil: for source row x in the
-// read box lo..hi, the (target row, tap) pairs reading it, packed r*16 + m.
+// read box lo..lo+ext-1, the (target row, tap) pairs reading it, packed
+// r*16 + m, in list[start[x-lo]] .. list[start[x-lo+1]-1].
 struct alignas(16) DevGroupT {
   int32_t lo[3], ext[3];
-  int32_t pts;
-  int32_t pad;
-  FastDiv dx[2];
-  const int32_t* start[3];  // ext[d] + 1 offsets into list[d]
+  const int32_t* start[3];
   const int32_t* list[3];
 };
 
-struct alignas(16) AdjItem {
-  int32_t sblock, first, count, pad;
+// One pass over one transfer: output point (i0, i1, i2) of box o, variable
+// v, at out_off + ((i2*o1 + i1)*o0 + i0)*nvars + v, sums weight * input at
+// in_off + isv*v + is[d]*i[d] (d not contracted) + is[D]*r over the pairs
+// (r, tap) of the contracted dimension D's row i[D]. A pass tile takes rows
+// row0 .. row0+kAdjPerThread-1 of D and plane values pe0 .. pe0+kThreads-1,
+// a plane value being (v, then the lower, then the upper other dimension).
+struct alignas(16) AdjPass {
+  int64_t in_off, out_off;
+  int32_t group, plane;  // plane: values per row of D
+  int32_t o[3];
+  int32_t is[3];
+  int32_t isv, pad;
+  FastDiv flo;  // division by the lower other dimension's extent
+};
+
+struct alignas(16) AdjPassTile {
+  int32_t item, row0, pe0, pad;
+};
+
+// What one transfer adds to source point p of variable v: the sum over
+// rows r < c of buffer[base + a.p + av*v + rs.r], times the factor.
+// Buffer 0 is the working array (a copy: c > 1 along a clamped dimension),
+// buffer 1 the staged contributions (c = 1).
+struct alignas(16) AdjEntry {
+  int64_t base;
+  int32_t a[3];
+  int32_t av;
+  int32_t c[3];
+  int32_t rs[3];
+  int32_t buf;
+  int32_t factorcol;
+};
+
+// A box of one source block whose points are all read by the same
+// transfers: entries[first .. first+count-1].
+struct alignas(16) AdjCell {
+  int32_t sblock, first, count, npts;
+  int32_t lo[3], ext[3];
+  FastDiv dx[2];  // division by ext[0], ext[1]
+};
+
+// A workgroup's share: kAdjTile values of cell (or pass item) `item` from e0.
+struct alignas(8) AdjTile {
+  int32_t item, e0;
+};
+
+constexpr int kAdjPerThread = 4;
+constexpr int kAdjTile = kThreads * kAdjPerThread;
+
+struct AdjStage {
+  int64_t pass_tiles[3][2];  // per pass (z, y, x): first tile, count
+  int64_t gather[2];
+  int64_t zero[2];
+  int64_t scratch_off[3];   // outputs of the passes, in the scratch buffer
 };
 
 struct AdjointTables {
   char* dmem = nullptr;
+  double* scratch = nullptr;
+  int64_t scratch_values = 0;
   DevGroupT* dgt = nullptr;
-  AdjItem* ditems = nullptr;
-  int32_t* dentries = nullptr;  // (group, transfer) pairs
+  AdjPass* dpass = nullptr;
+  AdjPassTile* dptiles = nullptr;
+  AdjCell* dcells = nullptr;
+  AdjTile* dtiles = nullptr;
+  AdjEntry* dentries = nullptr;
   Tile* dzero = nullptr;
-  // per stage: item range and zero-tile range
-  std::vector<std::array<int64_t, 4>> stage_ranges;
+  std::vector<AdjStage> stages;
 };
 
 void free_adjoint(tah_plan* p) {
   if (!p->adj) return;
   if (p->adj->dmem) (void)hipFree(p->adj->dmem);
+  if (p->adj->scratch) (void)hipFree(p->adj->scratch);
   delete p->adj;
   p->adj = nullptr;
 }
 
+// Pass D (2: z, 1: y, 0: x) of every interpolating transfer of a stage.
+// The contracted row is one for the whole workgroup, so its inverted
+// stencil is read through scalar loads; lanes run over the plane.
+template <int D>
+__global__ void __launch_bounds__(kThreads) adjoint_pass_kernel(const Job* __restrict__ jobs,
+                                                                const DevGroupT* __restrict__ gts,
+                                                                const AdjPass* __restrict__ items,
+                                                                const AdjPassTile* __restrict__ tiles,
+                                                                uint32_t ntiles,
+                                                                const double* __restrict__ scratch_in,
+                                                                double* __restrict__ out) {
+  constexpr int DL = D == 0 ? 1 : 0;  // the lower other dimension
+  constexpr int DU = D == 2 ? 1 : 2;  // the upper one
+  const AdjPassTile tile = tiles[xcd_tile(blockIdx.x, ntiles)];
+  const AdjPass& it = items[uniform(tile.item)];
+  const Job& job = jobs[0];
+  const int32_t gi = uniform(it.group);
+  const DevGroup& g = job.groups[gi];
+  const DevGroupT& T = gts[gi];
+  const double* __restrict__ in = (D == 2 ? job.work : scratch_in) + it.in_off;
+  double* __restrict__ o = out + it.out_off;
+  const int32_t nvars = job.nvars;
+  const uint32_t pe = uint32_t(tile.pe0) + threadIdx.x;
+  if (pe >= uint32_t(it.plane)) return;
+  const uint32_t r = job.dnv.div(pe);
+  const int32_t v = int32_t(pe - r * uint32_t(nvars));
+  const uint32_t q = it.flo.div(r);
+  int32_t i[3];
+  i[DL] = int32_t(r - q * uint32_t(it.o[DL]));
+  i[DU] = int32_t(q);
+  const int32_t b = it.isv * v + it.is[DL] * i[DL] + it.is[DU] * i[DU];
+  const int32_t ob = (((D == 2 ? 0 : i[2]) * it.o[1] + (D == 1 ? 0 : i[1])) * it.o[0] + (D == 0 ? 0 : i[0])) * nvars + v;
+  const int32_t ostep = (D == 0 ? 1 : D == 1 ? it.o[0] : it.o[0] * it.o[1]) * nvars;
+  const auto start = constant(T.start[D]);
+  const auto list = constant(T.list[D]);
+  const auto w = constant(g.weights[D]);
+  const auto src = global(in);
+  const int32_t order = uniform(g.order[D]);
+  const int32_t isd = it.is[D];
+  const int32_t row0 = uniform(tile.row0);
+  const int32_t rows = min(kAdjPerThread, uniform(it.o[D]) - row0);
+  for (int k = 0; k < rows; ++k) {
+    const int32_t x = row0 + k;
+    double acc = 0.0;
+    const int32_t j1 = start[x + 1];
+#pragma unroll 4
+    for (int32_t j = start[x]; j < j1; ++j) {
+      const int32_t pk = list[j], rr = pk >> 4, m = pk & 15;
+      acc = __dadd_rn(acc, __dmul_rn(w[order * rr + m], src[b + isd * rr]));
+    }
+    o[ob + ostep * x] = acc;
+  }
+}
+
 __global__ void __launch_bounds__(kThreads) adjoint_gather_kernel(const Job* __restrict__ jobs,
-                                                                  const DevGroupT* __restrict__ gts,
-                                                                  const AdjItem* __restrict__ items,
-                                                                  const int32_t* __restrict__ entries) {
-  const AdjItem item = items[blockIdx.x];
+                                                                  const AdjCell* __restrict__ cells,
+                                                                  const AdjTile* __restrict__ tiles,
+                                                                  const AdjEntry* __restrict__ entries,
+                                                                  uint32_t ntiles,
+                                                                  double* __restrict__ staged) {
+  const AdjTile tile = tiles[xcd_tile(blockIdx.x, ntiles)];
+  const AdjCell& c = cells[uniform(tile.item)];
   const Job& job = jobs[0];
   const int32_t s0 = int32_t(job.stride[0]), s1 = int32_t(job.stride[1]),
                 s2 = int32_t(job.stride[2]), s3 = int32_t(job.stride[3]);
-  const int64_t s4 = job.stride[4];
   const int32_t nvars = job.nvars;
-  double* __restrict__ sb = job.work + s4 * item.sblock;
-  for (int32_t e = 0; e < item.count; ++e) {
-    const int32_t gi = entries[2 * (item.first + e)];
-    const int32_t t = entries[2 * (item.first + e) + 1];
-    const DevGroup& g = job.groups[gi];
-    const DevGroupT& T = gts[gi];
-    const double* __restrict__ tb = job.work + s4 * g.tblocks[t];
-    const uint32_t n = uint32_t(T.pts) * uint32_t(nvars);
-    for (uint32_t idx = threadIdx.x; idx < n; idx += kThreads) {
-      uint32_t r;
-      int32_t v;
-      if (job.vfast) {
-        r = job.dnv.div(idx);
-        v = int32_t(idx - r * uint32_t(nvars));
-      } else {
-        v = int32_t(idx / uint32_t(T.pts));
-        r = idx - uint32_t(v) * uint32_t(T.pts);
-      }
-      const uint32_t q = T.dx[0].div(r);
-      const int32_t x = int32_t(r - q * uint32_t(T.ext[0]));
-      const uint32_t zq = T.dx[1].div(q);
-      const int32_t y = int32_t(q - zq * uint32_t(T.ext[1]));
-      const int32_t z = int32_t(zq);
-      double acc = 0.0;
-      if (g.unit == 2) {
-        // A copy: along a dimension with step +-1 one target row reads this
-        // point, along a clamped one (step 0) every row does; rows in order.
-        const int32_t p0 = T.lo[0] + x, p1 = T.lo[1] + y, p2 = T.lo[2] + z;
-        int32_t a[3], b[3];
-        const int32_t pd[3] = {p0, p1, p2};
-        bool any = true;
-        for (int d = 0; d < 3; ++d) {
-          if (g.cs[d] == 0) {
-            a[d] = 0;
-            b[d] = g.n[d];
-          } else {
-            const int32_t r = (pd[d] - g.ca[d]) * g.cs[d];
-            a[d] = r;
-            b[d] = r + 1;
-            any = any && r >= 0 && r < g.n[d];
-          }
-        }
-        if (any)
-          for (int32_t r2 = a[2]; r2 < b[2]; ++r2)
-            for (int32_t r1 = a[1]; r1 < b[1]; ++r1)
-              for (int32_t r0 = a[0]; r0 < b[0]; ++r0)
-                acc = __dadd_rn(acc, tb[s3 * v + s0 * (g.tfirst[0] + r0) + s1 * (g.tfirst[1] + r1) +
-                                       s2 * (g.tfirst[2] + r2)]);
-      } else {
-      for (int32_t j2 = T.start[2][z]; j2 < T.start[2][z + 1]; ++j2) {
-        const int32_t p2 = T.list[2][j2], r2 = p2 >> 4, m2 = p2 & 15;
-        const double w2 = g.weights[2][g.order[2] * r2 + m2];
-        for (int32_t j1 = T.start[1][y]; j1 < T.start[1][y + 1]; ++j1) {
-          const int32_t p1 = T.list[1][j1], r1 = p1 >> 4, m1 = p1 & 15;
-          const double w1 = g.weights[1][g.order[1] * r1 + m1];
-          for (int32_t j0 = T.start[0][x]; j0 < T.start[0][x + 1]; ++j0) {
-            const int32_t p0 = T.list[0][j0], r0 = p0 >> 4, m0 = p0 & 15;
-            const double w = __dmul_rn(__dmul_rn(g.weights[0][g.order[0] * r0 + m0], w1), w2);
-            acc = __dadd_rn(acc, __dmul_rn(w, tb[s3 * v + s0 * (g.tfirst[0] + r0) +
-                                                  s1 * (g.tfirst[1] + r1) + s2 * (g.tfirst[2] + r2)]));
-          }
-        }
-      }
-      }
-      if (g.factorcol) acc = __dmul_rn(acc, job.factors[v + int64_t(nvars) * (g.factorcol - 1)]);
-      double* dst = sb + (s3 * v + s0 * (T.lo[0] + x) + s1 * (T.lo[1] + y) + s2 * (T.lo[2] + z));
-      *dst = __dadd_rn(*dst, acc);
+  const uint32_t npts = uint32_t(c.npts);
+  const uint32_t total = npts * uint32_t(nvars);
+  double* __restrict__ sb = job.work + job.stride[4] * c.sblock;
+  int32_t p[kAdjPerThread][3], var[kAdjPerThread], off[kAdjPerThread];
+  bool ok[kAdjPerThread];
+  double tot[kAdjPerThread];
+#pragma unroll
+  for (int k = 0; k < kAdjPerThread; ++k) {
+    const uint32_t e = uint32_t(tile.e0) + uint32_t(k * kThreads) + threadIdx.x;
+    ok[k] = e < total;
+    const uint32_t ee = ok[k] ? e : 0;
+    uint32_t r;
+    if (job.vfast) {
+      r = job.dnv.div(ee);
+      var[k] = int32_t(ee - r * uint32_t(nvars));
+    } else {
+      var[k] = int32_t(ee / npts);
+      r = ee - uint32_t(var[k]) * npts;
     }
-    __syncthreads();
+    const uint32_t q = c.dx[0].div(r);
+    const uint32_t zq = c.dx[1].div(q);
+    p[k][0] = c.lo[0] + int32_t(r - q * uint32_t(c.ext[0]));
+    p[k][1] = c.lo[1] + int32_t(q - zq * uint32_t(c.ext[1]));
+    p[k][2] = c.lo[2] + int32_t(zq);
+    off[k] = s3 * var[k] + s0 * p[k][0] + s1 * p[k][1] + s2 * p[k][2];
+    tot[k] = ok[k] ? sb[off[k]] : 0.0;
   }
+  const int32_t first = uniform(c.first), count = uniform(c.count);
+  for (int32_t q = 0; q < count; ++q) {
+    const AdjEntry& E = entries[first + q];
+    const bool copy = E.buf == 0;
+    // A copy's targets are read by this thread alone: it zeroes them too.
+    double* __restrict__ src = (copy ? job.work : staged) + E.base;
+    const int32_t c0 = E.c[0], c1 = E.c[1], c2 = E.c[2];
+    const bool single = c0 * c1 * c2 == 1;
+#pragma unroll
+    for (int k = 0; k < kAdjPerThread; ++k) {
+      if (!ok[k]) continue;
+      const int32_t o = E.a[0] * p[k][0] + E.a[1] * p[k][1] + E.a[2] * p[k][2] + E.av * var[k];
+      double acc;
+      if (single) {
+        acc = src[o];
+        if (copy) src[o] = 0.0;
+      } else {
+        acc = 0.0;
+        for (int32_t r2 = 0; r2 < c2; ++r2)
+          for (int32_t r1 = 0; r1 < c1; ++r1)
+            for (int32_t r0 = 0; r0 < c0; ++r0) {
+              const int32_t oo = o + E.rs[0] * r0 + E.rs[1] * r1 + E.rs[2] * r2;
+              acc = __dadd_rn(acc, src[oo]);
+              if (copy) src[oo] = 0.0;
+            }
+      }
+      if (E.factorcol) acc = __dmul_rn(acc, job.factors[var[k] + int64_t(nvars) * (E.factorcol - 1)]);
+      tot[k] = __dadd_rn(tot[k], acc);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kAdjPerThread; ++k)
+    if (ok[k]) sb[off[k]] = tot[k];
 }
 
 // Zero the target boxes of one stage (tiles as the copy kernel's).
@@ -159,6 +274,64 @@ __global__ void __launch_bounds__(kThreads) zero_kernel(const Job* __restrict__ 
 
 namespace {
 
+struct Box {
+  int32_t lo[3], hi[3];  // half-open
+};
+
+struct Reader {
+  int32_t group, transfer;
+  AdjEntry entry;
+};
+
+// Cut the read region of one source block (the boxes of the groups that
+// read it) into cells along every box face; each cell with at least one
+// reader gets the readers' entries in their given order.
+void cut_cells(int32_t sblock, const std::vector<Reader>& readers, const std::vector<Box>& boxes,
+               std::vector<AdjCell>& cells, std::vector<AdjEntry>& entries) {
+  std::vector<int32_t> cuts[3];
+  for (const auto& r : readers)
+    for (int d = 0; d < 3; ++d) {
+      cuts[d].push_back(boxes[r.group].lo[d]);
+      cuts[d].push_back(boxes[r.group].hi[d]);
+    }
+  for (auto& c : cuts) {
+    std::sort(c.begin(), c.end());
+    c.erase(std::unique(c.begin(), c.end()), c.end());
+  }
+  for (size_t iz = 0; iz + 1 < cuts[2].size(); ++iz)
+    for (size_t iy = 0; iy + 1 < cuts[1].size(); ++iy)
+      for (size_t ix = 0; ix + 1 < cuts[0].size(); ++ix) {
+        const int32_t lo[3] = {cuts[0][ix], cuts[1][iy], cuts[2][iz]};
+        const int32_t hi[3] = {cuts[0][ix + 1], cuts[1][iy + 1], cuts[2][iz + 1]};
+        const int32_t first = int32_t(entries.size());
+        for (const auto& r : readers) {
+          const Box& b = boxes[r.group];
+          bool in = true;
+          for (int d = 0; d < 3; ++d) in = in && b.lo[d] <= lo[d] && hi[d] <= b.hi[d];
+          if (in) entries.push_back(r.entry);
+        }
+        const int32_t count = int32_t(entries.size()) - first;
+        if (count == 0) continue;
+        AdjCell c{};
+        c.sblock = sblock;
+        c.first = first;
+        c.count = count;
+        for (int d = 0; d < 3; ++d) {
+          c.lo[d] = lo[d];
+          c.ext[d] = hi[d] - lo[d];
+        }
+        c.npts = c.ext[0] * c.ext[1] * c.ext[2];
+        c.dx[0] = FastDiv(uint32_t(c.ext[0]));
+        c.dx[1] = FastDiv(uint32_t(c.ext[1]));
+        cells.push_back(c);
+      }
+}
+
+void push_tiles(std::vector<AdjTile>& tiles, int32_t item, int64_t elems) {
+  if (elems > INT32_MAX) fail("an adjoint work item exceeds 2^31 values", kUnsupported);
+  for (int64_t e = 0; e < elems; e += kAdjTile) tiles.push_back(AdjTile{item, int32_t(e)});
+}
+
 void build_adjoint(tah_plan* p) {
   auto* A = new AdjointTables();
   p->adj = A;
@@ -169,11 +342,15 @@ void build_adjoint(tah_plan* p) {
     if (bytes) std::memcpy(host.data() + off, src, bytes);
     return off;
   };
+  const tah_layout& L = p->layout;
+  const int32_t nvars = L.nvars;
+  // Transposed stencils and read boxes of every group.
   struct GT {
     int32_t lo[3], ext[3];
     int64_t start[3], list[3];
   };
   std::vector<GT> gts(p->groups.size());
+  std::vector<Box> boxes(p->groups.size());
   for (size_t gi = 0; gi < p->groups.size(); ++gi) {
     const PlanGroup& g = p->groups[gi];
     GT& h = gts[gi];
@@ -186,6 +363,8 @@ void build_adjoint(tah_plan* p) {
       if (g.srcstart[d].empty()) lo = hi = 0;
       h.lo[d] = lo;
       h.ext[d] = hi - lo + 1;
+      boxes[gi].lo[d] = lo;
+      boxes[gi].hi[d] = hi + 1;
       if (g.order[d] > 16) fail("the adjoint supports stencils of at most 16 taps", kUnsupported);
       std::vector<std::vector<int32_t>> by(h.ext[d]);
       for (int32_t r = 0; r < int32_t(g.srcstart[d].size()); ++r)
@@ -199,45 +378,184 @@ void build_adjoint(tah_plan* p) {
       h.list[d] = put(list.data(), 4 * int64_t(list.size()));
     }
   }
-  // Items and entries, stage by stage.
-  std::vector<AdjItem> items;
-  std::vector<int32_t> entries;
+  std::vector<AdjPass> passes;
+  std::vector<AdjPassTile> ptiles;
+  std::vector<AdjTile> tiles;
+  std::vector<AdjCell> cells;
+  std::vector<AdjEntry> entries;
   std::vector<Tile> zeros;
-  const int32_t nvars = p->layout.nvars;
+  int64_t scratch_max = 0;
   for (int32_t st = 0; st < p->nstages; ++st) {
-    std::array<int64_t, 4> rng{int64_t(items.size()), 0, int64_t(zeros.size()), 0};
-    std::vector<std::array<int32_t, 3>> es;  // sblock, group, transfer
+    AdjStage S{};
+    std::vector<std::array<int32_t, 2>> interp;  // (group, transfer)
+    std::vector<std::array<int32_t, 3>> es;      // sblock, group, transfer
+    S.zero[0] = int64_t(zeros.size());
     for (size_t gi = 0; gi < p->groups.size(); ++gi) {
       const PlanGroup& g = p->groups[gi];
       if (g.stage != st || g.points == 0) continue;
+      int32_t ca[3], cs[3];
+      const bool copy = affine_copy(g, ca, cs);
       for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t) {
         es.push_back({g.sblocks[t], int32_t(gi), t});
-        int64_t elems = int64_t(g.points) * nvars;
+        if (copy) continue;  // the gather zeroes a copy's targets
+        interp.push_back({int32_t(gi), t});
+        const int64_t elems = int64_t(g.points) * nvars;
         for (int64_t e = 0; e < elems; e += kCopyTile) zeros.push_back(Tile{0, int32_t(gi), t, int32_t(e), 0, 0, 0, 0});
       }
     }
+    S.zero[1] = int64_t(zeros.size()) - S.zero[0];
+    // Scratch: the outputs of the z, y and x passes of every interpolating
+    // transfer, one after the other.
+    std::vector<int64_t> off_of(interp.size() * 3);
+    int64_t tot[3] = {0, 0, 0};
+    for (size_t k = 0; k < interp.size(); ++k) {
+      const PlanGroup& g = p->groups[interp[k][0]];
+      const Box& b = boxes[interp[k][0]];
+      const int64_t e0 = b.hi[0] - b.lo[0], e1 = b.hi[1] - b.lo[1], e2 = b.hi[2] - b.lo[2];
+      const int64_t sz[3] = {int64_t(g.n[0]) * g.n[1] * e2, int64_t(g.n[0]) * e1 * e2, e0 * e1 * e2};
+      for (int q = 0; q < 3; ++q) {
+        off_of[3 * k + q] = tot[q];
+        tot[q] += sz[q] * nvars;
+      }
+    }
+    S.scratch_off[0] = 0;
+    S.scratch_off[1] = tot[0];
+    S.scratch_off[2] = tot[0] + tot[1];
+    scratch_max = std::max(scratch_max, tot[0] + tot[1] + tot[2]);
+    for (int q = 0; q < 3; ++q) {
+      S.pass_tiles[q][0] = int64_t(ptiles.size());
+      for (size_t k = 0; k < interp.size(); ++k) {
+        const int32_t gi = interp[k][0], t = interp[k][1];
+        const PlanGroup& g = p->groups[gi];
+        const Box& b = boxes[gi];
+        const int32_t e[3] = {b.hi[0] - b.lo[0], b.hi[1] - b.lo[1], b.hi[2] - b.lo[2]};
+        AdjPass P{};
+        P.group = gi;
+        if (q == 0) {  // z: target box of the working array -> (n0, n1, e2)
+          P.in_off = L.stride[4] * g.tblocks[t] + L.stride[0] * g.tfirst[0] + L.stride[1] * g.tfirst[1] +
+                     L.stride[2] * g.tfirst[2];
+          for (int d = 0; d < 3; ++d) P.is[d] = int32_t(L.stride[d]);
+          P.isv = int32_t(L.stride[3]);
+          P.o[0] = g.n[0];
+          P.o[1] = g.n[1];
+          P.o[2] = e[2];
+        } else if (q == 1) {  // y: (n0, n1, e2) -> (n0, e1, e2)
+          P.in_off = off_of[3 * k + 0];
+          P.is[0] = nvars;
+          P.is[1] = nvars * g.n[0];
+          P.is[2] = nvars * g.n[0] * g.n[1];
+          P.isv = 1;
+          P.o[0] = g.n[0];
+          P.o[1] = e[1];
+          P.o[2] = e[2];
+        } else {  // x: (n0, e1, e2) -> (e0, e1, e2)
+          P.in_off = S.scratch_off[1] + off_of[3 * k + 1];
+          P.is[0] = nvars;
+          P.is[1] = nvars * g.n[0];
+          P.is[2] = nvars * g.n[0] * e[1];
+          P.isv = 1;
+          P.o[0] = e[0];
+          P.o[1] = e[1];
+          P.o[2] = e[2];
+        }
+        P.out_off = S.scratch_off[q] + off_of[3 * k + q];
+        const int D = 2 - q, DL = D == 0 ? 1 : 0, DU = D == 2 ? 1 : 2;
+        const int64_t plane = int64_t(P.o[DL]) * P.o[DU] * nvars;
+        if (plane > INT32_MAX) fail("an adjoint pass plane exceeds 2^31 values", kUnsupported);
+        P.plane = int32_t(plane);
+        P.flo = FastDiv(uint32_t(std::max(1, P.o[DL])));
+        passes.push_back(P);
+        const int32_t item = int32_t(passes.size() - 1);
+        for (int32_t r0 = 0; r0 < P.o[D]; r0 += kAdjPerThread)
+          for (int64_t e = 0; e < plane; e += kThreads) ptiles.push_back(AdjPassTile{item, r0, int32_t(e), 0});
+      }
+      S.pass_tiles[q][1] = int64_t(ptiles.size()) - S.pass_tiles[q][0];
+    }
+    // The gather: cells of every source block, entries in (group, transfer) order.
     std::stable_sort(es.begin(), es.end(), [](const std::array<int32_t, 3>& a, const std::array<int32_t, 3>& b) {
       return a[0] < b[0];
     });
+    std::vector<int64_t> interp_index(p->groups.size(), -1);  // first interp slot of a group
+    for (size_t k = 0; k < interp.size(); ++k)
+      if (interp_index[interp[k][0]] < 0) interp_index[interp[k][0]] = int64_t(k);
+    S.gather[0] = int64_t(tiles.size());
+    std::vector<Reader> readers;
     for (size_t i = 0; i < es.size();) {
       size_t j = i;
-      while (j < es.size() && es[j][0] == es[i][0]) ++j;
-      items.push_back(AdjItem{es[i][0], int32_t(entries.size() / 2), int32_t(j - i), 0});
-      for (size_t k = i; k < j; ++k) {
-        entries.push_back(es[k][1]);
-        entries.push_back(es[k][2]);
+      readers.clear();
+      for (; j < es.size() && es[j][0] == es[i][0]; ++j) {
+        const int32_t gi = es[j][1], t = es[j][2];
+        const PlanGroup& g = p->groups[gi];
+        const Box& b = boxes[gi];
+        Reader R{gi, t, AdjEntry{}};
+        AdjEntry& E = R.entry;
+        E.factorcol = g.factorcol;
+        int32_t ca[3], cs[3];
+        if (affine_copy(g, ca, cs)) {
+          // target row r = (p - ca)*cs along a stepped dimension; all rows
+          // of a clamped one
+          E.buf = 0;
+          E.base = L.stride[4] * g.tblocks[t];
+          E.av = int32_t(L.stride[3]);
+          for (int d = 0; d < 3; ++d) {
+            E.base += L.stride[d] * g.tfirst[d];
+            if (cs[d] == 0) {
+              E.a[d] = 0;
+              E.c[d] = g.n[d];
+              E.rs[d] = int32_t(L.stride[d]);
+            } else {
+              // the box is the exact image of the rows: every point has its row
+              for (int32_t x : {b.lo[d], b.hi[d] - 1}) {
+                const int32_t r = (x - ca[d]) * cs[d];
+                if (r < 0 || r >= g.n[d]) fail("adjoint: a copy's read box exceeds its rows");
+              }
+              E.a[d] = int32_t(L.stride[d]) * cs[d];
+              E.base -= int64_t(L.stride[d]) * cs[d] * ca[d];
+              E.c[d] = 1;
+              E.rs[d] = 0;
+            }
+          }
+        } else {
+          // staged value of point p at ((p2-lo2)*e1 + (p1-lo1))*e0 + (p0-lo0)
+          const int64_t k = interp_index[gi] + t;  // transfers of a group are consecutive
+          const int32_t e0 = b.hi[0] - b.lo[0], e1 = b.hi[1] - b.lo[1];
+          E.buf = 1;
+          E.av = 1;
+          E.a[0] = nvars;
+          E.a[1] = nvars * e0;
+          E.a[2] = nvars * e0 * e1;
+          E.base = S.scratch_off[2] + off_of[3 * k + 2] -
+                   (int64_t(E.a[0]) * b.lo[0] + int64_t(E.a[1]) * b.lo[1] + int64_t(E.a[2]) * b.lo[2]);
+          for (int d = 0; d < 3; ++d) {
+            E.c[d] = 1;
+            E.rs[d] = 0;
+          }
+        }
+        readers.push_back(R);
       }
+      const size_t c0 = cells.size();
+      cut_cells(es[i][0], readers, boxes, cells, entries);
+      for (size_t c = c0; c < cells.size(); ++c)
+        push_tiles(tiles, int32_t(c), int64_t(cells[c].npts) * nvars);
       i = j;
     }
-    rng[1] = int64_t(items.size()) - rng[0];
-    rng[3] = int64_t(zeros.size()) - rng[2];
-    A->stage_ranges.push_back(rng);
+    S.gather[1] = int64_t(tiles.size()) - S.gather[0];
+    if (S.gather[1] > INT32_MAX / 2 || S.zero[1] > INT32_MAX / 2)
+      fail("too many adjoint tiles in one stage", kUnsupported);
+    A->stages.push_back(S);
   }
-  int64_t off_items = put(items.data(), int64_t(sizeof(AdjItem) * items.size()));
-  int64_t off_entries = put(entries.data(), 4 * int64_t(entries.size()));
-  int64_t off_zero = put(zeros.data(), int64_t(sizeof(Tile) * zeros.size()));
-  int64_t off_gt = put(nullptr, int64_t(sizeof(DevGroupT) * std::max<size_t>(1, gts.size())));
+  if (cells.size() > size_t(INT32_MAX) || passes.size() > size_t(INT32_MAX) || entries.size() > size_t(INT32_MAX))
+    fail("adjoint tables too large", kUnsupported);
+  const int64_t off_pass = put(passes.data(), int64_t(sizeof(AdjPass) * passes.size()));
+  const int64_t off_ptiles = put(ptiles.data(), int64_t(sizeof(AdjPassTile) * ptiles.size()));
+  const int64_t off_cells = put(cells.data(), int64_t(sizeof(AdjCell) * cells.size()));
+  const int64_t off_tiles = put(tiles.data(), int64_t(sizeof(AdjTile) * tiles.size()));
+  const int64_t off_entries = put(entries.data(), int64_t(sizeof(AdjEntry) * entries.size()));
+  const int64_t off_zero = put(zeros.data(), int64_t(sizeof(Tile) * zeros.size()));
+  const int64_t off_gt = put(nullptr, int64_t(sizeof(DevGroupT) * std::max<size_t>(1, gts.size())));
   HIPCHECK(hipMalloc(&A->dmem, host.size()));
+  if (scratch_max > 0) HIPCHECK(hipMalloc(&A->scratch, sizeof(double) * scratch_max));
+  A->scratch_values = scratch_max;
   char* base = A->dmem;
   std::vector<DevGroupT> dg(gts.size());
   for (size_t gi = 0; gi < gts.size(); ++gi) {
@@ -248,17 +566,30 @@ void build_adjoint(tah_plan* p) {
       d.start[k] = reinterpret_cast<const int32_t*>(base + gts[gi].start[k]);
       d.list[k] = reinterpret_cast<const int32_t*>(base + gts[gi].list[k]);
     }
-    d.pts = d.ext[0] * d.ext[1] * d.ext[2];
-    d.dx[0] = FastDiv(uint32_t(std::max(1, d.ext[0])));
-    d.dx[1] = FastDiv(uint32_t(std::max(1, d.ext[1])));
     dg[gi] = d;
   }
   if (!dg.empty()) std::memcpy(host.data() + off_gt, dg.data(), sizeof(DevGroupT) * dg.size());
   HIPCHECK(hipMemcpy(A->dmem, host.data(), host.size(), hipMemcpyHostToDevice));
   A->dgt = reinterpret_cast<DevGroupT*>(base + off_gt);
-  A->ditems = reinterpret_cast<AdjItem*>(base + off_items);
-  A->dentries = reinterpret_cast<int32_t*>(base + off_entries);
+  A->dpass = reinterpret_cast<AdjPass*>(base + off_pass);
+  A->dptiles = reinterpret_cast<AdjPassTile*>(base + off_ptiles);
+  A->dcells = reinterpret_cast<AdjCell*>(base + off_cells);
+  A->dtiles = reinterpret_cast<AdjTile*>(base + off_tiles);
+  A->dentries = reinterpret_cast<AdjEntry*>(base + off_entries);
   A->dzero = reinterpret_cast<Tile*>(base + off_zero);
+}
+
+template <int D>
+void launch_pass(tah_plan* p, const AdjStage& S, int q, hipStream_t s) {
+  const AdjointTables& A = *p->adj;
+  const int64_t n = S.pass_tiles[q][1];
+  if (n == 0) return;
+  // pass items address the scratch buffer from its start
+  const double* in = q == 0 ? nullptr : A.scratch;
+  double* out = A.scratch;
+  hipLaunchKernelGGL(adjoint_pass_kernel<D>, dim3(uint32_t(n)), dim3(kThreads), 0, s, p->djob, A.dgt, A.dpass,
+                     A.dptiles + S.pass_tiles[q][0], uint32_t(n), in, out);
+  HIPCHECK(hipGetLastError());
 }
 
 }  // namespace
@@ -279,16 +610,21 @@ extern "C" tah_status tah_fill_adjoint(tah_plan* p, double* adj, const double* f
       p->last_job = job;
       HIPCHECK(hipMemcpyAsync(p->djob, &p->last_job, sizeof(Job), hipMemcpyHostToDevice, s));
     }
+    const AdjointTables& A = *p->adj;
     for (int32_t st = p->nstages - 1; st >= 0; --st) {
-      const auto& r = p->adj->stage_ranges[st];
-      if (r[1] > 0) {
-        hipLaunchKernelGGL(adjoint_gather_kernel, dim3(uint32_t(r[1])), dim3(kThreads), 0, s, p->djob,
-                           p->adj->dgt, p->adj->ditems + r[0], p->adj->dentries);
+      const AdjStage& S = A.stages[st];
+      launch_pass<2>(p, S, 0, s);
+      launch_pass<1>(p, S, 1, s);
+      launch_pass<0>(p, S, 2, s);
+      if (S.gather[1] > 0) {
+        hipLaunchKernelGGL(adjoint_gather_kernel, dim3(uint32_t(S.gather[1])), dim3(kThreads), 0, s, p->djob,
+                           A.dcells, A.dtiles + S.gather[0], A.dentries, uint32_t(S.gather[1]),
+                           A.scratch);
         HIPCHECK(hipGetLastError());
       }
-      if (r[3] > 0) {
-        hipLaunchKernelGGL(zero_kernel, dim3(uint32_t(r[3])), dim3(kThreads), 0, s, p->djob,
-                           p->adj->dzero + r[2]);
+      if (S.zero[1] > 0) {
+        hipLaunchKernelGGL(zero_kernel, dim3(uint32_t(S.zero[1])), dim3(kThreads), 0, s, p->djob,
+                           A.dzero + S.zero[0]);
         HIPCHECK(hipGetLastError());
       }
     }

@@ -29,6 +29,18 @@ __device__ __forceinline__ T* uniform(T* p) {
   return reinterpret_cast<T*>((hi << 32) | lo);
 }
 
+// A uniform table pointer in the constant address space, so that what is
+// read through it arrives by scalar loads into SGPRs.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* constant(const T* p) {
+  return (const __attribute__((address_space(4))) T*)(uniform(const_cast<T*>(p)));
+}
+// A pointer in the global address space (plain vector loads, not flat ones).
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* global(const T* p) {
+  return (const __attribute__((address_space(1))) T*)(p);
+}
+
 __device__ __forceinline__ double scaled(double acc, const double* __restrict__ factors,
                                          int32_t factorcol, int32_t nvars, int32_t v) {
   if (factorcol == 0) return acc;

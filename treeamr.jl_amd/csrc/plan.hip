@@ -8,6 +8,21 @@
 
 namespace tah {
 
+// A unit copy whose source row is affine in the target row along every
+// dimension: ca + cs*r with cs in {-1, 0, 1}.
+bool affine_copy(const PlanGroup& g, int32_t ca[3], int32_t cs[3]) {
+  if (!g.unit) return false;
+  for (int d = 0; d < 3; ++d) {
+    const auto& s = g.srcstart[d];
+    ca[d] = s.empty() ? 0 : s[0];
+    cs[d] = s.size() > 1 ? s[1] - s[0] : 0;
+    if (cs[d] < -1 || cs[d] > 1) return false;
+    for (size_t r = 0; r < s.size(); ++r)
+      if (s[r] != ca[d] + cs[d] * int32_t(r)) return false;
+  }
+  return true;
+}
+
 namespace {
 
 int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
@@ -39,21 +54,6 @@ void pair_rows(const std::vector<int32_t>& ss, int shift, std::vector<std::array
       i += 1;
     }
   }
-}
-
-// A unit copy whose source row is affine in the target row along every
-// dimension: ca + cs*r with cs in {-1, 0, 1}.
-bool affine_copy(const PlanGroup& g, int32_t ca[3], int32_t cs[3]) {
-  if (!g.unit) return false;
-  for (int d = 0; d < 3; ++d) {
-    const auto& s = g.srcstart[d];
-    ca[d] = s.empty() ? 0 : s[0];
-    cs[d] = s.size() > 1 ? s[1] - s[0] : 0;
-    if (cs[d] < -1 || cs[d] > 1) return false;
-    for (size_t r = 0; r < s.size(); ++r)
-      if (s[r] != ca[d] + cs[d] * int32_t(r)) return false;
-  }
-  return true;
 }
 
 // Bitwise key of a vector of doubles.

@@ -102,6 +102,9 @@ struct tah_plan {
 };
 
 namespace tah {
+// A unit copy whose source row is affine in the target row along every
+// dimension: ca + cs*r with cs in {-1, 0, 1}.
+bool affine_copy(const PlanGroup& g, int32_t ca[3], int32_t cs[3]);
 // Split groups into classes, upload tables, build tiles and launches.
 void build_plan_device(tah_plan* p);
 // The Job of one fill of `work` by plan p.
