@@ -87,6 +87,25 @@ def cpu_baseline(sched_builder, seconds=12.0):
                        f"{reps} fills, {dt * 1e3:.1f} ms per fill, OpenMP threads pinned by block owner")
 
 
+def traffic_of(kernel, rec, layout, nvars):
+    """HBM bytes per launch of `kernel` from the committed PMC summary of this
+    workload (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE per dispatch,
+    scaled against a copy of known size), when it covers this kernel at this
+    layout and launch size; else None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            table = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    e = table.get("kernels", {}).get(kernel)
+    if not e or table.get("layout") != layout or table.get("nvars") != nvars:
+        return None
+    if e.get("bytes_per_launch") != rec["bytes_per_launch"]:
+        return None
+    return e["hbm_bytes_per_launch"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +118,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=3)
+    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
+                    help="at the end, copy a buffer of this many GiB once (a kernel of known "
+                         "traffic, to scale HBM counters by)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path "
@@ -106,7 +128,8 @@ def main():
         sys.exit(2)
 
     import torch
-    from treeamr_hip import FieldSet, FillGraph, GhostSchedule, fill_ghosts, fill_ghosts_adjoint
+    from treeamr_hip import (FieldSet, FillGraph, GhostSchedule, fill_ghosts, fill_ghosts_adjoint,
+                             fill_ghosts_timed)
     from treeamr_hip import workloads as W
 
     def builder(nvars):
@@ -144,6 +167,26 @@ def main():
     graph = FillGraph([(fs, sched)])
     graphed = time_fill(graph.launch, args.steps, args.warmup, stream)
     log(f"[bench] graph: {graphed}")
+    log("[bench] timing every launch of the fill (in order, HIP events on the fill's stream)")
+    kernels = {}
+    for it in range(args.warmup + args.steps):
+        recs = fill_ghosts_timed(fs, sched)
+        if it < args.warmup:
+            continue
+        for name, vals, ms in recs:
+            k = kernels.setdefault(name, dict(launches=0, values=0, ms=0.0))
+            k["launches"] += 1
+            k["values"] += vals
+            k["ms"] += ms
+    for k in kernels.values():
+        k["avg_launch_ms"] = round(k["ms"] / k["launches"], 4)
+        k["bytes_per_launch"] = 16 * k["values"] // k["launches"]
+        k["GBs"] = round(16.0 * k["values"] / (k["ms"] * 1e-3) / 1e9, 1)
+        k["ms_per_fill"] = round(k["ms"] / args.steps, 4)
+        k["ms"] = round(k["ms"], 3)
+    dominant = max(kernels, key=lambda n: kernels[n]["ms"])
+    dk = kernels[dominant]
+    log(f"[bench] dominant kernel {dominant}: {dk}")
     adjoint = None
     if not args.no_adjoint:
         log("[bench] timing the adjoint fill")
@@ -225,16 +268,30 @@ def main():
         "batched": batched,
         "schedule_build_s": round(build_s, 3),
         "roofline": {
+            "kernel": dominant,
             "bound": "hbm",
-            "achieved": round(gbs, 1),
+            "achieved": dk["GBs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "frac": round(dk["GBs"] / HBM_PEAK_GBS, 4),
+            "traffic": traffic_of(dominant, dk, args.layout, nvars),
+            "algorithmic_bytes_per_launch": dk["bytes_per_launch"],
+            "avg_launch_ms": dk["avg_launch_ms"],
         },
+        "kernels": kernels,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if args.copy_probe_gib > 0:
+        del fs, graph
+        torch.cuda.synchronize()
+        n = int(args.copy_probe_gib * (1 << 30)) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: {n * 8} bytes read, {n * 8} bytes written")
 
 
 if __name__ == "__main__":

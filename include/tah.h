@@ -16,7 +16,8 @@
  * a check fails. Indices in descriptors are 0-based (the reference's are
  * 1-based: a shim subtracts one). Field data is Float64 only
  * (check_floattype, src/device.jl:53-61). Device pointers are hipMalloc'd
- * (or the arena of tah_arena_alloc); streams are hipStream_t passed as void*.
+ * by the caller, sized by tah_layout_size; streams are hipStream_t passed as
+ * void*.
  * Two fills over one working array must not overlap (src/ghosts.jl:1773).
  */
 #ifndef TAH_H
@@ -173,9 +174,10 @@ tah_status tah_schedule_points(const tah_schedule* s, int64_t* points);
 /* ------------------------------------------------------------------ */
 
 /* The working array: element (i, j, k, v, b) lives at
- * stride[0]*i + stride[1]*j + stride[2]*k + stride[3]*v + stride[4]*b,
- * stride[0] == 1. The reference's dense layout work[i,j,k,var,block]
- * (src/storage.jl:196-200) is the unpadded case. */
+ * stride[0]*i + stride[1]*j + stride[2]*k + stride[3]*v + stride[4]*b.
+ * The reference's dense layout work[i,j,k,var,block] (src/storage.jl:196-200)
+ * is strides (1, S0, S0*S1, S0*S1*S2, nvars*S0*S1*S2). Any strides under
+ * which blocks are outermost and no two elements meet are accepted. */
 typedef struct tah_layout {
   int32_t stored[3];
   int32_t nvars;
@@ -186,8 +188,10 @@ typedef struct tah_layout {
 
 /* The dense layout of a schedule's field set. */
 tah_status tah_layout_dense(const tah_schedule_desc* d, int32_t nvars, tah_layout* out);
-/* The padded arena layout (north-star (b)): rows padded so that the low
- * and high ghost slabs of each row start on 32-byte sectors. */
+/* The arena layout: variables innermost, strides (nvars, nvars*S0,
+ * nvars*S0*S1, 1, block) with every block starting on a 256-byte boundary,
+ * so that a ghost slab of any orientation is read and written in runs of at
+ * least nvars consecutive doubles. */
 tah_status tah_layout_arena(const tah_schedule_desc* d, int32_t nvars, tah_layout* out);
 /* Elements a layout spans (what to allocate, in doubles). */
 tah_status tah_layout_size(const tah_layout* l, int64_t* elements);
@@ -217,6 +221,17 @@ tah_status tah_fill_plain(tah_plan* p, double* work, const double* factors, void
 /* The staged fill: one launch per stage group over a tile list of every
  * group of its stages, class-specialised kernels. Same results, bit for bit. */
 tah_status tah_fill(tah_plan* p, double* work, const double* factors, void* stream);
+
+/* The staged fill with its launches run one after another on `stream`, each
+ * timed by HIP events (for measurement: the same kernels and results as
+ * tah_fill). Call with cls, values and ms NULL to get the launch count in
+ * *n; then with arrays of *n entries: per launch, in fill order, its kernel
+ * class, the values it writes and its duration in ms. No counterpart in the
+ * reference, which times whole fills (bench/ghosts.jl:69-78). */
+tah_status tah_fill_timed(tah_plan* p, double* work, const double* factors, void* stream,
+                          int32_t* n, int32_t* cls, int64_t* values, float* ms);
+/* The kernel name of a class, as tah_fill_timed reports them. */
+tah_status tah_kernel_class_name(int32_t cls, char* buf, int32_t len);
 
 /* Many independent fills (forests, field sets) in one launch per stage
  * position. */

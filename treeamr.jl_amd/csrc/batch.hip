@@ -29,6 +29,7 @@ struct Batch {
   std::vector<std::vector<Launch>> launches;  // per stage position
   std::vector<hipEvent_t> events;
   int device = 0;
+  bool fork = false;  // forked when any of its plans is
   ~Batch() {
     if (djobs) (void)hipFree(djobs);
     if (dtiles) (void)hipFree(dtiles);
@@ -52,6 +53,7 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
     if (p->needs_factors && !f)
       fail("plan " + std::to_string(i) + " mirrors ghosts across a reflecting face, but no factor table was given");
     b->plans.push_back(p);
+    b->fork = b->fork || p->fork != 0;
     b->works.push_back(works[i]);
     b->factors.push_back(f);
     Job j = plan_job(p, works[i], f);
@@ -98,7 +100,7 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
 }
 
 void run_batch(tah_ctx* ctx, Batch& b, hipStream_t s) {
-  for (const auto& ls : b.launches) run_launches(ctx, ls, s, b.djobs, b.dtiles, b.events.data());
+  for (const auto& ls : b.launches) run_launches(ctx, ls, s, b.djobs, b.dtiles, b.events.data(), b.fork);
 }
 
 bool same_batch(const Batch& b, int32_t n, tah_plan* const* plans, double* const* works,

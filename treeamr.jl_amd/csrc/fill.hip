@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -311,7 +312,75 @@ tah_status tah_fill(tah_plan* p, double* work, const double* factors, void* stre
         HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         p->events.push_back(e);
       }
-    for (const auto& sg : p->launches) run_launches(p->ctx, sg, s, p->djob, p->dtiles, p->events.data());
+    for (const auto& sg : p->launches)
+      run_launches(p->ctx, sg, s, p->djob, p->dtiles, p->events.data(), p->fork != 0);
+  });
+}
+
+tah_status tah_fill_timed(tah_plan* p, double* work, const double* factors, void* stream, int32_t* n,
+                          int32_t* cls, int64_t* values, float* ms) {
+  return guard([&] {
+    if (!p || !n) fail("null argument");
+    int32_t total = 0;
+    for (const auto& sg : p->launches) total += int32_t(sg.size());
+    if (!cls && !values && !ms) {
+      *n = total;
+      return;
+    }
+    if (!work || !cls || !values || !ms) fail("null argument");
+    if (*n < total) fail("the record arrays hold " + std::to_string(*n) + " launches; the fill makes " +
+                         std::to_string(total));
+    if (p->needs_factors && !factors)
+      fail("this schedule mirrors ghosts across a reflecting face, but no factor table was given");
+    HIPCHECK(hipSetDevice(p->ctx->device));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    Job job = plan_job(p, work, factors);
+    if (std::memcmp(&job, &p->last_job, sizeof(Job)) != 0) {
+      p->last_job = job;
+      HIPCHECK(hipMemcpyAsync(p->djob, &p->last_job, sizeof(Job), hipMemcpyHostToDevice, s));
+    }
+    std::vector<hipEvent_t> ev(size_t(total) + 1);
+    for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
+    try {
+      int32_t k = 0;
+      HIPCHECK(hipEventRecord(ev[0], s));
+      for (const auto& sg : p->launches)
+        for (const Launch& L : sg) {
+          launch_class(L.cls, uint32_t(L.count), s, p->djob, p->dtiles + L.first);
+          HIPCHECK(hipEventRecord(ev[k + 1], s));
+          cls[k] = L.cls;
+          values[k] = L.values;
+          ++k;
+        }
+      HIPCHECK(hipEventSynchronize(ev[total]));
+      for (int32_t i = 0; i < total; ++i) HIPCHECK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+    } catch (...) {
+      for (auto& e : ev) (void)hipEventDestroy(e);
+      throw;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    *n = total;
+  });
+}
+
+tah_status tah_kernel_class_name(int32_t cls, char* buf, int32_t len) {
+  return guard([&] {
+    if (!buf || len < 1) fail("null argument");
+    std::string name;
+    if (cls == kClsGeneric) {
+      name = "generic_kernel";
+    } else if (cls == kClsCopy) {
+      name = "copy_kernel";
+    } else if (cls >= kClsInterp && (cls - kClsInterp) / 8 < int(kInterpTable.size())) {
+      const int id = cls - kClsInterp, rb = id % 8;
+      const auto& e = kInterpTable[id / 8];
+      name = "interp_kernel<" + std::to_string(e[0]) + ", " + std::to_string(e[1]) + ", " +
+             std::to_string(1 + (rb & 1)) + ", " + std::to_string(1 + ((rb >> 1) & 1)) + ", " +
+             std::to_string(1 + ((rb >> 2) & 1)) + ">";
+    } else {
+      fail("no kernel class " + std::to_string(cls));
+    }
+    std::snprintf(buf, size_t(len), "%s", name.c_str());
   });
 }
 

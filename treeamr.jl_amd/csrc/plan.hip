@@ -336,6 +336,10 @@ void build_plan_device(tah_plan* p) {
     for (int cls : classes) {
       std::vector<std::pair<int32_t, Tile>> tl;
       int64_t per = class_tile_elems(cls);
+      int64_t values = 0;
+      // targets per interpolation unit: 2 along every dimension with R = 2
+      const int rb = cls >= kClsInterp ? (cls - kClsInterp) % 8 : 0;
+      const int64_t per_unit = (1 + (rb & 1)) * (1 + ((rb >> 1) & 1)) * (1 + ((rb >> 2) & 1));
       for (size_t gi = 0; gi < p->groups.size(); ++gi) {
         const PlanGroup& g = p->groups[gi];
         if (g.points == 0) continue;
@@ -355,9 +359,11 @@ void build_plan_device(tah_plan* p) {
             for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t) {
               if (run_in[gi][t] != k) continue;
               for (int32_t v0 = 0; v0 < nvars; v0 += nvt)
-                for (int32_t u0 = 0; u0 < nun; u0 += nu)
+                for (int32_t u0 = 0; u0 < nun; u0 += nu) {
                   tl.push_back({g.tblocks[t], Tile{0, ci, t, u0, std::min(nu, nun - u0), v0,
                                                    std::min(nvt, nvars - v0), 0}});
+                  values += int64_t(std::min(nu, nun - u0)) * std::min(nvt, nvars - v0) * per_unit;
+                }
             }
           }
         } else if (g.cls == cls) {
@@ -366,6 +372,7 @@ void build_plan_device(tah_plan* p) {
             if (run_in[gi][t] != k) continue;
             for (int64_t e = 0; e < elems; e += per)
               tl.push_back({g.tblocks[t], Tile{0, int32_t(gi), t, int32_t(e), 0, 0, 0, 0}});
+            values += elems;
           }
         }
       }
@@ -373,7 +380,7 @@ void build_plan_device(tah_plan* p) {
                        [](const std::pair<int32_t, Tile>& x, const std::pair<int32_t, Tile>& y) {
                          return x.first < y.first;
                        });
-      Launch L{cls, int64_t(p->tiles.size()), int64_t(tl.size())};
+      Launch L{cls, int64_t(p->tiles.size()), int64_t(tl.size()), values};
       for (const auto& e : tl) p->tiles.push_back(e.second);
       if (L.count > 0) ls.push_back(L);
     }
@@ -416,8 +423,8 @@ const std::vector<hipStream_t>& ctx_streams(tah_ctx* c) {
 }
 
 void run_launches(tah_ctx* c, const std::vector<Launch>& ls, hipStream_t s, const Job* jobs,
-                  const Tile* tiles, hipEvent_t* ev) {
-  if (ls.size() <= 1) {
+                  const Tile* tiles, hipEvent_t* ev, bool fork) {
+  if (!fork || ls.size() <= 1) {
     for (const Launch& L : ls) launch_class(L.cls, uint32_t(L.count), s, jobs, tiles + L.first);
     return;
   }

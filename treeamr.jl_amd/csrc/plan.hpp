@@ -66,6 +66,7 @@ struct PlanWClass {
 struct Launch {
   int cls;
   int64_t first, count;
+  int64_t values = 0;  // values the launch writes
 };
 
 struct AdjointTables;
@@ -95,6 +96,10 @@ struct tah_plan {
   tah::Job last_job{};
   std::vector<std::vector<tah::Launch>> launches;  // per stage group
   int64_t deferred = 0;            // transfers moved to run with the last stage group
+  // Run a stage group's launches forked over the context's streams (1) or
+  // one after another on the caller's stream (0, the default; tah_tune
+  // picks per plan).
+  int32_t fork = 0;
   std::vector<hipEvent_t> events;  // fork/join events, created on first use
   // The adjoint's inverted schedule, built on first use (adjoint.hip).
   tah::AdjointTables* adj = nullptr;
@@ -114,5 +119,5 @@ const std::vector<hipStream_t>& ctx_streams(tah_ctx* c);
 // Run a stage group's launches forked over the context's streams and
 // joined back into `s`. ev must hold launches.size() + 1 events.
 void run_launches(tah_ctx* c, const std::vector<Launch>& ls, hipStream_t s, const Job* jobs,
-                  const Tile* tiles, hipEvent_t* ev);
+                  const Tile* tiles, hipEvent_t* ev, bool fork);
 }  // namespace tah

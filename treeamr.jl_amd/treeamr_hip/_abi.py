@@ -80,6 +80,8 @@ SIGNATURES = {
     "tah_graph_create": [P, C.c_int32, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P)],
     "tah_graph_launch": [P, P],
     "tah_graph_destroy": [P],
+    "tah_fill_timed": [P, P, P, P, I32P, I32P, C.POINTER(C.c_int64), C.POINTER(C.c_float)],
+    "tah_kernel_class_name": [C.c_int32, C.c_char_p, C.c_int32],
     "tah_fill_adjoint": [P, P, P, P],
     "tah_tune": [P, P, P, C.c_char_p],
 }

@@ -465,6 +465,31 @@ def fill_ghosts(fs, schedule, *, boundary=None, method="staged", ctx=None):
     return fs
 
 
+def kernel_class_name(cls):
+    buf = C.create_string_buffer(96)
+    check(lib().tah_kernel_class_name(int(cls), buf, 96))
+    return buf.value.decode()
+
+
+def fill_ghosts_timed(fs, schedule, *, ctx=None):
+    """The staged fill with its launches run in order, each timed by HIP
+    events on fs's stream: [(kernel name, values written, ms)] in fill order."""
+    _check_fill(fs, schedule)
+    if fs.work.device.type != "cuda":
+        raise RuntimeError("fill_ghosts_timed runs on the GPU only")
+    plan = schedule.plan(fs.layout, ctx)
+    fptr = C.c_void_p(fs.factors.data_ptr()) if fs.factors is not None else None
+    n = C.c_int32()
+    work = C.c_void_p(fs.work.data_ptr())
+    check(lib().tah_fill_timed(plan._h, work, fptr, _stream_ptr(fs), C.byref(n), None, None, None))
+    k = n.value
+    cls = (C.c_int32 * max(k, 1))()
+    vals = (C.c_int64 * max(k, 1))()
+    ms = (C.c_float * max(k, 1))()
+    check(lib().tah_fill_timed(plan._h, work, fptr, _stream_ptr(fs), C.byref(n), cls, vals, ms))
+    return [(kernel_class_name(cls[i]), int(vals[i]), float(ms[i])) for i in range(n.value)]
+
+
 def fill_ghosts_adjoint(fs, schedule, adj, *, ctx=None):
     """The transpose of the fill applied in place to adj (a tensor shaped like fs.work)."""
     _check_fill(fs, schedule)
