@@ -89,10 +89,10 @@ def cpu_baseline(sched_builder, seconds=12.0):
 
 def traffic_of(kernel, rec, layout, nvars):
     """HBM bytes per launch of `kernel` from the committed PMC summary of this
-    workload (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE per dispatch,
+    workload (profiles/hbm_traffic_large_forest_arena.json: FETCH_SIZE and WRITE_SIZE per dispatch,
     scaled against a copy of known size), when it covers this kernel at this
     layout and launch size; else None."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_large_forest_arena.json")
     try:
         with open(path) as fh:
             table = json.load(fh)
