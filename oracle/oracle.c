@@ -22,8 +22,9 @@
  * Built with -ffp-contract=off so that no product and sum fuse.
  *
  * Also: oracle_fill_adjoint, the transpose of the same linear map assembled
- * on the CPU from the schedule (stages reversed; within a stage every
- * source accumulates w*f*adj[target], then the targets are zeroed). It has
+ * on the CPU from the schedule (stages reversed; within a stage the targets
+ * are read and zeroed, then every source accumulates w*f*adj[target] from
+ * the values read, so that a transfer onto itself -- stage P -- scales). It has
  * no counterpart in the reference (which has no autodiff); it is pinned by
  * the dot-product identity <fill(x), y> = <x, fill^T(y)> in tests/.
  */
@@ -152,21 +153,53 @@ int oracle_fill_adjoint(const tah_schedule_desc* d, const tah_layout* L, double*
   if (!d || !L || !adj) return 1;
   const int64_t* st = L->stride;
   for (int32_t stage = d->nstages - 1; stage >= 0; --stage) {
+    /* The stage's target values before it (a transfer of stage P reads its
+     * own target), then the targets zeroed: they were overwritten. */
+    int64_t total = 0;
     for (int32_t gi = 0; gi < d->ngroups; ++gi) {
       const tah_group_desc* g = &d->groups[gi];
       if (g->stage != stage) continue;
       if (g->factorcol && !factors) return 2;
       const tah_stencil1d* S = g->stencils;
+      total += (int64_t)g->ntransfers * L->nvars * S[0].ntarget * S[1].ntarget * S[2].ntarget;
+    }
+    double* old = (double*)malloc(sizeof(double) * (size_t)(total > 0 ? total : 1));
+    if (!old) return 3;
+    int64_t n = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      n = 0;
+      for (int32_t gi = 0; gi < d->ngroups; ++gi) {
+        const tah_group_desc* g = &d->groups[gi];
+        if (g->stage != stage) continue;
+        const tah_stencil1d* S = g->stencils;
+        for (int32_t t = 0; t < g->ntransfers; ++t) {
+          double* tb = adj + st[4] * g->targetblocks[t];
+          for (int32_t v = 0; v < L->nvars; ++v)
+            for (int32_t k = 0; k < S[2].ntarget; ++k)
+              for (int32_t j = 0; j < S[1].ntarget; ++j)
+                for (int32_t i = 0; i < S[0].ntarget; ++i) {
+                  double* x = tb + st[0] * (S[0].targetfirst + i) + st[1] * (S[1].targetfirst + j) +
+                              st[2] * (S[2].targetfirst + k) + st[3] * v;
+                  if (pass == 0) old[n] = *x; else *x = 0.0;
+                  ++n;
+                }
+        }
+      }
+    }
+    /* Every source accumulates w*f*adj[target], from the values before. */
+    n = 0;
+    for (int32_t gi = 0; gi < d->ngroups; ++gi) {
+      const tah_group_desc* g = &d->groups[gi];
+      if (g->stage != stage) continue;
+      const tah_stencil1d* S = g->stencils;
       for (int32_t t = 0; t < g->ntransfers; ++t) {
-        double* tb = adj + st[4] * g->targetblocks[t];
         double* sb = adj + st[4] * g->sourceblocks[t];
         for (int32_t v = 0; v < L->nvars; ++v) {
           double f = g->factorcol ? factors[v + (int64_t)L->nvars * (g->factorcol - 1)] : 1.0;
           for (int32_t k = 0; k < S[2].ntarget; ++k)
             for (int32_t j = 0; j < S[1].ntarget; ++j)
               for (int32_t i = 0; i < S[0].ntarget; ++i) {
-                double a = tb[st[0] * (S[0].targetfirst + i) + st[1] * (S[1].targetfirst + j) +
-                              st[2] * (S[2].targetfirst + k) + st[3] * v] * f;
+                double a = old[n++] * f;
                 const double* w0 = S[0].weights + (int64_t)S[0].order * i;
                 const double* w1 = S[1].weights + (int64_t)S[1].order * j;
                 const double* w2 = S[2].weights + (int64_t)S[2].order * k;
@@ -181,21 +214,7 @@ int oracle_fill_adjoint(const tah_schedule_desc* d, const tah_layout* L, double*
         }
       }
     }
-    /* Then every target of the stage is zeroed: it was overwritten. */
-    for (int32_t gi = 0; gi < d->ngroups; ++gi) {
-      const tah_group_desc* g = &d->groups[gi];
-      if (g->stage != stage) continue;
-      const tah_stencil1d* S = g->stencils;
-      for (int32_t t = 0; t < g->ntransfers; ++t) {
-        double* tb = adj + st[4] * g->targetblocks[t];
-        for (int32_t v = 0; v < L->nvars; ++v)
-          for (int32_t k = 0; k < S[2].ntarget; ++k)
-            for (int32_t j = 0; j < S[1].ntarget; ++j)
-              for (int32_t i = 0; i < S[0].ntarget; ++i)
-                tb[st[0] * (S[0].targetfirst + i) + st[1] * (S[1].targetfirst + j) +
-                   st[2] * (S[2].targetfirst + k) + st[3] * v] = 0.0;
-      }
-    }
+    free(old);
   }
   return 0;
 }

@@ -21,6 +21,8 @@ pytestmark = pytest.mark.gpu
 def _setup(which="config1", layout="dense"):
     if which == "config1":
         f, ops, kw, _ = W.config1()
+    elif which == "vertex":
+        f, ops, kw, _ = W.config2(roots=2, N=8, radii=(0.5, 0.3))
     else:
         f, ops, kw, _ = W.config5(nvars=3, roots=2, N=16)
     kw = dict(kw)
@@ -30,7 +32,8 @@ def _setup(which="config1", layout="dense"):
 
 
 @pytest.mark.parametrize("which,layout", [("config1", "dense"), ("config1", "arena"),
-                                          ("reflecting", "dense"), ("reflecting", "arena")])
+                                          ("reflecting", "dense"), ("reflecting", "arena"),
+                                          ("vertex", "dense"), ("vertex", "arena")])
 def test_dot_product_identity(which, layout):
     fs, s = _setup(which, layout)
     g = torch.Generator(device="cuda")
@@ -49,7 +52,7 @@ def test_dot_product_identity(which, layout):
     assert abs(lhs - rhs) <= 1e-13 * max(abs(lhs), 1.0), (lhs, rhs)
 
 
-@pytest.mark.parametrize("which", ["config1", "reflecting"])
+@pytest.mark.parametrize("which", ["config1", "reflecting", "vertex"])
 def test_adjoint_matches_cpu_transpose(which):
     fs, s = _setup(which)
     g = torch.Generator(device="cuda")

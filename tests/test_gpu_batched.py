@@ -1,6 +1,7 @@
 """Batched fills of many independent forests (the 64-forest configuration,
-smaller draws here) and their hipGraph form: bit for bit the per-forest
-fills, graph replays checked one by one with eager work in between."""
+smaller draws here, cell and vertex centering) and their hipGraph form: bit
+for bit the per-forest fills, graph replays checked one by one with eager work
+in between."""
 import pytest
 import torch
 
@@ -13,10 +14,10 @@ pytestmark = pytest.mark.gpu
 def _forests(seeds, max_blocks=300):
     out = []
     for sd in seeds:
-        f, ops, G, nvars = W.random_forest(sd, max_blocks=max_blocks)
+        f, ops, G, nvars, centering = W.random_forest(sd, max_blocks=max_blocks)
         for k, nv in enumerate(nvars):
             layout = "arena" if (sd + k) % 2 else "dense"
-            fs = FieldSet(f, nv, G=G, device="cuda", layout=layout, fill=float("nan"))
+            fs = FieldSet(f, nv, G=G, centering=centering, device="cuda", layout=layout, fill=float("nan"))
             W.fill_uniform(fs, 100 * sd + k)
             out.append((fs, GhostSchedule(fs, ops)))
     return out

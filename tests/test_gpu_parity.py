@@ -66,11 +66,28 @@ def test_config5_small_bit_exact(method, layout):
     _check(fs, sched, method)
 
 
-@pytest.mark.parametrize("seed", [2, 3, 4, 5])
+@pytest.mark.parametrize("seed", [2, 3, 4, 5, 6, 12])
 def test_random_forests_bit_exact(seed):
-    f, ops, G, nvars = W.random_forest(seed, max_blocks=600)
-    fs, sched = _prepare(f, ops, dict(nvars=nvars[0], G=G), seed=seed)
+    """Forests of the batched configuration, cell (seeds 2, 4, 5) and vertex
+    (3, 6, 12) centering."""
+    f, ops, G, nvars, centering = W.random_forest(seed, max_blocks=600)
+    fs, sched = _prepare(f, ops, dict(nvars=nvars[0], G=G, centering=centering), seed=seed)
     _check(fs, sched, "staged")
+
+
+@pytest.mark.parametrize("method", ["plain", "staged"])
+@pytest.mark.parametrize("layout", ["dense", "arena"])
+def test_config2_vertex_bit_exact(method, layout):
+    """The nested vertex-centred mesh (232 blocks of 16^3, G=3, 8 variables,
+    PointValue(6,6), reflecting z-low with odd variables 7-8, outflow
+    elsewhere): stage P, stage 0, phase 1 and three prolongation levels."""
+    f, ops, kw, _ = W.config2()
+    kw = dict(kw)
+    nvars = kw.pop("nvars")
+    fs = FieldSet(f, nvars, device="cuda", layout=layout, fill=float("nan"), **kw)
+    W.fill_waves(fs, 1)
+    sched = GhostSchedule(fs, ops)
+    _check(fs, sched, method)
 
 
 def test_config5_full_mesh_one_variable():

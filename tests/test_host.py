@@ -15,8 +15,8 @@ import numpy as np
 import pytest
 
 import oracle
-from treeamr_hip import (Conservative, EvenParity, FieldSet, Forest, GhostSchedule, OddParity,
-                         Operators, StaleScheduleError, UnsupportedError)
+from treeamr_hip import (Conservative, EvenParity, FieldSet, Forest, GhostSchedule, NoParity,
+                         OddParity, Operators, StaleScheduleError, UnsupportedError)
 from treeamr_hip import _abi
 from treeamr_hip import workloads as W
 
@@ -104,12 +104,9 @@ def test_check_operators_against_the_layout():
         FieldSet(f, 1, G=5, device="cpu")
 
 
-def test_bounded_and_vertex_refused_as_unsupported():
+def test_bounded_refused_as_unsupported():
     with pytest.raises(UnsupportedError):
         Operators(prolongation=4, restriction=4, bounded=True)
-    f = Forest((2, 2, 2), N=8, periodic=(True,) * 3)
-    with pytest.raises(UnsupportedError):
-        GhostSchedule(f, Operators(prolongation=4, restriction=4), G=2, centering="vertex")
 
 
 # --- the forest (test/tree_tests.jl) ---------------------------------------
@@ -165,8 +162,9 @@ def test_every_ghost_written_exactly_once(build):
     elif build == "config5_small":
         f, ops, kw, _ = W.config5(nvars=2, roots=4, N=16)
     else:
-        f, ops, G, nv = W.random_forest(7, max_blocks=400)
-        kw = dict(nvars=nv[0], G=G)
+        f, ops, G, nv, centering = W.random_forest(2, max_blocks=400)
+        assert centering == "cell"
+        kw = dict(nvars=nv[0], G=G, centering=centering)
     kw = dict(kw)
     fs = FieldSet(f, kw.pop("nvars"), device="cpu", **kw)
     s = GhostSchedule(fs, ops)
@@ -175,6 +173,166 @@ def test_every_ghost_written_exactly_once(build):
     assert wc[own].max() == 0
     assert wc[~own].min() == 1 and wc[~own].max() == 1
     assert s.points() == int(wc.sum())
+
+
+# --- closed blocks (vertex centering): stage 0, stage P, ghost-plane parts ----
+
+def _positions(fs):
+    """Integer position of every stored point, in units of the finest
+    spacing, wrapped along periodic dimensions: (nblocks, S2, S1, S0, 3).
+    The forest spans the unit cube."""
+    lows, h, level = fs.forest.block_geometry()
+    scale = 1.0 / h.min()
+    top = int(round(scale))
+    S, G = fs.stored, fs.G
+    out = np.zeros((fs.nblocks, S[2], S[1], S[0], 3), np.int64)
+    for d in range(3):
+        step = np.rint(h * scale).astype(np.int64)
+        base = np.rint(lows[:, d] * scale).astype(np.int64)
+        idx = np.arange(S[d]) - G[d]
+        x = base[:, None] + idx[None, :] * step[:, None]
+        if fs.forest.periodic[d]:
+            x = np.mod(x, top)
+        shape = [fs.nblocks, 1, 1, 1]
+        shape[3 - d] = S[d]
+        out[..., d] = x.reshape(shape)
+    return out
+
+
+def _vertex_case(which):
+    if which == "config2":
+        f, ops, kw, _ = W.config2()
+    elif which == "config2_small":
+        f, ops, kw, _ = W.config2(roots=2, N=8, radii=(0.5, 0.3))
+    else:
+        f, ops, G, nv, centering = W.random_forest(int(which[6:]), max_blocks=300)
+        assert centering == "vertex"
+        kw = dict(nvars=nv[0], G=G, centering="vertex")
+    kw = dict(kw)
+    fs = FieldSet(f, kw.pop("nvars"), device="cpu", fill=float("nan"), **kw)
+    return fs, GhostSchedule(fs, ops)
+
+
+@pytest.mark.parametrize("which", ["config2", "random3", "random12"])
+def test_vertex_ghosts_written_once_own_points_at_most_once(which):
+    """write_counts (test/ghost_oracles.jl:228-255) on closed blocks: every
+    ghost once; an own point at most once (stage 0 writes secondaries, stage
+    P primaries on a reflecting wall, never both)."""
+    fs, s = _vertex_case(which)
+    wc = oracle.write_counts(s._descptr, fs.layout)
+    own = _ownmask(fs)
+    assert wc[~own].min() == 1 and wc[~own].max() == 1
+    assert wc[own].max() == 1 and wc[own].sum() > 0
+    assert s.points() == int(wc.sum())
+    tags = s.stage_tags()
+    assert 0 in tags and 1 in tags
+
+
+@pytest.mark.parametrize("which", ["config2", "random3", "random12"])
+def test_vertex_one_value_per_point_and_injection(which):
+    """After a fill of per-copy noisy data (M22 rule 1): every stored copy of
+    an own point holds one value, the value a copy on the finest level held
+    before; and every ghost at a point some block of its level or a finer one
+    owns holds that value bit for bit (injection_report,
+    test/ghost_oracles.jl:186-220)."""
+    fs, s = _vertex_case(which)
+    rng = np.random.default_rng(5)
+    a = fs.array().numpy()
+    own = (slice(None), slice(None)) + fs.own_slices()
+    a[own] = rng.uniform(-1, 1, size=a[own].shape)
+    before = a.copy()
+    fac = fs.factors.numpy().copy() if fs.factors is not None else None
+    oracle.fill(s._descptr, fs.layout, fs.work.numpy(), fac, nthreads=8)
+    _, _, level = fs.forest.block_geometry()
+    pos = _positions(fs)
+    nb, nv = fs.nblocks, fs.nvars
+    ownm = _ownmask(fs)
+    # own copies: group by position
+    bidx = np.broadcast_to(np.arange(nb)[:, None, None, None], ownm.shape)
+    P = pos[ownm]
+    B = bidx[ownm]
+    vals = np.moveaxis(a, 1, -1)[ownm]           # (npoints, nv)
+    vals0 = np.moveaxis(before, 1, -1)[ownm]
+    keys, inv = np.unique(P, axis=0, return_inverse=True)
+    inv = inv.ravel()
+    order = np.argsort(inv, kind="stable")
+    first = np.zeros(len(keys), np.int64)
+    first[inv[order[::-1]]] = order[::-1]      # some copy of each point
+    ref = vals[first]
+    assert np.array_equal(vals, ref[inv]), "copies of a point differ after the fill"
+    # the value is one a finest copy held before the fill, projected (stage P)
+    # onto its parity on a reflecting wall plane: times (1 + sigma)/2 there
+    lv = level[B]
+    maxlv = np.full(len(keys), -1)
+    np.maximum.at(maxlv, inv, lv)
+    fine = lv == maxlv[inv]
+    proj = np.ones((len(P), nv))
+    if fs.parity is not None:
+        top = np.rint(1.0 / fs.forest.block_geometry()[1].min()).astype(np.int64)
+        for d in range(3):
+            for side, at in ((0, P[:, d] == 0), (1, P[:, d] == top)):
+                if not fs.forest.reflecting[d][side]:
+                    continue
+                sig = np.array([-1.0 if fs.parity[v][d] == OddParity else 1.0 for v in range(nv)])
+                proj[at] *= (1 + sig) / 2
+    hit = np.zeros(len(keys), bool)
+    np.logical_or.at(hit, inv, fine & np.all(vals0 * proj + 0.0 == ref[inv], axis=1))
+    assert hit.all(), "a point's value is not one of its finest copies' values"
+    # ghosts at a point owned on their level or a finer one
+    lookup = {tuple(k): i for i, k in enumerate(keys)}
+    ghost = ~ownm
+    G_pos = pos[ghost]
+    G_blk = bidx[ghost]
+    G_val = np.moveaxis(a, 1, -1)[ghost]
+    checked = 0
+    for p_, b_, v_ in zip(G_pos, G_blk, G_val):
+        i = lookup.get(tuple(p_))
+        if i is None or maxlv[i] < level[b_]:
+            continue
+        checked += 1
+        assert np.array_equal(v_, ref[i]), (b_, p_)
+    assert checked > 0
+
+
+def test_vertex_wall_point_even_kept_odd_zero():
+    """The wall point of a reflecting vertex-like dimension is the state's own
+    for an even variable, bit for bit, and zero for an odd one (stage P;
+    test/reflect_tests.jl:132-154, in 3D with a refined corner)."""
+    f = Forest((1, 1, 1), N=8, extents=((0, 1),) * 3, periodic=(False, True, True),
+               reflecting=((True, True), (False, False), (False, False)))
+    f.refine([0])
+    f.balance()
+    par = ((EvenParity, NoParity, NoParity), (OddParity, NoParity, NoParity), (EvenParity, NoParity, NoParity))
+    fs = FieldSet(f, 3, G=1, centering="vertex", parity=par, device="cpu", fill=float("nan"))
+    s = GhostSchedule(fs, Operators(prolongation=4, restriction=4))
+    X = fs.coordinates()
+    a = fs.array().numpy()
+    fns = [lambda x: 1 + (x[0] - 1) ** 2, lambda x: 1 + x[0], lambda x: 1 + (x[0] - 1) ** 4]
+    for v, fn in enumerate(fns):
+        a[(slice(None), v) + fs.own_slices()] = fn(X)[(slice(None),) + fs.own_slices()]
+    before = a.copy()
+    oracle.fill(s._descptr, fs.layout, fs.work.numpy(), fs.factors.numpy().copy())
+    G, N = fs.G[0], f.N
+    lows, h, _ = f.block_geometry()
+    for b in range(f.nleaves):
+        for wall, at_wall in ((G, lows[b, 0] == 0.0), (G + N, lows[b, 0] + N * h[b] == 1.0)):
+            if not at_wall:
+                continue
+            sl = (b, slice(None), slice(G, G + N + 1), slice(G, G + N + 1), wall)
+            np.testing.assert_array_equal(a[sl][0], before[sl][0])
+            assert (a[sl][1] == 0.0).all()
+            np.testing.assert_array_equal(a[sl][2], before[sl][2])
+
+
+@pytest.mark.parametrize("p", [2, 4, 6])
+def test_vertex_pointvalue_polynomial_exactness(p):
+    """On closed blocks, order p reproduces degree p-1 everywhere (prolongation
+    at coarse nodes and midpoints, injection, stage-0 copies) and not degree p."""
+    f = _nested(N=8)
+    ops = Operators(prolongation=p, restriction=p)
+    G = max(1, p // 2)
+    assert _exchange_error(f, ops, _poly(p - 1), G=G, centering="vertex") < 1e-12
+    assert _exchange_error(f, ops, _poly(p), G=G, centering="vertex") > 1e-10
 
 
 def _nested(N, refl=None, outf=None, periodic=False):
@@ -192,11 +350,11 @@ def _poly(deg):
                             for d in range(3) for e in range(deg + 1))
 
 
-def _exchange_error(f, ops, fn, G, nvars=2):
+def _exchange_error(f, ops, fn, G, nvars=2, centering=None):
     """exchange_error (test/ghost_oracles.jl:111-117): own points and the
     outer-face regions hold fn, the fill computes every other ghost; the
     worst deviation from fn anywhere."""
-    fs = FieldSet(f, nvars, G=G, device="cpu", fill=float("nan"))
+    fs = FieldSet(f, nvars, G=G, centering=centering, device="cpu", fill=float("nan"))
     s = GhostSchedule(fs, ops)
     X = fs.coordinates()
     a = fs.array().numpy()
@@ -365,21 +523,30 @@ def test_oracle_one_tap_skip_equals_full_sum_on_finite_data():
     assert np.array_equal(w1.view(np.int64), w2.view(np.int64))
 
 
-def test_oracle_adjoint_dot_product():
-    """<fill(x), y> = <x, fill^T(y)> for the CPU transpose, with x own-only."""
-    f, ops, kw, _ = W.config1()
+@pytest.mark.parametrize("which", ["config1", "vertex"])
+def test_oracle_adjoint_dot_product(which):
+    """<fill(x), y> = <x, fill^T(y)> for the CPU transpose, with x own-only;
+    on closed blocks with a reflecting wall too (stage P scales a point onto
+    itself, stage 0 copies own points)."""
+    if which == "config1":
+        f, ops, kw, _ = W.config1()
+    else:
+        f, ops, kw, _ = W.config2(roots=2, N=8, radii=(0.5, 0.3))
     kw = dict(kw)
     fs = FieldSet(f, kw.pop("nvars"), device="cpu", **kw)
     s = GhostSchedule(fs, ops)
+    fac = fs.factors.numpy().copy() if fs.factors is not None else None
+    if which == "vertex":
+        assert s.stage_tags()[:2] == [-1, 0]
     rng = np.random.default_rng(0)
     own = np.zeros(fs.array().shape, bool)
     own[(slice(None), slice(None)) + fs.own_slices()] = True
     x = np.where(own, rng.uniform(-1, 1, own.shape), 0.0).ravel()
     y = rng.uniform(-1, 1, own.shape).ravel()
     fx = x.copy()
-    oracle.fill(s._descptr, fs.layout, fx, None)
+    oracle.fill(s._descptr, fs.layout, fx, fac)
     aty = y.copy()
-    oracle.fill_adjoint(s._descptr, fs.layout, aty, None)
+    oracle.fill_adjoint(s._descptr, fs.layout, aty, fac)
     lhs = float(fx @ y)
     rhs = float(x @ aty)
     assert abs(lhs - rhs) <= 1e-13 * max(abs(lhs), 1.0)

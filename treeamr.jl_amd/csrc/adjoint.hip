@@ -81,11 +81,14 @@ struct alignas(16) AdjEntry {
 };
 
 // A box of one source block whose points are all read by the same
-// transfers: entries[first .. first+count-1].
+// transfers: entries[first .. first+count-1]. `target`: the points are also
+// targets of the stage, read by a transfer onto themselves (stage P's
+// projection), so they start from zero rather than from their own adjoint.
 struct alignas(16) AdjCell {
   int32_t sblock, first, count, npts;
   int32_t lo[3], ext[3];
   FastDiv dx[2];  // division by ext[0], ext[1]
+  int32_t target, pad[3];
 };
 
 // A workgroup's share: kAdjTile values of cell (or pass item) `item` from e0.
@@ -216,7 +219,7 @@ __global__ void __launch_bounds__(kThreads) adjoint_gather_kernel(const Job* __r
     p[k][1] = c.lo[1] + int32_t(q - zq * uint32_t(c.ext[1]));
     p[k][2] = c.lo[2] + int32_t(zq);
     off[k] = s3 * var[k] + s0 * p[k][0] + s1 * p[k][1] + s2 * p[k][2];
-    tot[k] = ok[k] ? sb[off[k]] : 0.0;
+    tot[k] = ok[k] && !c.target ? sb[off[k]] : 0.0;
   }
   const int32_t first = uniform(c.first), count = uniform(c.count);
   for (int32_t q = 0; q < count; ++q) {
@@ -281,6 +284,7 @@ struct Box {
 struct Reader {
   int32_t group, transfer;
   AdjEntry entry;
+  bool self;  // a copy of every point onto itself
 };
 
 // Cut the read region of one source block (the boxes of the groups that
@@ -304,11 +308,14 @@ void cut_cells(int32_t sblock, const std::vector<Reader>& readers, const std::ve
         const int32_t lo[3] = {cuts[0][ix], cuts[1][iy], cuts[2][iz]};
         const int32_t hi[3] = {cuts[0][ix + 1], cuts[1][iy + 1], cuts[2][iz + 1]};
         const int32_t first = int32_t(entries.size());
+        bool target = false;
         for (const auto& r : readers) {
           const Box& b = boxes[r.group];
           bool in = true;
           for (int d = 0; d < 3; ++d) in = in && b.lo[d] <= lo[d] && hi[d] <= b.hi[d];
-          if (in) entries.push_back(r.entry);
+          if (!in) continue;
+          entries.push_back(r.entry);
+          target = target || r.self;
         }
         const int32_t count = int32_t(entries.size()) - first;
         if (count == 0) continue;
@@ -323,6 +330,7 @@ void cut_cells(int32_t sblock, const std::vector<Reader>& readers, const std::ve
         c.npts = c.ext[0] * c.ext[1] * c.ext[2];
         c.dx[0] = FastDiv(uint32_t(c.ext[0]));
         c.dx[1] = FastDiv(uint32_t(c.ext[1]));
+        c.target = target ? 1 : 0;
         cells.push_back(c);
       }
 }
@@ -487,11 +495,13 @@ void build_adjoint(tah_plan* p) {
         const int32_t gi = es[j][1], t = es[j][2];
         const PlanGroup& g = p->groups[gi];
         const Box& b = boxes[gi];
-        Reader R{gi, t, AdjEntry{}};
+        Reader R{gi, t, AdjEntry{}, false};
         AdjEntry& E = R.entry;
         E.factorcol = g.factorcol;
         int32_t ca[3], cs[3];
         if (affine_copy(g, ca, cs)) {
+          R.self = g.tblocks[t] == g.sblocks[t];
+          for (int d = 0; d < 3; ++d) R.self = R.self && ca[d] == g.tfirst[d] && (cs[d] == 1 || g.n[d] == 1);
           // target row r = (p - ca)*cs along a stepped dimension; all rows
           // of a clamped one
           E.buf = 0;

@@ -279,16 +279,500 @@ struct Builder {
   }
 
   void record(int kind, const Dir& dir, const Dir& offset, int lvl,
-              const std::array<int8_t, kMaxDim>& mirror, int b, int s) {
+              const std::array<int8_t, kMaxDim>& mirror, int b, int s,
+              const std::array<int8_t, kMaxDim>& span = {-1, -1, -1}) {
     GroupKey key;
     key.kind = kind;
     key.direction = dir;
     key.offset = offset;
     key.level = lvl;
     key.mirror = mirror;
+    key.span = span;
     auto& e = pairs[key];
     e.first.push_back(b);
     e.second.push_back(s);
+  }
+
+  bool closed() const {
+    for (int d = 0; d < D; ++d)
+      if (c[d] == 1) return true;
+    return false;
+  }
+
+  // A coarser neighbour's ghosts, recorded whole or split by
+  // ghost_plane_parts into prolongation spans and copies.
+  void record_prolong(int b, const Dir& delta, const Dir& look, int src,
+                      const std::array<int8_t, kMaxDim>& mirror) {
+    const Key& k = F.leaves[b];
+    const Dir zero{0, 0, 0};
+    std::vector<PlanePart> parts;
+    if (!closed() || !ghost_plane_parts(b, delta, look, src, parts)) {
+      record(KindProlong, delta, childoffset(k), k.level, mirror, b, src);
+      return;
+    }
+    for (const PlanePart& pp : parts) {
+      if (pp.kind == KindProlong)
+        record(KindProlong, delta, childoffset(k), k.level, mirror, b, pp.source, pp.span);
+      else
+        record(KindCopy, pp.dir, zero, 0, mirror, b, pp.source, pp.span);
+    }
+  }
+
+  // --- Closed blocks: ghost-plane parts, stage 0 and stage P ----------
+
+  struct PlanePart {
+    int kind;
+    Dir dir;
+    std::array<int8_t, kMaxDim> span;
+    int source;
+  };
+
+  // src/schedule.jl:1434-1507. The ghosts of leaf b in direction delta,
+  // whose neighbour there (global leaf `coarse`) is coarser: along each
+  // tangential vertex-like dimension the face-plane rows that a block of
+  // b's level holds (found across `look` plus steps along the plane
+  // dimensions) are copies from it; the rest stays a prolongation. False
+  // where nothing splits.
+  bool ghost_plane_parts(int b, const Dir& delta, const Dir& look, int coarse,
+                         std::vector<PlanePart>& parts) const {
+    const Key& k = F.leaves[b];
+    std::vector<int> tdims;
+    for (int d = 0; d < D; ++d)
+      if (delta[d] == 0 && c[d] == 1) tdims.push_back(d);
+    if (tdims.empty()) return false;
+    const int l = k.level;
+    std::map<Dir, std::pair<int, int>> found;  // dir -> (level, leaf); level -1: none
+    auto holder = [&](const Dir& dir) {
+      auto it = found.find(dir);
+      if (it != found.end()) return it->second;
+      std::pair<int, int> h{-1, -1};
+      auto nb = F.neighbors(k, dir);
+      if (!nb.empty()) {
+        int lv = F.leaves[nb[0]].level;
+        if (lv > l)
+          fail("a ghost face-plane row is held by a finer block than the block itself next to a "
+               "coarser neighbour; 2:1 balance forbids that, so this is a bug");
+        h = {lv, nb[0]};
+      }
+      found[dir] = h;
+      return h;
+    };
+    const int nt = int(tdims.size());
+    int npieces = 1;
+    for (int i = 0; i < nt; ++i) npieces *= 3;
+    std::vector<std::array<int, kMaxDim>> pieces(npieces);
+    for (int q = 0; q < npieces; ++q) {
+      int r = q;
+      for (int i = 0; i < nt; ++i) {  // the first plane dimension fastest
+        pieces[q][i] = r % 3;
+        r /= 3;
+      }
+    }
+    struct Src {
+      bool any = false;
+      Dir dir;
+      int leaf = -1;
+    };
+    std::vector<Src> sources(npieces);
+    bool anysrc = false;
+    for (int q = 0; q < npieces; ++q) {
+      std::vector<std::pair<int, int>> plane;  // (dimension, step)
+      for (int i = 0; i < nt; ++i)
+        if (pieces[q][i] != 1) plane.push_back({tdims[i], pieces[q][i] == 0 ? -1 : 1});
+      const int np = int(plane.size());
+      std::vector<int> subsets;
+      for (int m = 1; m < (1 << np); ++m) subsets.push_back(m);
+      std::stable_sort(subsets.begin(), subsets.end(), [](int x, int y) {
+        int cx = __builtin_popcount(unsigned(x)), cy = __builtin_popcount(unsigned(y));
+        return cx != cy ? cx < cy : x < y;
+      });
+      int bestm = -1, bestleaf = -1;
+      for (int m : subsets) {
+        Dir dir = look;
+        for (int i = 0; i < np; ++i)
+          if ((m >> i) & 1) dir[plane[i].first] = plane[i].second;
+        auto h = holder(dir);
+        if (h.first < 0) continue;
+        if (h.second == coarse) continue;
+        if (h.first == l && bestm < 0) {
+          bestm = m;
+          bestleaf = h.second;
+        }
+      }
+      if (bestm < 0) continue;
+      Dir dir = delta;
+      for (int i = 0; i < np; ++i)
+        if ((bestm >> i) & 1) dir[plane[i].first] = plane[i].second;
+      sources[q] = Src{true, dir, bestleaf};
+      anysrc = true;
+    }
+    if (!anysrc) return false;
+    auto code = [](int cls) { return int8_t(cls == 0 ? 0 : cls == 1 ? 8 * 1 + 3 : 8 * 4 + 4); };
+    auto spanof = [&](const std::array<int, kMaxDim>& cls) {
+      std::array<int8_t, kMaxDim> sp{-1, -1, -1};
+      for (int i = 0; i < nt; ++i) sp[tdims[i]] = code(cls[i]);
+      return sp;
+    };
+    std::vector<int> rest;
+    for (int q = 0; q < npieces; ++q)
+      if (!sources[q].any) rest.push_back(q);
+    if (!rest.empty()) {
+      std::array<int, kMaxDim> lo{9, 9, 9}, hi{-1, -1, -1};
+      for (int q : rest)
+        for (int i = 0; i < nt; ++i) {
+          lo[i] = std::min(lo[i], pieces[q][i]);
+          hi[i] = std::max(hi[i], pieces[q][i]);
+        }
+      int box = 1;
+      for (int i = 0; i < nt; ++i) box *= hi[i] - lo[i] + 1;
+      if (box == int(rest.size())) {
+        auto first_sub = [](int x) { return x == 0 ? 0 : x == 1 ? 1 : 4; };
+        auto last_sub = [](int x) { return x == 0 ? 0 : x == 1 ? 3 : 4; };
+        std::array<int8_t, kMaxDim> sp{-1, -1, -1};
+        for (int i = 0; i < nt; ++i) sp[tdims[i]] = int8_t(8 * first_sub(lo[i]) + last_sub(hi[i]));
+        parts.push_back(PlanePart{KindProlong, delta, sp, coarse});
+      } else {
+        for (int q : rest) parts.push_back(PlanePart{KindProlong, delta, spanof(pieces[q]), coarse});
+      }
+    }
+    for (int q = 0; q < npieces; ++q)
+      if (sources[q].any) parts.push_back(PlanePart{KindCopy, sources[q].dir, spanof(pieces[q]), sources[q].leaf});
+    return true;
+  }
+
+  // The neighbours of one leaf across each direction, looked up once
+  // (NeighbourTable, src/schedule.jl:1743-1788): level (-1: none to copy
+  // from) and each neighbour with its child offset.
+  struct Table {
+    struct Entry {
+      bool known = false;
+      int level = -1;
+      std::vector<std::pair<int, Dir>> nbrs;
+    };
+    const Forest& F;
+    int j, level;
+    std::vector<Entry> e;
+    Table(const Forest& f, int jj) : F(f), j(jj), level(f.leaves[jj].level), e(27) {}
+    const Entry& at(const Dir& delta) {
+      int i = 0, p = 1;
+      for (int d = 0; d < F.D; ++d) {
+        i += (delta[d] + 1) * p;
+        p *= 3;
+      }
+      Entry& x = e[i];
+      if (!x.known) {
+        x.known = true;
+        auto nb = F.neighbors(F.leaves[j], delta);
+        if (!nb.empty()) {
+          x.level = F.leaves[nb[0]].level;
+          for (int leaf : nb) {
+            Dir o{0, 0, 0};
+            for (int d = 0; d < F.D; ++d) o[d] = int(F.leaves[leaf].c[d] & 1u);
+            x.nbrs.push_back({leaf, o});
+          }
+        }
+      }
+      return x;
+    }
+  };
+
+  struct PieceCopy {
+    int level, leaf;
+    Dir cls, delta, offset;
+  };
+
+  static int piece_class(int sub) { return sub == 0 ? 0 : sub == 4 ? 2 : 1; }
+  Dir piece_of(const Dir& sub) const {
+    Dir p{1, 1, 1};
+    for (int d = 0; d < D; ++d) p[d] = piece_class(sub[d]);
+    return p;
+  }
+  int piece_index(const Dir& cls) const {
+    int i = 0, p = 1;
+    for (int d = 0; d < D; ++d) {
+      i += cls[d] * p;
+      p *= 3;
+    }
+    return i;
+  }
+  // The sub-classes each dimension has (closed_subpieces, src/schedule.jl:1687-1703).
+  std::vector<int> allowed(int d) const {
+    if (c[d] == 0) return {1, 3};
+    if (N == 2) return {0, 2, 4};
+    return {0, 1, 2, 3, 4};
+  }
+  // Every sub-piece (or piece), dimension 1 fastest.
+  std::vector<Dir> product(const std::array<std::vector<int>, kMaxDim>& vals) const {
+    std::vector<Dir> out;
+    int n = 1;
+    for (int d = 0; d < D; ++d) n *= int(vals[d].size());
+    for (int q = 0; q < n; ++q) {
+      Dir x{0, 0, 0};
+      int r = q;
+      for (int d = 0; d < D; ++d) {
+        x[d] = vals[d][r % vals[d].size()];
+        r /= int(vals[d].size());
+      }
+      out.push_back(x);
+    }
+    return out;
+  }
+  std::vector<Dir> subpieces() const {
+    std::array<std::vector<int>, kMaxDim> v;
+    for (int d = 0; d < D; ++d) v[d] = allowed(d);
+    return product(v);
+  }
+  std::vector<Dir> pieces() const {
+    std::array<std::vector<int>, kMaxDim> v;
+    for (int d = 0; d < D; ++d) v[d] = c[d] == 1 ? std::vector<int>{0, 1, 2} : std::vector<int>{1};
+    return product(v);
+  }
+  Dir representative(const Dir& cls) const {
+    Dir r{0, 0, 0};
+    for (int d = 0; d < D; ++d) r[d] = c[d] == 0 ? 1 : 2 * cls[d];
+    return r;
+  }
+
+  // Every copy of sub-piece `sub` of t's leaf on its level or a finer one,
+  // its own first (piece_copies, src/schedule.jl:1802-1847).
+  std::vector<PieceCopy> piece_copies(Table& t, const Dir& sub) const {
+    const Dir zero{0, 0, 0};
+    Dir cls = piece_of(sub);
+    std::vector<PieceCopy> copies{{t.level, t.j, cls, zero, zero}};
+    Dir side{0, 0, 0};
+    for (int d = 0; d < D; ++d) side[d] = c[d] == 1 && cls[d] != 1 ? (cls[d] == 0 ? -1 : 1) : 0;
+    for (int m = 1; m < (1 << D); ++m) {
+      bool skip = false;
+      for (int d = 0; d < D; ++d)
+        if (((m >> d) & 1) && side[d] == 0) skip = true;
+      if (skip) continue;
+      Dir delta{0, 0, 0};
+      for (int d = 0; d < D; ++d) delta[d] = ((m >> d) & 1) ? side[d] : 0;
+      const Table::Entry& e = t.at(delta);
+      if (e.level < 0) continue;
+      Dir across{1, 1, 1};
+      for (int d = 0; d < D; ++d) across[d] = delta[d] == 0 ? cls[d] : 2 - cls[d];
+      if (e.level < t.level) continue;
+      if (e.level == t.level) {
+        copies.push_back({e.level, e.nbrs[0].first, across, delta, zero});
+        continue;
+      }
+      for (const auto& no : e.nbrs) {
+        const Dir& o = no.second;
+        bool holds = true;
+        for (int d = 0; d < D; ++d)
+          if (delta[d] == 0)
+            holds = holds && (c[d] == 0 ? o[d] == (sub[d] == 3 ? 1 : 0)
+                                        : (sub[d] == 2 || o[d] == (sub[d] >= 3 ? 1 : 0)));
+        if (!holds) continue;
+        Dir fcls{1, 1, 1};
+        for (int d = 0; d < D; ++d)
+          fcls[d] = delta[d] != 0 ? across[d]
+                    : c[d] == 0   ? 1
+                    : sub[d] == 2 ? (o[d] == 0 ? 2 : 0)
+                    : (sub[d] == 0 || sub[d] == 4) ? cls[d]
+                                                   : 1;
+        copies.push_back({e.level, no.first, fcls, delta, o});
+      }
+    }
+    return copies;
+  }
+
+  // The primary among the copies (piece_primary, src/schedule.jl:1921):
+  // the finest level, then the greatest class tuple, dimension 1 first.
+  PieceCopy primary(const std::vector<PieceCopy>& copies) const {
+    size_t best = 0;
+    for (size_t i = 1; i < copies.size(); ++i) {
+      const PieceCopy &a = copies[i], &b = copies[best];
+      bool better = false;
+      if (a.level != b.level) {
+        better = a.level > b.level;
+      } else {
+        for (int d = 0; d < D; ++d)
+          if (a.cls[d] != b.cls[d]) {
+            better = a.cls[d] > b.cls[d];
+            break;
+          }
+      }
+      if (better) best = i;
+    }
+    return copies[best];
+  }
+
+  // The span codes of the bounding box of `subs` where they fill it
+  // (box_spans, src/schedule.jl:2125-2131).
+  bool box_spans(const std::vector<Dir>& subs, std::array<int8_t, kMaxDim>& spans) const {
+    Dir lo{9, 9, 9}, hi{-1, -1, -1};
+    for (const Dir& p : subs)
+      for (int d = 0; d < D; ++d) {
+        lo[d] = std::min(lo[d], p[d]);
+        hi[d] = std::max(hi[d], p[d]);
+      }
+    long box = 1;
+    for (int d = 0; d < D; ++d) {
+      long n = 0;
+      for (int x : allowed(d)) n += lo[d] <= x && x <= hi[d];
+      box *= n;
+    }
+    if (box != long(subs.size())) return false;
+    spans = {-1, -1, -1};
+    for (int d = 0; d < D; ++d) spans[d] = int8_t(8 * lo[d] + hi[d]);
+    return true;
+  }
+
+  // Stage 0's transfers into leaf b (unify_transfers and merged_transfers,
+  // src/schedule.jl:1990-2021, 2077-2122), recorded; returns b's primary
+  // bits, one per piece of its closed box.
+  uint32_t unify_sources(int b) {
+    Table t(F, b);
+    const int npieces = [&] {
+      int n = 1;
+      for (int d = 0; d < D; ++d) n *= 3;
+      return n;
+    }();
+    std::vector<int> shared(npieces, 0);
+    std::vector<PieceCopy> prim(npieces);
+    uint32_t bits = 0;
+    struct Entry {
+      Dir sub;
+      int kind, source;
+      Dir delta, offset;
+    };
+    std::vector<Entry> entries;
+    for (const Dir& sub : subpieces()) {
+      Dir cls = piece_of(sub);
+      int i = piece_index(cls);
+      if (shared[i] == 0) {
+        auto copies = piece_copies(t, representative(cls));
+        prim[i] = primary(copies);
+        bool onelevel = true;
+        for (const auto& x : copies) onelevel = onelevel && x.level == t.level;
+        shared[i] = onelevel ? 1 : 2;
+        if (prim[i].leaf == b && prim[i].cls == cls) bits |= 1u << i;
+      }
+      bool interior = true;
+      for (int d = 0; d < D; ++d) interior = interior && cls[d] == 1;
+      if (interior) continue;
+      PieceCopy p = shared[i] == 1 ? prim[i] : primary(piece_copies(t, sub));
+      if (p.leaf == b && p.cls == cls) continue;
+      entries.push_back({sub, p.level == t.level ? KindCopy : KindRestrict, p.leaf, p.delta, p.offset});
+    }
+    // merged_transfers: by (kind, source, direction, offset), sorted.
+    std::map<std::tuple<int, int, Dir, Dir>, std::vector<Dir>> bysource;
+    for (const Entry& e : entries) {
+      auto& v = bysource[std::make_tuple(e.kind, e.source, e.delta, e.offset)];
+      if (std::find(v.begin(), v.end(), e.sub) == v.end()) v.push_back(e.sub);
+    }
+    const std::array<int8_t, kMaxDim> nomirror{0, 0, 0};
+    for (const auto& kv : bysource) {
+      const int kind = std::get<0>(kv.first), s = std::get<1>(kv.first);
+      const Dir &delta = std::get<2>(kv.first), &offset = std::get<3>(kv.first);
+      const std::vector<Dir>& subs = kv.second;
+      std::array<int8_t, kMaxDim> spans;
+      if (kind == KindCopy && box_spans(subs, spans)) {
+        record(kind, delta, offset, 0, nomirror, b, s, spans);
+        continue;
+      }
+      std::vector<Dir> ps;
+      for (const Dir& x : subs) ps.push_back(piece_of(x));
+      std::sort(ps.begin(), ps.end());
+      ps.erase(std::unique(ps.begin(), ps.end()), ps.end());
+      for (const Dir& piece : ps) {
+        std::vector<Dir> part;
+        for (const Dir& x : subs)
+          if (piece_of(x) == piece) part.push_back(x);
+        if (box_spans(part, spans)) {
+          record(kind, delta, offset, 0, nomirror, b, s, spans);
+        } else {
+          for (const Dir& x : part) {
+            std::array<int8_t, kMaxDim> one{-1, -1, -1};
+            for (int d = 0; d < D; ++d) one[d] = int8_t(8 * x[d] + x[d]);
+            record(kind, delta, offset, 0, nomirror, b, s, one);
+          }
+        }
+      }
+    }
+    return bits;
+  }
+
+  // Stage P of leaf b (selfimage_sources!, src/schedule.jl:2606-2636):
+  // every primary piece on a reflecting wall plane, projected in place.
+  void selfimage_sources(int b, uint32_t bits, const std::vector<std::array<int8_t, kMaxDim>>& parities,
+                         std::map<GroupKey, std::pair<std::vector<int32_t>, std::vector<int32_t>>, KeyLess>& ppairs) const {
+    const Key& k = F.leaves[b];
+    std::array<std::array<bool, 2>, kMaxDim> refl{};
+    bool any = false;
+    for (int d = 0; d < D; ++d)
+      for (int s = 0; s < 2; ++s) {
+        refl[d][s] = false;
+        if (c[d] != 1) continue;
+        Dir delta{0, 0, 0};
+        delta[d] = s == 0 ? -1 : 1;
+        std::array<int8_t, kMaxDim> walls;
+        F.wall_direction(k, delta, walls);
+        refl[d][s] = walls[d] == ReflectingFace;
+        any = any || refl[d][s];
+      }
+    if (!any) return;
+    std::map<std::array<int8_t, kMaxDim>, std::vector<Dir>> bystate;
+    for (const Dir& cls : pieces()) {
+      if (!((bits >> piece_index(cls)) & 1u)) continue;
+      std::array<int8_t, kMaxDim> state{0, 0, 0};
+      bool anystate = false;
+      for (int d = 0; d < D; ++d) {
+        state[d] = (cls[d] == 0 && refl[d][0]) || (cls[d] == 2 && refl[d][1]) ? 2 : 0;
+        anystate = anystate || state[d] != 0;
+      }
+      if (!anystate) continue;
+      if (!parities.empty()) {
+        bool odd = false;
+        for (const auto& p : parities)
+          for (int d = 0; d < D; ++d) odd = odd || (state[d] == 2 && p[d] == OddParity);
+        if (!odd) continue;
+      }
+      bystate[state].push_back(cls);
+    }
+    const Dir zero{0, 0, 0};
+    auto first_sub = [](int x) { return x == 0 ? 0 : x == 1 ? 1 : 4; };
+    auto last_sub = [](int x) { return x == 0 ? 0 : x == 1 ? 3 : 4; };
+    for (const auto& kv : bystate) {
+      std::vector<Dir> ps = kv.second;
+      std::vector<Dir> uniq;
+      for (const Dir& x : ps)
+        if (std::find(uniq.begin(), uniq.end(), x) == uniq.end()) uniq.push_back(x);
+      Dir lo{9, 9, 9}, hi{-1, -1, -1};
+      for (const Dir& x : uniq)
+        for (int d = 0; d < D; ++d) {
+          lo[d] = std::min(lo[d], x[d]);
+          hi[d] = std::max(hi[d], x[d]);
+        }
+      long box = 1;
+      for (int d = 0; d < D; ++d) box *= hi[d] - lo[d] + 1;
+      std::vector<std::array<int8_t, kMaxDim>> spanlist;
+      if (box == long(uniq.size())) {
+        std::array<int8_t, kMaxDim> sp{-1, -1, -1};
+        for (int d = 0; d < D; ++d) sp[d] = int8_t(8 * first_sub(lo[d]) + last_sub(hi[d]));
+        spanlist.push_back(sp);
+      } else {
+        for (const Dir& x : uniq) {
+          std::array<int8_t, kMaxDim> sp{-1, -1, -1};
+          for (int d = 0; d < D; ++d) sp[d] = int8_t(8 * first_sub(x[d]) + last_sub(x[d]));
+          spanlist.push_back(sp);
+        }
+      }
+      for (const auto& sp : spanlist) {
+        GroupKey key;
+        key.kind = KindCopy;
+        key.direction = zero;
+        key.offset = zero;
+        key.level = 0;
+        key.mirror = kv.first;
+        key.span = sp;
+        auto& e = ppairs[key];
+        e.first.push_back(b);
+        e.second.push_back(b);
+      }
+    }
   }
 
   void boundary(int b, const Dir& delta) {
@@ -330,7 +814,7 @@ struct Builder {
       if (nblevel == k.level) {
         record(KindCopy, delta, zero, 0, nomirror, b, nbrs[0]);
       } else if (nblevel < k.level) {
-        record(KindProlong, delta, childoffset(k), k.level, nomirror, b, nbrs[0]);
+        record_prolong(b, delta, delta, nbrs[0], nomirror);
       } else {
         for (int nb : nbrs)
           record(KindRestrict, delta, childoffset(F.leaves[nb]), 0, nomirror, b, nb);
@@ -359,7 +843,7 @@ struct Builder {
     if (nblevel == k.level) {
       record(KindCopy, delta, zero, 0, state, b, nbrs[0]);
     } else if (nblevel < k.level) {
-      record(KindProlong, delta, childoffset(k), k.level, state, b, nbrs[0]);
+      record_prolong(b, delta, dprime, nbrs[0], state);
     } else {
       for (int nb : nbrs) {
         Dir o = childoffset(F.leaves[nb]);
@@ -390,9 +874,6 @@ GhostSchedule::GhostSchedule(const Forest& f, const Operators& ops_, const int* 
   check_operators(N, G.data(), c.data(), D, ops);
   bool closed = false;
   for (int d = 0; d < D; ++d) closed = closed || c[d] == 1;
-  if (closed)
-    fail("vertex-like dimensions (closed blocks, stage 0 and stage P) are not implemented yet",
-         kUnsupported);
   // A variable needs a parity in every dimension that has a reflecting face
   // (src/storage.jl:512-526).
   bool reflecting = false;
@@ -406,23 +887,45 @@ GhostSchedule::GhostSchedule(const Forest& f, const Operators& ops_, const int* 
 
   Builder B(f, G, c);
   auto dirs = alldirections(D);
-  for (int b = 0; b < f.nleaves(); ++b) B.block_sources(b, dirs);
+  // Stage P's transfers, apart from the others (src/schedule.jl:3268-3276).
+  std::map<GroupKey, std::pair<std::vector<int32_t>, std::vector<int32_t>>, KeyLess> ppairs;
+  for (int b = 0; b < f.nleaves(); ++b) {
+    B.block_sources(b, dirs);
+    // The secondaries of the block's closed box from their primaries, and
+    // its primaries on reflecting wall planes (src/schedule.jl:1599-1607).
+    if (closed) {
+      uint32_t bits = B.unify_sources(b);
+      bool reflecting_set = false;
+      for (int d = 0; d < D; ++d)
+        reflecting_set = reflecting_set || f.faces[d][0] == ReflectingFace || f.faces[d][1] == ReflectingFace;
+      B.selfimage_sources(b, bits, reflecting_set ? par : std::vector<std::array<int8_t, kMaxDim>>{}, ppairs);
+    }
+  }
   boundaries = std::move(B.boundaries);
 
   // One group per key (src/schedule.jl:3193-3232, 2970-2992).
-  for (auto& kv : B.pairs) {
-    const GroupKey& key = kv.first;
+  auto add_group = [&](const GroupKey& key, const std::vector<int32_t>& tb, const std::vector<int32_t>& sb,
+                       int stage) {
     TransferGroup g;
     g.key = key;
     g.kind = key.kind;
-    g.targetblocks = kv.second.first;
-    g.sourceblocks = kv.second.second;
+    g.targetblocks = tb;
+    g.sourceblocks = sb;
     for (int d = 0; d < kMaxDim; ++d) {
       if (d >= D) {
         g.stencils[d] = unit_point();
         continue;
       }
       int dd = key.direction[d], od = key.offset[d], st = key.mirror[d];
+      // A span of stage 0 or P: a copy or a restriction into the target's
+      // closed box (unify_stencil, src/schedule.jl:973-981).
+      if (key.isunify(D)) {
+        g.stencils[d] = key.kind == KindCopy
+                            ? copy_stencil(N, G[d], c[d], dd, key.span[d])
+                            : restrict_stencil_over(piece_range(N, G[d], c[d], key.span[d]), N, G[d], c[d],
+                                                    dd, od, ops.restriction);
+        continue;
+      }
       int ds = st == 0 ? dd : 0;
       Stencil1D s = (st != 0 && key.kind == KindRestrict && c[d] == 1)
                         ? mirror_restriction(N, G[d], od, ops.restriction)
@@ -441,11 +944,16 @@ GhostSchedule::GhostSchedule(const Forest& f, const Operators& ops_, const int* 
       p3 *= 3;
     }
     g.factorcol = signed_ ? col : 0;
-    // src/schedule.jl:653-656
-    g.stage = key.isunify(D) ? kUnifyTag
-              : key.kind == KindProlong ? prolongation_tag(key.level) : kPhase1Tag;
+    g.stage = stage;
     groups.push_back(std::move(g));
+  };
+  for (auto& kv : B.pairs) {
+    const GroupKey& key = kv.first;
+    // src/schedule.jl:653-656
+    int stage = key.isunify(D) ? kUnifyTag : key.kind == KindProlong ? prolongation_tag(key.level) : kPhase1Tag;
+    add_group(key, kv.second.first, kv.second.second, stage);
   }
+  for (auto& kv : ppairs) add_group(kv.first, kv.second.first, kv.second.second, kStageP);
   std::stable_sort(groups.begin(), groups.end(), [](const TransferGroup& a, const TransferGroup& b) {
     if (a.stage != b.stage) return a.stage < b.stage;
     return key_less(a.key, b.key);
@@ -453,6 +961,7 @@ GhostSchedule::GhostSchedule(const Forest& f, const Operators& ops_, const int* 
 
   // Stages in fill order; phase 1 always present (src/schedule.jl:3259-3266).
   std::vector<int> tags;
+  if (!ppairs.empty()) tags.push_back(kStageP);
   if (closed) tags.push_back(kUnifyTag);
   tags.push_back(kPhase1Tag);
   for (auto& g : groups)

@@ -7,6 +7,11 @@ that a ghost the fill misses stays visible.
   config1: 2x2x2 periodic roots, N=8, cell, G=2, 4 vars, Conservative(3,2);
            every root refined once, then every level-1 leaf whose centre is
            within 0.3 of the domain centre: 120 blocks (56 + 64).
+  config2: 4x4x4 roots, N=16, vertex, G=3, 8 vars, PointValue(6,6); nested
+           refinement around the domain centre at radii 0.35/0.2/0.1 (leaf
+           centre within the radius, then balance): 232 blocks (56+56+56+64);
+           reflecting z-low face (variables 1-6 even, 7-8 odd in z), outflow
+           elsewhere; data sin(2 pi k.x + phase) from seed 1 plus 1e-3 noise.
   config5: 8x8x8 roots, N=32, cell, G=3, 20 vars, PointValue(6,6); refined
            twice where leaves meet 4 seeded spheres, 2:1 balanced; reflecting
            z-low face (parities alternating even/odd in z), outflow elsewhere.
@@ -16,7 +21,7 @@ import math
 
 import numpy as np
 
-from .core import (Conservative, EvenParity, FieldSet, Forest, OddParity, Operators,
+from .core import (Conservative, EvenParity, FieldSet, Forest, NoParity, OddParity, Operators,
                    PointValue)
 
 
@@ -34,6 +39,49 @@ def config1():
     f = config1_forest()
     ops = Operators(prolongation=3, restriction=2, family=Conservative)
     return f, ops, dict(nvars=4, G=2), "config1: 2^3 periodic roots, N=8, cell, G=2, 4 vars, Conservative(3,2)"
+
+
+def config2_forest(roots=4, N=16, radii=(0.35, 0.2, 0.1)):
+    f = Forest((roots,) * 3, N=N, extents=((0.0, 1.0),) * 3,
+               reflecting=((False, False), (False, False), (True, False)),
+               outflow=((True, True), (True, True), (False, True)))
+    for lvl, r in enumerate(radii):
+        c = f.block_centers()
+        _, _, level = f.block_geometry()
+        f.refine([i for i in range(f.nleaves) if level[i] == lvl and np.linalg.norm(c[i] - 0.5) < r])
+        f.balance()
+    return f
+
+
+def config2(nvars=8, roots=4, N=16, radii=(0.35, 0.2, 0.1)):
+    f = config2_forest(roots=roots, N=N, radii=radii)
+    ops = Operators(prolongation=6, restriction=6)
+    parity = tuple((NoParity, NoParity, EvenParity if v < nvars - 2 else OddParity)
+                   for v in range(nvars))
+    return f, ops, dict(nvars=nvars, G=3, centering="vertex", parity=parity), (
+        f"config2: {roots}^3 roots, N={N}, vertex, G=3, {nvars} vars, PointValue(6,6), "
+        "nested refinement, reflecting z-lo, outflow elsewhere")
+
+
+def fill_waves(fs, seed, noise=1e-3):
+    """Own points = sin(2 pi k_v.x + phase_v) per variable (k_v, phase_v from
+    `seed`), plus uniform noise of amplitude `noise` on every stored own copy."""
+    import torch
+    rng = np.random.default_rng(seed)
+    dev = fs.work.device
+    X = [torch.tensor(x, device=dev) for x in fs.coordinates()]
+    a = fs.array()
+    own = fs.own_slices()
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    for v in range(fs.nvars):
+        k = rng.integers(1, 4, size=3).astype(float)
+        ph = rng.uniform(0.0, 2 * math.pi)
+        val = torch.sin(2 * math.pi * (k[0] * X[0] + k[1] * X[1] + k[2] * X[2]) + ph)
+        val = val[(slice(None),) + own]
+        val = val + noise * (torch.rand(val.shape, generator=g, dtype=torch.float64, device=dev) * 2 - 1)
+        a[(slice(None), v) + own] = val
+    return fs
 
 
 def _spheres(seed, n, rlo, rhi):
@@ -119,9 +167,9 @@ def config3(count=64, first_seed=2, max_bytes=2 << 30, layout="arena", device="c
     pairs = []
     for i in range(count):
         sd = first_seed + i
-        f, ops, G, nvars = random_forest(sd, max_blocks=4096, max_bytes=max_bytes)
+        f, ops, G, nvars, centering = random_forest(sd, max_blocks=4096, max_bytes=max_bytes)
         for k, nv in enumerate(nvars):
-            fs = FieldSet(f, nv, G=G, device=device, layout=layout, fill=0.0)
+            fs = FieldSet(f, nv, G=G, centering=centering, device=device, layout=layout, fill=0.0)
             if seed_data:
                 fill_uniform(fs, 1000 * sd + k)
             pairs.append((fs, GhostSchedule(fs, ops)))
@@ -136,10 +184,10 @@ def random_forest(seed, max_blocks=4096, max_bytes=None):
     """One forest of the batched configuration, drawn from `seed`: brick of
     r^3 roots (r in 1..4), N in {8,16,32}, 1-4 refinement levels around 1-3
     spheres of radius 0.05-0.3, G in {2,3}, 1-3 field sets of 2-10 variables.
-    All faces periodic; PointValue of order 2G. Cell centering (vertex
-    centering is not yet supported by this library). Draws whose blocks
-    exceed max_blocks, or whose field sets together exceed max_bytes, are
-    redrawn with fewer levels."""
+    All faces periodic; PointValue of order 2G; cell or vertex centering.
+    Draws whose blocks exceed max_blocks, or whose field sets together exceed
+    max_bytes, are redrawn with fewer levels. Returns (forest, operators, G,
+    nvars per field set, centering)."""
     rng = np.random.default_rng(seed)
     r = int(rng.integers(1, 5))
     N = int(rng.choice([8, 16, 32]))
@@ -150,11 +198,14 @@ def random_forest(seed, max_blocks=4096, max_bytes=None):
     G = int(rng.choice([2, 3]))
     nsets = int(rng.integers(1, 4))
     nvars = [int(rng.integers(2, 11)) for _ in range(nsets)]
+    centering = "vertex" if rng.integers(0, 2) == 1 else "cell"
+    c = 1 if centering == "vertex" else 0
+
     def too_big(f):
         if f.nleaves > max_blocks:
             return True
         if max_bytes is not None:
-            per_block = (N + 2 * G) ** 3 * 8 * sum(nvars)
+            per_block = (N + 2 * G + c) ** 3 * 8 * sum(nvars)
             return f.nleaves * per_block > max_bytes
         return False
 
@@ -175,4 +226,4 @@ def random_forest(seed, max_blocks=4096, max_bytes=None):
             break
         levels -= 1
     ops = Operators(prolongation=2 * G, restriction=2 * G)
-    return f, ops, G, nvars
+    return f, ops, G, nvars, centering
