@@ -117,6 +117,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-config2", action="store_true")
+    ap.add_argument("--config2-repeats", type=int, default=100)
     ap.add_argument("--batched-steps", type=int, default=3)
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="at the end, copy a buffer of this many GiB once (a kernel of known "
@@ -228,6 +230,49 @@ def main():
         }
         del bgraph_obj, pairs
 
+    nested = None
+    if not args.no_config2:
+        log("[bench] the nested vertex-centred mesh (config 2)")
+        f2, ops2, kw2, _ = W.config2()
+        kw2 = dict(kw2)
+        nv2 = kw2.pop("nvars")
+        fs2 = FieldSet(f2, nv2, device="cuda", layout="dense", fill=0.0, **kw2)
+        W.fill_waves(fs2, 1)
+        s2 = GhostSchedule(fs2, ops2)
+        v2 = s2.points() * nv2
+        reps = args.config2_repeats
+        p2 = time_fill(lambda: fill_ghosts(fs2, s2, method="plain"), reps, args.warmup, stream)
+        st2 = time_fill(lambda: fill_ghosts(fs2, s2, method="staged"), reps, args.warmup, stream)
+        g2obj = FillGraph([(fs2, s2)])
+        g2 = time_fill(g2obj.launch, reps, args.warmup, stream)
+        c2 = None
+        if not args.no_cpu:
+            import oracle
+            host = fs2.work.cpu().numpy().copy()
+            fac = fs2.factors.cpu().numpy().copy()
+            nth = oracle.max_threads()
+            oracle.fill(s2._descptr, fs2.layout, host, fac, nthreads=nth)
+            times = []
+            for _ in range(20):
+                t0 = time.perf_counter()
+                oracle.fill(s2._descptr, fs2.layout, host, fac, nthreads=nth)
+                times.append(time.perf_counter() - t0)
+            times.sort()
+            c2 = dict(min_ms=round(times[0] * 1e3, 3), median_ms=round(times[10] * 1e3, 3), threads=nth)
+        nested = {
+            "workload": "config 2: 4^3 roots, N=16, vertex, G=3, 8 vars, PointValue(6,6), nested "
+                        "refinement at radii 0.35/0.2/0.1 (232 blocks), reflecting z-lo (vars 7-8 odd), "
+                        "outflow elsewhere; dense layout",
+            "blocks": f2.nleaves, "values_per_fill": v2, "repeats": reps,
+            "plain_ms": {"min": round(p2["min_ms"], 4), "median": round(p2["median_ms"], 4)},
+            "staged_ms": {"min": round(st2["min_ms"], 4), "median": round(st2["median_ms"], 4)},
+            "graph_ms": {"min": round(g2["min_ms"], 4), "median": round(g2["median_ms"], 4)},
+            "graph_GBs_at_min": round(16.0 * v2 / (g2["min_ms"] * 1e-3) / 1e9, 1),
+            "cpu_oracle_ms": c2,
+        }
+        log(f"[bench] config 2: {nested}")
+        del g2obj, fs2, s2
+
     cpu = None
     if not args.no_cpu:
         log("[bench] timing the CPU baseline (oracle)")
@@ -266,6 +311,7 @@ def main():
         "adjoint_ms": r(adjoint),
         "adjoint_over_forward": round(adjoint["mean_ms"] / t_ms, 3) if adjoint else None,
         "batched": batched,
+        "config2": nested,
         "schedule_build_s": round(build_s, 3),
         "roofline": {
             "kernel": dominant,

@@ -251,9 +251,32 @@ tah_status tah_graph_destroy(tah_graph* g);
  * in a fixed order (no float atomics), then the targets zeroed. */
 tah_status tah_fill_adjoint(tah_plan* p, double* adj, const double* factors, void* stream);
 
-/* Pick tile shapes and work-group sizes per stencil class by timing on the
- * device; cached in the file at cache_path (may be NULL) by mesh signature. */
-tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char* cache_path);
+/* A plan's launch options (no counterpart in the reference, whose launches
+ * are fixed): fork runs a stage group's kernel classes on four forked
+ * streams (1) or in order on the fill's stream (0, default); merge runs
+ * every interpolation unit class of one order in one launch (1, default) or
+ * each in its own (0); defer moves phase-1 transfers that no later stage
+ * reads or overwrites into the last stage group (1, default). Setting merge,
+ * defer or copy_per_thread rebuilds the plan's tiles (it waits for the
+ * device first);
+ * batches and graphs made earlier keep the options they were made with. */
+typedef struct tah_plan_options {
+  int32_t fork;
+  int32_t merge;
+  int32_t defer;
+  int32_t copy_per_thread; /* the copy kernel's tile: 4, 8 (default) or 16 values per thread */
+} tah_plan_options;
+tah_status tah_plan_get_options(const tah_plan* p, tah_plan_options* out);
+tah_status tah_plan_set_options(tah_plan* p, const tah_plan_options* o);
+
+/* The autotuner: times every combination of the plan's options on `work`
+ * (fills it a few dozen times: a fill is idempotent) and keeps the fastest.
+ * With cache_path, the choice is appended to that text file under a
+ * signature of the mesh, layout and device, and a later call with the same
+ * signature takes it from there without timing (best_ms then -1).
+ * best_ms (may be NULL): the chosen options' median fill time. */
+tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char* cache_path,
+                    double* best_ms);
 
 #ifdef __cplusplus
 }

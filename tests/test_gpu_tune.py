@@ -1,0 +1,63 @@
+"""Plan options and the autotuner: every combination of the launch options
+fills bit for bit as the defaults do, and a tuned choice is cached by mesh
+signature and taken back from the cache without timing."""
+import itertools
+
+import pytest
+import torch
+
+from treeamr_hip import FieldSet, GhostSchedule, fill_ghosts, tune
+from treeamr_hip import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(which):
+    if which == "cell":
+        f, ops, kw, _ = W.config5(nvars=3, roots=4, N=16)
+    else:
+        f, ops, kw, _ = W.config2(roots=2, N=8, radii=(0.5, 0.3))
+    kw = dict(kw)
+    fs = FieldSet(f, kw.pop("nvars"), device="cuda", layout="arena", fill=float("nan"), **kw)
+    W.fill_uniform(fs, 9)
+    return fs, GhostSchedule(fs, ops)
+
+
+@pytest.mark.parametrize("which", ["cell", "vertex"])
+def test_every_option_combination_fills_alike(which):
+    fs, s = _case(which)
+    init = fs.work.clone()
+    fill_ghosts(fs, s)
+    torch.cuda.synchronize()
+    ref = fs.work.clone()
+    plan = s.plan(fs.layout)
+    for fork, merge, defer, cpt in itertools.product((0, 1), (0, 1), (0, 1), (4, 8, 16)):
+        print(f"[tune] {which} fork={fork} merge={merge} defer={defer} copy_per_thread={cpt}", flush=True)
+        plan.set_options(fork=fork, merge=merge, defer=defer, copy_per_thread=cpt)
+        fs.work.copy_(init)
+        fill_ghosts(fs, s)
+        torch.cuda.synchronize()
+        assert torch.equal(fs.work.view(torch.int64), ref.view(torch.int64)), (fork, merge, defer, cpt)
+    plan.set_options(fork=0, merge=1, defer=1, copy_per_thread=8)
+
+
+def test_tune_caches_by_signature(tmp_path):
+    fs, s = _case("cell")
+    path = tmp_path / "tune.txt"
+    opts, ms = tune(fs, s, path)
+    assert ms is not None and ms > 0
+    lines = path.read_text().splitlines()
+    assert len(lines) == 1 and lines[0].startswith("tah-tune-1 ")
+    s.plan(fs.layout).set_options(fork=1 - opts["fork"])
+    opts2, ms2 = tune(fs, s, path)
+    assert ms2 is None and opts2 == opts
+    assert len(path.read_text().splitlines()) == 1
+    # the tuned plan still fills correctly
+    init = fs.work.clone()
+    fill_ghosts(fs, s)
+    torch.cuda.synchronize()
+    once = fs.work.clone()
+    fill_ghosts(fs, s)
+    torch.cuda.synchronize()
+    assert torch.equal(once.view(torch.int64), fs.work.view(torch.int64))
+    del init

@@ -402,7 +402,7 @@ void build_adjoint(tah_plan* p) {
       const PlanGroup& g = p->groups[gi];
       if (g.stage != st || g.points == 0) continue;
       int32_t ca[3], cs[3];
-      const bool copy = affine_copy(g, ca, cs);
+      const bool copy = affine_copy(g, ca, cs, 1);
       for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t) {
         es.push_back({g.sblocks[t], int32_t(gi), t});
         if (copy) continue;  // the gather zeroes a copy's targets
@@ -499,7 +499,7 @@ void build_adjoint(tah_plan* p) {
         AdjEntry& E = R.entry;
         E.factorcol = g.factorcol;
         int32_t ca[3], cs[3];
-        if (affine_copy(g, ca, cs)) {
+        if (affine_copy(g, ca, cs, 1)) {
           R.self = g.tblocks[t] == g.sblocks[t];
           for (int d = 0; d < 3; ++d) R.self = R.self && ca[d] == g.tfirst[d] && (cs[d] == 1 || g.n[d] == 1);
           // target row r = (p - ca)*cs along a stepped dimension; all rows

@@ -192,10 +192,4 @@ tah_status tah_graph_destroy(tah_graph* g) {
   return TAH_OK;
 }
 
-tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char* cache_path) {
-  (void)p; (void)work; (void)factors; (void)cache_path;
-  set_last_error("tah_tune is not implemented yet");
-  return TAH_EUNSUPPORTED;
-}
-
 }  // extern "C"

@@ -60,6 +60,8 @@ struct alignas(16) DevWClass {
   int32_t nunits;
   const int32_t* units;  // 8 per unit: x rows (2), y rows (2), z rows (2), -, -
   const double* W;
+  int32_t rbits;  // bit d set where the unit has two rows along d
+  int32_t pad;
 };
 
 // One fill of one working array. Element (i, j, k, v) of block b is at

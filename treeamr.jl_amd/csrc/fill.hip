@@ -55,8 +55,23 @@ void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, cons
   dim3 grid(ntiles), blk(kThreads);
   if (cls == kClsGeneric) {
     hipLaunchKernelGGL(generic_kernel, grid, blk, 0, s, jobs, tiles);
-  } else if (cls == kClsCopy) {
-    hipLaunchKernelGGL(copy_kernel, grid, blk, 0, s, jobs, tiles);
+  } else if (cls == copy_launch(4)) {
+    hipLaunchKernelGGL(copy_kernel<4>, grid, blk, 0, s, jobs, tiles);
+  } else if (cls == copy_launch(8)) {
+    hipLaunchKernelGGL(copy_kernel<8>, grid, blk, 0, s, jobs, tiles);
+  } else if (cls == copy_launch(16)) {
+    hipLaunchKernelGGL(copy_kernel<16>, grid, blk, 0, s, jobs, tiles);
+  } else if (cls >= kClsMerged) {
+    switch (cls - kClsMerged) {
+      case 0: hipLaunchKernelGGL((interp_all_kernel<2, 0>), grid, blk, 0, s, jobs, tiles); break;
+      case 1: hipLaunchKernelGGL((interp_all_kernel<3, 0>), grid, blk, 0, s, jobs, tiles); break;
+      case 2: hipLaunchKernelGGL((interp_all_kernel<4, 0>), grid, blk, 0, s, jobs, tiles); break;
+      case 3: hipLaunchKernelGGL((interp_all_kernel<5, 0>), grid, blk, 0, s, jobs, tiles); break;
+      case 4: hipLaunchKernelGGL((interp_all_kernel<6, 0>), grid, blk, 0, s, jobs, tiles); break;
+      case 5: hipLaunchKernelGGL((interp_all_kernel<2, 2>), grid, blk, 0, s, jobs, tiles); break;
+      case 6: hipLaunchKernelGGL((interp_all_kernel<4, 2>), grid, blk, 0, s, jobs, tiles); break;
+      default: hipLaunchKernelGGL((interp_all_kernel<6, 2>), grid, blk, 0, s, jobs, tiles); break;
+    }
   } else {
     int id = cls - kClsInterp, ps = id / 8, rbits = id % 8;
     switch (ps) {
@@ -74,7 +89,8 @@ void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, cons
 }
 
 int class_tile_elems(int cls) {
-  return cls == kClsGeneric ? kGenericTile : cls == kClsCopy ? kCopyTile : kThreads;
+  if (cls > kClsCopyLaunch && cls <= kClsCopyLaunch + 16) return kThreads * (cls - kClsCopyLaunch);
+  return cls == kClsGeneric ? kGenericTile : kThreads;
 }
 
 }  // namespace tah
@@ -369,8 +385,11 @@ tah_status tah_kernel_class_name(int32_t cls, char* buf, int32_t len) {
     std::string name;
     if (cls == kClsGeneric) {
       name = "generic_kernel";
-    } else if (cls == kClsCopy) {
-      name = "copy_kernel";
+    } else if (cls > kClsCopyLaunch && cls <= kClsCopyLaunch + 16) {
+      name = "copy_kernel<" + std::to_string(cls - kClsCopyLaunch) + ">";
+    } else if (cls >= kClsMerged && cls - kClsMerged < int(kInterpTable.size())) {
+      const auto& e = kInterpTable[cls - kClsMerged];
+      name = "interp_all_kernel<" + std::to_string(e[0]) + ", " + std::to_string(e[1]) + ">";
     } else if (cls >= kClsInterp && (cls - kClsInterp) / 8 < int(kInterpTable.size())) {
       const int id = cls - kClsInterp, rb = id % 8;
       const auto& e = kInterpTable[id / 8];

@@ -165,6 +165,8 @@ constexpr int kCopyPerThread = 8;
 constexpr int kCopyTile = kThreads * kCopyPerThread;
 
 #ifdef TAH_DEFINE_KERNELS
+// CPT values per thread (the tile shape a plan option picks: 4, 8 or 16).
+template <int CPT>
 __global__ void __launch_bounds__(kThreads) copy_kernel(const Job* __restrict__ jobs,
                                                         const Tile* __restrict__ tiles) {
   const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
@@ -181,10 +183,10 @@ __global__ void __launch_bounds__(kThreads) copy_kernel(const Job* __restrict__ 
   const int32_t tb0 = s0 * g.tfirst[0] + s1 * g.tfirst[1] + s2 * g.tfirst[2];
   const int32_t sb0 = s0 * g.ca[0] + s1 * g.ca[1] + s2 * g.ca[2];
   const int32_t ds0 = s0 * g.cs[0], ds1 = s1 * g.cs[1], ds2 = s2 * g.cs[2];
-  int32_t dst[kCopyPerThread], var[kCopyPerThread];
-  double val[kCopyPerThread];
+  int32_t dst[CPT], var[CPT];
+  double val[CPT];
 #pragma unroll
-  for (int r = 0; r < kCopyPerThread; ++r) {
+  for (int r = 0; r < CPT; ++r) {
     uint32_t e = uint32_t(tile.e0) + uint32_t(r * kThreads) + threadIdx.x;
     const bool ok = e < total;
     e = ok ? e : 0u;
@@ -196,7 +198,7 @@ __global__ void __launch_bounds__(kThreads) copy_kernel(const Job* __restrict__ 
     val[r] = sbase[vo + sb0 + ds0 * i + ds1 * j + ds2 * k];
   }
 #pragma unroll
-  for (int r = 0; r < kCopyPerThread; ++r) {
+  for (int r = 0; r < CPT; ++r) {
     if (dst[r] < 0) continue;
     tbase[dst[r]] = scaled(__dadd_rn(0.0, val[r]), job.factors, g.factorcol, nvars, var[r]);
   }
@@ -215,14 +217,10 @@ __global__ void __launch_bounds__(kThreads) copy_kernel(const Job* __restrict__ 
 // of every (target, tap) is the class's table W, read in visit order through
 // uniform loads, identical for every lane.
 template <int P, int SHIFT, int R0, int R1, int R2>
-__global__ void __launch_bounds__(kThreads) interp_kernel(const Job* __restrict__ jobs,
-                                                          const Tile* __restrict__ tiles) {
+__device__ __forceinline__ void interp_body(const Job& job, const Tile& tile, const DevWClass& wc) {
   constexpr int U0 = P + (R0 == 2 ? SHIFT : 0);
   constexpr int U1 = P + (R1 == 2 ? SHIFT : 0);
   constexpr int U2 = P + (R2 == 2 ? SHIFT : 0);
-  const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-  const Job& job = jobs[tile.job];
-  const DevWClass& wc = job.wclasses[tile.item];
   const DevGroup& g = job.groups[wc.group];
   const int32_t t = int32_t(threadIdx.x);
   if (t >= tile.nu * tile.nvt) return;
@@ -313,5 +311,38 @@ __global__ void __launch_bounds__(kThreads) interp_kernel(const Job* __restrict_
         dst[s0 * (g.tfirst[0] + ra[a]) + s1 * (g.tfirst[1] + rb[b]) + s2 * (g.tfirst[2] + rc[c])] =
             scaled(acc[a][b][c], job.factors, g.factorcol, nvars, v);
 }
+
+#ifdef TAH_DEFINE_KERNELS
+// One unit class per launch.
+template <int P, int SHIFT, int R0, int R1, int R2>
+__global__ void __launch_bounds__(kThreads) interp_kernel(const Job* __restrict__ jobs,
+                                                          const Tile* __restrict__ tiles) {
+  const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  const Job& job = jobs[tile.job];
+  interp_body<P, SHIFT, R0, R1, R2>(job, tile, job.wclasses[tile.item]);
+}
+
+// Every unit class of one order and shift in one launch, the class picked
+// per tile (a branch uniform over the workgroup), so that the tiles of all
+// classes of one transfer -- which read the same source windows -- can run
+// side by side and meet in one L2.
+template <int P, int SHIFT>
+__global__ void __launch_bounds__(kThreads) interp_all_kernel(const Job* __restrict__ jobs,
+                                                              const Tile* __restrict__ tiles) {
+  const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  const Job& job = jobs[tile.job];
+  const DevWClass& wc = job.wclasses[tile.item];
+  switch (uniform(wc.rbits)) {
+    case 0: interp_body<P, SHIFT, 1, 1, 1>(job, tile, wc); break;
+    case 1: interp_body<P, SHIFT, 2, 1, 1>(job, tile, wc); break;
+    case 2: interp_body<P, SHIFT, 1, 2, 1>(job, tile, wc); break;
+    case 3: interp_body<P, SHIFT, 2, 2, 1>(job, tile, wc); break;
+    case 4: interp_body<P, SHIFT, 1, 1, 2>(job, tile, wc); break;
+    case 5: interp_body<P, SHIFT, 2, 1, 2>(job, tile, wc); break;
+    case 6: interp_body<P, SHIFT, 1, 2, 2>(job, tile, wc); break;
+    default: interp_body<P, SHIFT, 2, 2, 2>(job, tile, wc); break;
+  }
+}
+#endif  // TAH_DEFINE_KERNELS
 
 }  // namespace tah

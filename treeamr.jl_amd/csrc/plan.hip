@@ -9,14 +9,14 @@
 namespace tah {
 
 // A unit copy whose source row is affine in the target row along every
-// dimension: ca + cs*r with cs in {-1, 0, 1}.
-bool affine_copy(const PlanGroup& g, int32_t ca[3], int32_t cs[3]) {
+// dimension: ca + cs*r with |cs| <= max_step (2 takes in vertex injection).
+bool affine_copy(const PlanGroup& g, int32_t ca[3], int32_t cs[3], int max_step) {
   if (!g.unit) return false;
   for (int d = 0; d < 3; ++d) {
     const auto& s = g.srcstart[d];
     ca[d] = s.empty() ? 0 : s[0];
     cs[d] = s.size() > 1 ? s[1] - s[0] : 0;
-    if (cs[d] < -1 || cs[d] > 1) return false;
+    if (cs[d] < -max_step || cs[d] > max_step) return false;
     for (size_t r = 0; r < s.size(); ++r)
       if (s[r] != ca[d] + cs[d] * int32_t(r)) return false;
   }
@@ -75,7 +75,7 @@ void classify(tah_plan* p) {
     g.wclasses.clear();
     int32_t ca[3], cs[3];
     if (g.points == 0) continue;
-    if (affine_copy(g, ca, cs)) {
+    if (affine_copy(g, ca, cs, 2)) {
       g.cls = kClsCopy;
       continue;
     }
@@ -211,7 +211,7 @@ void build_plan_device(tah_plan* p) {
     g.tblocks = reinterpret_cast<const int32_t*>(base + o.tb);
     g.sblocks = reinterpret_cast<const int32_t*>(base + o.sb);
     int32_t ca[3] = {0, 0, 0}, cs[3] = {0, 0, 0};
-    if (h.cls == kClsCopy) affine_copy(h, ca, cs);
+    if (h.cls == kClsCopy) affine_copy(h, ca, cs, 2);
     for (int d = 0; d < 3; ++d) {
       g.n[d] = h.n[d];
       g.order[d] = h.order[d];
@@ -234,13 +234,19 @@ void build_plan_device(tah_plan* p) {
     d.nunits = c.nunits();
     d.units = reinterpret_cast<const int32_t*>(base + unit_off[ci]);
     d.W = reinterpret_cast<const double*>(base + off_w) + c.wtable;
+    d.rbits = (c.cls - kClsInterp) % 8;
     dw.push_back(d);
   }
   if (!dw.empty()) std::memcpy(blob.host.data() + off_wc, dw.data(), sizeof(DevWClass) * dw.size());
   p->dgroups = reinterpret_cast<DevGroup*>(base + off_groups);
   p->dwclasses = reinterpret_cast<DevWClass*>(base + off_wc);
   HIPCHECK(hipMemcpy(p->dtables, blob.host.data(), blob.host.size(), hipMemcpyHostToDevice));
+  HIPCHECK(hipMalloc(&p->djob, sizeof(Job)));
+  std::memset(&p->last_job, 0, sizeof(Job));
+  build_launches(p);
+}
 
+void build_launches(tah_plan* p) {
   // Which stage group each transfer runs in: its own, or the last one when
   // no later stage reads what it writes nor writes what it reads, so that it
   // can run beside the last group's work (most of phase 1 beside the
@@ -285,7 +291,7 @@ void build_plan_device(tah_plan* p) {
           const PlanGroup& g = p->groups[gi];
           if (g.points == 0 || sg_of_stage[g.stage] != k) continue;
           const Box3 wb = wbox(g), rb = rbox(g);
-          for (size_t t = 0; t < g.tblocks.size(); ++t) {
+          for (size_t t = 0; t < g.tblocks.size() && p->defer; ++t) {
             bool ok = true;
             auto ir = later_reads.find(g.tblocks[t]);
             if (ir != later_reads.end())
@@ -319,6 +325,13 @@ void build_plan_device(tah_plan* p) {
   const bool vfast = p->layout.stride[3] < p->layout.stride[0];
   p->tiles.clear();
   p->launches.clear();
+  // The launch an interpolation class runs in: its own, or with every
+  // class of its order and shift (tah_plan::merge).
+  auto launch_of = [&](int cls) {
+    return p->merge && cls >= kClsInterp && cls < kClsMerged ? kClsMerged + (cls - kClsInterp) / 8 : cls;
+  };
+  // The launch a copy or generic group runs in.
+  auto group_launch = [&](int cls) { return cls == kClsCopy ? copy_launch(p->copy_per_thread) : cls; };
   for (int k = 0; k < nsg; ++k) {
     std::vector<Launch> ls;
     std::vector<int> classes;
@@ -326,9 +339,9 @@ void build_plan_device(tah_plan* p) {
       const PlanGroup& g = p->groups[gi];
       if (g.points == 0 || std::find(run_in[gi].begin(), run_in[gi].end(), k) == run_in[gi].end()) continue;
       if (g.cls == kClsInterp) {
-        for (int32_t ci : g.wclasses) classes.push_back(p->wclasses[ci].cls);
+        for (int32_t ci : g.wclasses) classes.push_back(launch_of(p->wclasses[ci].cls));
       } else {
-        classes.push_back(g.cls);
+        classes.push_back(group_launch(g.cls));
       }
     }
     std::sort(classes.begin(), classes.end());
@@ -337,27 +350,29 @@ void build_plan_device(tah_plan* p) {
       std::vector<std::pair<int32_t, Tile>> tl;
       int64_t per = class_tile_elems(cls);
       int64_t values = 0;
-      // targets per interpolation unit: 2 along every dimension with R = 2
-      const int rb = cls >= kClsInterp ? (cls - kClsInterp) % 8 : 0;
-      const int64_t per_unit = (1 + (rb & 1)) * (1 + ((rb >> 1) & 1)) * (1 + ((rb >> 2) & 1));
       for (size_t gi = 0; gi < p->groups.size(); ++gi) {
         const PlanGroup& g = p->groups[gi];
         if (g.points == 0) continue;
         if (g.cls == kClsInterp) {
-          for (int32_t ci : g.wclasses) {
-            const PlanWClass& c = p->wclasses[ci];
-            if (c.cls != cls) continue;
-            const int32_t nun = c.nunits();
-            int32_t nu, nvt;
-            if (vfast) {
-              nvt = std::min(nvars, kThreads);
-              nu = std::max(1, kThreads / nvt);
-            } else {
-              nu = std::min(nun, kThreads);
-              nvt = std::max(1, std::min(nvars, kThreads / nu));
-            }
-            for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t) {
-              if (run_in[gi][t] != k) continue;
+          // A transfer's classes one after another, so that a merged
+          // launch runs the tiles that share its windows together.
+          for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t) {
+            if (run_in[gi][t] != k) continue;
+            for (int32_t ci : g.wclasses) {
+              const PlanWClass& c = p->wclasses[ci];
+              if (launch_of(c.cls) != cls) continue;
+              // targets per unit: 2 along every dimension with R = 2
+              const int rb = (c.cls - kClsInterp) % 8;
+              const int64_t per_unit = (1 + (rb & 1)) * (1 + ((rb >> 1) & 1)) * (1 + ((rb >> 2) & 1));
+              const int32_t nun = c.nunits();
+              int32_t nu, nvt;
+              if (vfast) {
+                nvt = std::min(nvars, kThreads);
+                nu = std::max(1, kThreads / nvt);
+              } else {
+                nu = std::min(nun, kThreads);
+                nvt = std::max(1, std::min(nvars, kThreads / nu));
+              }
               for (int32_t v0 = 0; v0 < nvars; v0 += nvt)
                 for (int32_t u0 = 0; u0 < nun; u0 += nu) {
                   tl.push_back({g.tblocks[t], Tile{0, ci, t, u0, std::min(nu, nun - u0), v0,
@@ -366,7 +381,7 @@ void build_plan_device(tah_plan* p) {
                 }
             }
           }
-        } else if (g.cls == cls) {
+        } else if (group_launch(g.cls) == cls) {
           int64_t elems = int64_t(g.points) * nvars;
           for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t) {
             if (run_in[gi][t] != k) continue;
@@ -388,12 +403,16 @@ void build_plan_device(tah_plan* p) {
     std::stable_sort(ls.begin(), ls.end(), [](const Launch& x, const Launch& y) { return x.count > y.count; });
     p->launches.push_back(std::move(ls));
   }
+  if (p->dtiles) {
+    // a previous tile list may still be read by queued fills
+    HIPCHECK(hipDeviceSynchronize());
+    HIPCHECK(hipFree(p->dtiles));
+    p->dtiles = nullptr;
+  }
   if (!p->tiles.empty()) {
     HIPCHECK(hipMalloc(&p->dtiles, sizeof(Tile) * p->tiles.size()));
     HIPCHECK(hipMemcpy(p->dtiles, p->tiles.data(), sizeof(Tile) * p->tiles.size(), hipMemcpyHostToDevice));
   }
-  HIPCHECK(hipMalloc(&p->djob, sizeof(Job)));
-  std::memset(&p->last_job, 0, sizeof(Job));
 }
 
 Job plan_job(const tah_plan* p, double* work, const double* factors) {

@@ -34,7 +34,11 @@ namespace tah {
 // Kernel classes.
 constexpr int kClsGeneric = 0;
 constexpr int kClsCopy = 1;
+// The launch class of copies at CPT values per thread.
+constexpr int kClsCopyLaunch = 900;
+inline int copy_launch(int cpt) { return kClsCopyLaunch + cpt; }
 constexpr int kClsInterp = 2;  // + 8*(index into kInterpTable) + rbits
+constexpr int kClsMerged = 1000;  // + index into kInterpTable: every rbits in one launch
 
 // (P, SHIFT) of each specialised interpolation kernel family.
 extern const std::array<std::array<int, 2>, 8> kInterpTable;
@@ -100,6 +104,14 @@ struct tah_plan {
   // one after another on the caller's stream (0, the default; tah_tune
   // picks per plan).
   int32_t fork = 0;
+  // Run every unit class of one order and shift in one launch (1, the
+  // default) or each class in a launch of its own (0).
+  int32_t merge = 1;
+  // Move phase-1 transfers that no later stage reads or overwrites into the
+  // last stage group (1, the default) or keep them in phase 1 (0).
+  int32_t defer = 1;
+  // Values per thread of the copy kernel's tiles: 4, 8 (default) or 16.
+  int32_t copy_per_thread = 8;
   std::vector<hipEvent_t> events;  // fork/join events, created on first use
   // The adjoint's inverted schedule, built on first use (adjoint.hip).
   tah::AdjointTables* adj = nullptr;
@@ -108,10 +120,12 @@ struct tah_plan {
 
 namespace tah {
 // A unit copy whose source row is affine in the target row along every
-// dimension: ca + cs*r with cs in {-1, 0, 1}.
-bool affine_copy(const PlanGroup& g, int32_t ca[3], int32_t cs[3]);
+// dimension: ca + cs*r with |cs| <= max_step.
+bool affine_copy(const PlanGroup& g, int32_t ca[3], int32_t cs[3], int max_step);
 // Split groups into classes, upload tables, build tiles and launches.
 void build_plan_device(tah_plan* p);
+// (Re)build the tiles and launches for the plan's options.
+void build_launches(tah_plan* p);
 // The Job of one fill of `work` by plan p.
 Job plan_job(const tah_plan* p, double* work, const double* factors);
 // Streams of the context (created on first use).

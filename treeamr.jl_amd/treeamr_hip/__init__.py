@@ -16,6 +16,6 @@ the CPU is refused by fill_ghosts.
 from .core import (  # noqa: F401
     Forest, MortonKey, Operators, FieldSet, GhostSchedule, fill_ghosts, fill_ghosts_adjoint,
     PointValue, Conservative, NoParity, EvenParity, OddParity, cellcentered, vertexcentered,
-    Context, default_context, Plan, fill_ghosts_batched, FillGraph, fill_ghosts_timed, kernel_class_name,
+    Context, default_context, Plan, fill_ghosts_batched, FillGraph, fill_ghosts_timed, kernel_class_name, tune,
 )
 from ._abi import TahError, StaleScheduleError, LayoutError, UnsupportedError  # noqa: F401

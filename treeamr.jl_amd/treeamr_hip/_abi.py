@@ -47,6 +47,11 @@ I32P = C.POINTER(C.c_int32)
 DP = C.POINTER(C.c_double)
 
 # (name, argtypes) for every entry point of include/tah.h.
+class tah_plan_options(C.Structure):
+    _fields_ = [("fork", C.c_int32), ("merge", C.c_int32), ("defer", C.c_int32),
+                ("copy_per_thread", C.c_int32)]
+
+
 SIGNATURES = {
     "tah_lagrange_weights": [C.c_int32, C.c_int64, C.c_int64, DP],
     "tah_conservative_weights": [C.c_int32, DP, DP],
@@ -83,7 +88,9 @@ SIGNATURES = {
     "tah_fill_timed": [P, P, P, P, I32P, I32P, C.POINTER(C.c_int64), C.POINTER(C.c_float)],
     "tah_kernel_class_name": [C.c_int32, C.c_char_p, C.c_int32],
     "tah_fill_adjoint": [P, P, P, P],
-    "tah_tune": [P, P, P, C.c_char_p],
+    "tah_plan_get_options": [P, C.POINTER(tah_plan_options)],
+    "tah_plan_set_options": [P, C.POINTER(tah_plan_options)],
+    "tah_tune": [P, P, P, C.c_char_p, C.POINTER(C.c_double)],
 }
 
 _lib = None

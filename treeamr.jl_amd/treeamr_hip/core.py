@@ -320,6 +320,21 @@ class Plan:
         check(lib().tah_plan_values(self._h, C.byref(v)))
         return v.value
 
+    @property
+    def options(self):
+        """The launch options: dict(fork=, merge=, defer=) each 0 or 1, and
+        copy_per_thread 4, 8 or 16."""
+        o = _abi.tah_plan_options()
+        check(lib().tah_plan_get_options(self._h, C.byref(o)))
+        return dict(fork=o.fork, merge=o.merge, defer=o.defer, copy_per_thread=o.copy_per_thread)
+
+    def set_options(self, fork=None, merge=None, defer=None, copy_per_thread=None):
+        cur = self.options
+        pick = lambda name, v: cur[name] if v is None else int(v)
+        o = _abi.tah_plan_options(pick("fork", fork), pick("merge", merge), pick("defer", defer),
+                                  pick("copy_per_thread", copy_per_thread))
+        check(lib().tah_plan_set_options(self._h, C.byref(o)))
+
 
 def _layout_of(desc_ptr, nvars, kind):
     L = _abi.tah_layout()
@@ -463,6 +478,22 @@ def fill_ghosts(fs, schedule, *, boundary=None, method="staged", ctx=None):
     fn = lib().tah_fill if method == "staged" else lib().tah_fill_plain
     check(fn(plan._h, C.c_void_p(fs.work.data_ptr()), fptr, _stream_ptr(fs)))
     return fs
+
+
+def tune(fs, schedule, cache_path=None, *, ctx=None):
+    """tah_tune: time every combination of the plan's launch options on
+    fs.work and keep the fastest (cached in cache_path by mesh signature).
+    Returns (options, median ms of the choice, or None when taken from the
+    cache)."""
+    _check_fill(fs, schedule)
+    if fs.work.device.type != "cuda":
+        raise RuntimeError("tune runs on the GPU only")
+    plan = schedule.plan(fs.layout, ctx)
+    fptr = C.c_void_p(fs.factors.data_ptr()) if fs.factors is not None else None
+    ms = C.c_double()
+    path = None if cache_path is None else str(cache_path).encode()
+    check(lib().tah_tune(plan._h, C.c_void_p(fs.work.data_ptr()), fptr, path, C.byref(ms)))
+    return plan.options, (None if ms.value < 0 else ms.value)
 
 
 def kernel_class_name(cls):

@@ -199,13 +199,13 @@ def random_forest(seed, max_blocks=4096, max_bytes=None):
     nsets = int(rng.integers(1, 4))
     nvars = [int(rng.integers(2, 11)) for _ in range(nsets)]
     centering = "vertex" if rng.integers(0, 2) == 1 else "cell"
-    c = 1 if centering == "vertex" else 0
+    cv = 1 if centering == "vertex" else 0
 
     def too_big(f):
         if f.nleaves > max_blocks:
             return True
         if max_bytes is not None:
-            per_block = (N + 2 * G + c) ** 3 * 8 * sum(nvars)
+            per_block = (N + 2 * G + cv) ** 3 * 8 * sum(nvars)
             return f.nleaves * per_block > max_bytes
         return False
 
