@@ -118,6 +118,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-config2", action="store_true")
+    ap.add_argument("--no-reference-meshes", action="store_true")
     ap.add_argument("--config2-repeats", type=int, default=100)
     ap.add_argument("--batched-steps", type=int, default=3)
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
@@ -273,6 +274,42 @@ def main():
         log(f"[bench] config 2: {nested}")
         del g2obj, fs2, s2
 
+    refmeshes = None
+    if not args.no_reference_meshes:
+        refmeshes = []
+        cases = [
+            ("uniform 8^3 roots of 16^3, periodic, vertex, G=3, 20 vars, PointValue(4) (bench/copies.jl)",
+             lambda: W.uniform_vertex(N=16), 1.716, "CODE.md:2992-2997"),
+            ("uniform 8^3 roots of 32^3, periodic, vertex, G=3, 20 vars, PointValue(4) (bench/copies.jl)",
+             lambda: W.uniform_vertex(N=32), 4.969, "CODE.md:2992-2997"),
+            ("nested reflecting octant, 232 blocks of 32^3, vertex, G=3, 20 vars, PointValue(6) "
+             "(bench/nested.jl; outer-face hook regions not written here)",
+             lambda: W.nested_octant(N=32), 7.90, "CODE.md:2752-2760"),
+            ("two-level, 960 blocks of 32^3, periodic, cell, G=2, 2 vars, PointValue(4) (bench/gpu.jl)",
+             lambda: W.two_level(N=32), 2.8, "CODE.md:5532-5548"),
+        ]
+        for name, make, h200_ms, src in cases:
+            log(f"[bench] reference mesh: {name}")
+            fR, opsR, kwR = make()
+            kwR = dict(kwR)
+            nvR = kwR.pop("nvars")
+            rec = {"workload": name, "blocks": fR.nleaves, "h200_ms": h200_ms, "h200_source": src}
+            for layout in ("arena", "dense"):
+                fsR = FieldSet(fR, nvR, device="cuda", layout=layout, fill=0.0, **kwR)
+                W.fill_uniform(fsR, 3)
+                sR = GhostSchedule(fsR, opsR)
+                vR = sR.points() * nvR
+                gR = FillGraph([(fsR, sR)])
+                tR = time_fill(gR.launch, 20, args.warmup, stream)
+                rec["values_per_fill"] = vR
+                rec[f"{layout}_graph_ms"] = {"min": round(tR["min_ms"], 4), "median": round(tR["median_ms"], 4)}
+                rec[f"{layout}_GBs_at_min"] = round(16.0 * vR / (tR["min_ms"] * 1e-3) / 1e9, 1)
+                del gR, fsR, sR
+            rec["speedup_vs_h200_best_of"] = round(h200_ms / min(rec["arena_graph_ms"]["min"],
+                                                                  rec["dense_graph_ms"]["min"]), 2)
+            log(f"[bench] {rec}")
+            refmeshes.append(rec)
+
     cpu = None
     if not args.no_cpu:
         log("[bench] timing the CPU baseline (oracle)")
@@ -312,6 +349,7 @@ def main():
         "adjoint_over_forward": round(adjoint["mean_ms"] / t_ms, 3) if adjoint else None,
         "batched": batched,
         "config2": nested,
+        "reference_meshes": refmeshes,
         "schedule_build_s": round(build_s, 3),
         "roofline": {
             "kernel": dominant,

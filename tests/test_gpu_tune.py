@@ -31,14 +31,17 @@ def test_every_option_combination_fills_alike(which):
     torch.cuda.synchronize()
     ref = fs.work.clone()
     plan = s.plan(fs.layout)
-    for fork, merge, defer, cpt in itertools.product((0, 1), (0, 1), (0, 1), (4, 8, 16)):
-        print(f"[tune] {which} fork={fork} merge={merge} defer={defer} copy_per_thread={cpt}", flush=True)
-        plan.set_options(fork=fork, merge=merge, defer=defer, copy_per_thread=cpt)
+    for fork, merge, fuse, defer, cpt in itertools.product((0, 1), (0, 1), (0, 1), (0, 1), (4, 8, 16)):
+        if fuse and not merge:
+            continue
+        print(f"[tune] {which} fork={fork} merge={merge} fuse={fuse} defer={defer} copy_per_thread={cpt}",
+              flush=True)
+        plan.set_options(fork=fork, merge=merge, fuse=fuse, defer=defer, copy_per_thread=cpt)
         fs.work.copy_(init)
         fill_ghosts(fs, s)
         torch.cuda.synchronize()
-        assert torch.equal(fs.work.view(torch.int64), ref.view(torch.int64)), (fork, merge, defer, cpt)
-    plan.set_options(fork=0, merge=1, defer=1, copy_per_thread=8)
+        assert torch.equal(fs.work.view(torch.int64), ref.view(torch.int64)), (fork, merge, fuse, defer, cpt)
+    plan.set_options(fork=0, merge=1, fuse=0, defer=1, copy_per_thread=8)
 
 
 def test_tune_caches_by_signature(tmp_path):

@@ -164,13 +164,9 @@ __global__ void __launch_bounds__(kThreads) generic_kernel(const Job* __restrict
 constexpr int kCopyPerThread = 8;
 constexpr int kCopyTile = kThreads * kCopyPerThread;
 
-#ifdef TAH_DEFINE_KERNELS
 // CPT values per thread (the tile shape a plan option picks: 4, 8 or 16).
 template <int CPT>
-__global__ void __launch_bounds__(kThreads) copy_kernel(const Job* __restrict__ jobs,
-                                                        const Tile* __restrict__ tiles) {
-  const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-  const Job& job = jobs[tile.job];
+__device__ __forceinline__ void copy_body(const Job& job, const Tile& tile) {
   const DevGroup& g = job.groups[tile.item];
   // Offsets within one block fit 32 bits (plans refuse larger blocks).
   const int32_t s0 = int32_t(job.stride[0]), s1 = int32_t(job.stride[1]),
@@ -204,6 +200,13 @@ __global__ void __launch_bounds__(kThreads) copy_kernel(const Job* __restrict__ 
   }
 }
 
+#ifdef TAH_DEFINE_KERNELS
+template <int CPT>
+__global__ void __launch_bounds__(kThreads) copy_kernel(const Job* __restrict__ jobs,
+                                                        const Tile* __restrict__ tiles) {
+  const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  copy_body<CPT>(jobs[tile.job], tile);
+}
 #endif  // TAH_DEFINE_KERNELS
 
 // ---------------------------------------------------------------------------
@@ -331,6 +334,30 @@ __global__ void __launch_bounds__(kThreads) interp_all_kernel(const Job* __restr
                                                               const Tile* __restrict__ tiles) {
   const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const Job& job = jobs[tile.job];
+  const DevWClass& wc = job.wclasses[tile.item];
+  switch (uniform(wc.rbits)) {
+    case 0: interp_body<P, SHIFT, 1, 1, 1>(job, tile, wc); break;
+    case 1: interp_body<P, SHIFT, 2, 1, 1>(job, tile, wc); break;
+    case 2: interp_body<P, SHIFT, 1, 2, 1>(job, tile, wc); break;
+    case 3: interp_body<P, SHIFT, 2, 2, 1>(job, tile, wc); break;
+    case 4: interp_body<P, SHIFT, 1, 1, 2>(job, tile, wc); break;
+    case 5: interp_body<P, SHIFT, 2, 1, 2>(job, tile, wc); break;
+    case 6: interp_body<P, SHIFT, 1, 2, 2>(job, tile, wc); break;
+    default: interp_body<P, SHIFT, 2, 2, 2>(job, tile, wc); break;
+  }
+}
+
+// A stage group's copies (tiles marked pad = 1) and its interpolations of
+// one order and shift in one launch, the tile kind picked per workgroup.
+template <int P, int SHIFT, int CPT>
+__global__ void __launch_bounds__(kThreads) fused_kernel(const Job* __restrict__ jobs,
+                                                         const Tile* __restrict__ tiles) {
+  const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  const Job& job = jobs[tile.job];
+  if (uniform(tile.pad) == 1) {
+    copy_body<CPT>(job, tile);
+    return;
+  }
   const DevWClass& wc = job.wclasses[tile.item];
   switch (uniform(wc.rbits)) {
     case 0: interp_body<P, SHIFT, 1, 1, 1>(job, tile, wc); break;

@@ -322,17 +322,18 @@ class Plan:
 
     @property
     def options(self):
-        """The launch options: dict(fork=, merge=, defer=) each 0 or 1, and
-        copy_per_thread 4, 8 or 16."""
+        """The launch options: dict(fork=, merge=, defer=, fuse=) each 0 or 1,
+        and copy_per_thread 4, 8 or 16."""
         o = _abi.tah_plan_options()
         check(lib().tah_plan_get_options(self._h, C.byref(o)))
-        return dict(fork=o.fork, merge=o.merge, defer=o.defer, copy_per_thread=o.copy_per_thread)
+        return dict(fork=o.fork, merge=o.merge, defer=o.defer, copy_per_thread=o.copy_per_thread,
+                    fuse=o.fuse)
 
-    def set_options(self, fork=None, merge=None, defer=None, copy_per_thread=None):
+    def set_options(self, fork=None, merge=None, defer=None, copy_per_thread=None, fuse=None):
         cur = self.options
         pick = lambda name, v: cur[name] if v is None else int(v)
         o = _abi.tah_plan_options(pick("fork", fork), pick("merge", merge), pick("defer", defer),
-                                  pick("copy_per_thread", copy_per_thread))
+                                  pick("copy_per_thread", copy_per_thread), pick("fuse", fuse))
         check(lib().tah_plan_set_options(self._h, C.byref(o)))
 
 

@@ -84,6 +84,50 @@ def fill_waves(fs, seed, noise=1e-3):
     return fs
 
 
+# --- the reference's own benchmark meshes (H200 timings published with them) ---
+
+def uniform_vertex(N=16, roots=8, nvars=20, G=3, p=4):
+    """bench/copies.jl's default: a uniform periodic mesh of roots^3 blocks of
+    N^3, vertex-centred, G=3, 20 variables, PointValue(p) (p=4)."""
+    f = Forest((roots,) * 3, N=N, periodic=(True,) * 3, extents=((0.0, 1.0),) * 3)
+    return f, Operators(prolongation=p, restriction=p), dict(nvars=nvars, G=G, centering="vertex")
+
+
+def nested_octant(N=32, nvars=20):
+    """bench/nested.jl's mesh: the reflecting octant [0,32]^3 (low faces
+    reflecting, high faces outer), 4^3 roots, the cubes [0,16)^3, [0,8)^3 and
+    [0,4)^3 refined to levels 1-3 (232 leaves), vertex-centred, G=3,
+    PointValue(6,6); odd variables (1-based) scalars, even ones a vector
+    component odd along dimension mod1(v/2, 3)."""
+    f = Forest((4,) * 3, N=N, reflecting=((True, False),) * 3, extents=((0.0, 32.0),) * 3)
+    for lvl in range(1, 4):
+        edge = 32.0 / 2 ** lvl
+        lows, h, level = f.block_geometry()
+        hi = lows + (h * f.N)[:, None]
+        f.refine([i for i in range(f.nleaves) if level[i] == lvl - 1 and np.all(hi[i] <= edge)])
+    f.balance()
+    parity = []
+    for v in range(1, nvars + 1):
+        if v % 2 == 1:
+            parity.append((EvenParity,) * 3)
+        else:
+            d = ((v // 2) - 1) % 3
+            parity.append(tuple(OddParity if e == d else EvenParity for e in range(3)))
+    return f, Operators(prolongation=6, restriction=6), dict(nvars=nvars, G=3, centering="vertex",
+                                                               parity=tuple(parity))
+
+
+def two_level(N=32, roots=8, nvars=2):
+    """bench/gpu.jl's mesh: roots^3 periodic blocks of N^3 with the middle half
+    of the domain refined once (960 blocks at 8^3 roots), cell-centred, G=2,
+    2 variables, PointValue(4,4)."""
+    f = Forest((roots,) * 3, N=N, periodic=(True,) * 3, extents=((0.0, 1.0),) * 3)
+    c = f.block_centers()
+    f.refine([i for i in range(f.nleaves) if np.all((c[i] > 0.25) & (c[i] < 0.75))])
+    f.balance()
+    return f, Operators(prolongation=4, restriction=4), dict(nvars=nvars, G=2)
+
+
 def _spheres(seed, n, rlo, rhi):
     rng = np.random.default_rng(seed)
     centres = rng.uniform(0.15, 0.85, size=(n, 3))
