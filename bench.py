@@ -222,7 +222,8 @@ def main():
         log(f"[bench] batch, graph-captured batched fill: {bgraph}")
         batched = {
             "workload": "64 independent forests, seeds 2..65 (1^3-4^3 roots, N in {8,16,32}, 1-4 levels, "
-                        "G in {2,3}, 1-3 field sets of 2-10 vars, PointValue(2G), periodic, <=2 GiB each)",
+                        "G in {2,3}, cell or vertex, 1-3 field sets of 2-10 vars, PointValue(2G), periodic, "
+                        "<=2 GiB each)",
             "field_sets": len(pairs), "blocks": bblocks, "values_per_fill": bvalues,
             "plain_sequential_ms": round(bplain["mean_ms"], 3),
             "graph_batched_ms": round(bgraph["mean_ms"], 3),
