@@ -179,6 +179,36 @@ __device__ __forceinline__ void copy_body(const Job& job, const Tile& tile) {
   const int32_t tb0 = s0 * g.tfirst[0] + s1 * g.tfirst[1] + s2 * g.tfirst[2];
   const int32_t sb0 = s0 * g.ca[0] + s1 * g.ca[1] + s2 * g.ca[2];
   const int32_t ds0 = s0 * g.cs[0], ds1 = s1 * g.cs[1], ds2 = s2 * g.cs[2];
+  if (job.vfast && s3 == 1 && (nvars & 1) == 0) {
+    // Variables innermost and even in number: values 2e and 2e+1 are one
+    // point's neighbouring variables, 16 contiguous bytes in source and
+    // target alike (offsets even, blocks 256-byte aligned): one 16-byte
+    // load and store for both.
+    constexpr int CP = CPT / 2;
+    typedef double pair_t __attribute__((ext_vector_type(2)));
+    int32_t dst2[CP], var2[CP];
+    pair_t val2[CP];
+#pragma unroll
+    for (int r = 0; r < CP; ++r) {
+      uint32_t e = uint32_t(tile.e0) + 2u * (uint32_t(r * kThreads) + threadIdx.x);
+      const bool ok = e < total;
+      e = ok ? e : 0u;
+      int32_t i, j, k, v;
+      split_element(g, job, e, i, j, k, v);
+      dst2[r] = ok ? v + tb0 + s0 * i + s1 * j + s2 * k : -1;
+      var2[r] = v;
+      val2[r] = *reinterpret_cast<const pair_t*>(sbase + (v + sb0 + ds0 * i + ds1 * j + ds2 * k));
+    }
+#pragma unroll
+    for (int r = 0; r < CP; ++r) {
+      if (dst2[r] < 0) continue;
+      pair_t out;
+      out.x = scaled(__dadd_rn(0.0, val2[r].x), job.factors, g.factorcol, nvars, var2[r]);
+      out.y = scaled(__dadd_rn(0.0, val2[r].y), job.factors, g.factorcol, nvars, var2[r] + 1);
+      __builtin_nontemporal_store(out, reinterpret_cast<pair_t*>(tbase + dst2[r]));
+    }
+    return;
+  }
   int32_t dst[CPT], var[CPT];
   double val[CPT];
 #pragma unroll
@@ -193,10 +223,12 @@ __device__ __forceinline__ void copy_body(const Job& job, const Tile& tile) {
     var[r] = v;
     val[r] = sbase[vo + sb0 + ds0 * i + ds1 * j + ds2 * k];
   }
+  // Ghosts are not read again within the fill: stream them past the caches.
 #pragma unroll
   for (int r = 0; r < CPT; ++r) {
     if (dst[r] < 0) continue;
-    tbase[dst[r]] = scaled(__dadd_rn(0.0, val[r]), job.factors, g.factorcol, nvars, var[r]);
+    __builtin_nontemporal_store(scaled(__dadd_rn(0.0, val[r]), job.factors, g.factorcol, nvars, var[r]),
+                                tbase + dst[r]);
   }
 }
 
@@ -329,9 +361,11 @@ __global__ void __launch_bounds__(kThreads) interp_kernel(const Job* __restrict_
 // per tile (a branch uniform over the workgroup), so that the tiles of all
 // classes of one transfer -- which read the same source windows -- can run
 // side by side and meet in one L2.
+// Registers capped for seven waves per SIMD (72 VGPRs: no spills at order 6),
+// where the unconstrained octet body takes 94 and leaves room for five.
 template <int P, int SHIFT>
-__global__ void __launch_bounds__(kThreads) interp_all_kernel(const Job* __restrict__ jobs,
-                                                              const Tile* __restrict__ tiles) {
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8)))
+interp_all_kernel(const Job* __restrict__ jobs, const Tile* __restrict__ tiles) {
   const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const Job& job = jobs[tile.job];
   const DevWClass& wc = job.wclasses[tile.item];
