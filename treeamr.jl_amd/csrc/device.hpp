@@ -78,8 +78,10 @@ struct alignas(16) Job {
 };
 
 // A slice of work. Copy and generic tiles: group `item`, transfer, flat
-// elements from e0. Interpolation tiles: class `item`, transfer, units
-// e0 .. e0+nu-1 times variables v0 .. v0+nvt-1.
+// elements from e0. Interpolation tiles: class `item`, nu units from unit e0
+// of transfer `transfer` on through the next transfers (unit e0+i is unit
+// (e0+i) mod nunits of transfer transfer + (e0+i) div nunits), times
+// variables v0 .. v0+nvt-1.
 struct alignas(16) Tile {
   int32_t job;
   int32_t item;

@@ -268,12 +268,20 @@ __device__ __forceinline__ void interp_body(const Job& job, const Tile& tile, co
     ul = t - vl * tile.nu;
   }
   const int32_t v = tile.v0 + vl;
-  const int32_t* __restrict__ un = wc.units + 8 * (tile.e0 + ul);
+  // The tile's units run on across consecutive transfers of the group.
+  int32_t u = tile.e0 + ul, tr = tile.transfer;
+  const int32_t nun = wc.nunits;
+  if (u >= nun) {
+    const int32_t q = u / nun;
+    tr += q;
+    u -= q * nun;
+  }
+  const int32_t* __restrict__ un = wc.units + 8 * u;
   const int32_t s0 = int32_t(job.stride[0]), s1 = int32_t(job.stride[1]),
                 s2 = int32_t(job.stride[2]), s3 = int32_t(job.stride[3]);
   const int64_t s4 = job.stride[4];
   const int32_t ra[2] = {un[0], un[1]}, rb[2] = {un[2], un[3]}, rc[2] = {un[4], un[5]};
-  const double* __restrict__ src = job.work + s4 * g.sblocks[tile.transfer] +
+  const double* __restrict__ src = job.work + s4 * g.sblocks[tr] +
                                    (s3 * v + s0 * g.srcstart[0][ra[0]] + s1 * g.srcstart[1][rb[0]] +
                                     s2 * g.srcstart[2][rc[0]]);
   // The table is the same for every lane: read it through the constant
@@ -335,7 +343,7 @@ __device__ __forceinline__ void interp_body(const Job& job, const Tile& tile, co
           for (int c = 0; c < R2; ++c) asm volatile("" : "+v"(acc[a][b][c]));
     }
   }
-  double* __restrict__ dst = job.work + s4 * g.tblocks[tile.transfer] + s3 * v;
+  double* __restrict__ dst = job.work + s4 * g.tblocks[tr] + s3 * v;
   const int32_t nvars = job.nvars;
 #pragma unroll
   for (int c = 0; c < R2; ++c)

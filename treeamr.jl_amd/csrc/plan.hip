@@ -362,29 +362,41 @@ void build_launches(tah_plan* p) {
         if (g.cls == kClsInterp) {
           // A transfer's classes one after another, so that a merged
           // launch runs the tiles that share its windows together.
-          for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t) {
-            if (run_in[gi][t] != k) continue;
-            for (int32_t ci : g.wclasses) {
-              const PlanWClass& c = p->wclasses[ci];
-              if (launch_of(c.cls) != cls) continue;
-              // targets per unit: 2 along every dimension with R = 2
-              const int rb = (c.cls - kClsInterp) % 8;
-              const int64_t per_unit = (1 + (rb & 1)) * (1 + ((rb >> 1) & 1)) * (1 + ((rb >> 2) & 1));
-              const int32_t nun = c.nunits();
-              int32_t nu, nvt;
-              if (vfast) {
-                nvt = std::min(nvars, kThreads);
-                nu = std::max(1, kThreads / nvt);
-              } else {
-                nu = std::min(nun, kThreads);
-                nvt = std::max(1, std::min(nvars, kThreads / nu));
+          // A tile takes nu consecutive units of the list (transfer, unit)
+          // over a run of consecutive transfers of this stage group, so
+          // that tiles stay full across transfers with few units.
+          for (int32_t ci : g.wclasses) {
+            const PlanWClass& c = p->wclasses[ci];
+            if (launch_of(c.cls) != cls) continue;
+            // targets per unit: 2 along every dimension with R = 2
+            const int rb = (c.cls - kClsInterp) % 8;
+            const int64_t per_unit = (1 + (rb & 1)) * (1 + ((rb >> 1) & 1)) * (1 + ((rb >> 2) & 1));
+            const int32_t nun = c.nunits();
+            int32_t nu, nvt;
+            if (vfast) {
+              nvt = std::min(nvars, kThreads);
+              nu = std::max(1, kThreads / nvt);
+            } else {
+              nvt = 1;
+              nu = kThreads;
+            }
+            const int32_t nt = int32_t(g.tblocks.size());
+            for (int32_t t = 0; t < nt;) {
+              if (run_in[gi][t] != k) {
+                ++t;
+                continue;
               }
+              int32_t t1 = t;
+              while (t1 < nt && run_in[gi][t1] == k) ++t1;
+              const int64_t count = int64_t(t1 - t) * nun;  // units of the run
               for (int32_t v0 = 0; v0 < nvars; v0 += nvt)
-                for (int32_t u0 = 0; u0 < nun; u0 += nu) {
-                  tl.push_back({g.tblocks[t], Tile{0, ci, t, u0, std::min(nu, nun - u0), v0,
-                                                   std::min(nvt, nvars - v0), 0}});
-                  values += int64_t(std::min(nu, nun - u0)) * std::min(nvt, nvars - v0) * per_unit;
+                for (int64_t i = 0; i < count; i += nu) {
+                  const int32_t tt = t + int32_t(i / nun), uu = int32_t(i % nun);
+                  const int32_t n = int32_t(std::min<int64_t>(nu, count - i));
+                  tl.push_back({g.tblocks[tt], Tile{0, ci, tt, uu, n, v0, std::min(nvt, nvars - v0), 0}});
+                  values += int64_t(n) * std::min(nvt, nvars - v0) * per_unit;
                 }
+              t = t1;
             }
           }
         } else if (group_launch(g.cls) == cls) {
