@@ -256,17 +256,16 @@ tah_status tah_fill_adjoint(tah_plan* p, double* adj, const double* factors, voi
  * streams (1) or in order on the fill's stream (0, default); merge runs
  * every interpolation unit class of one order in one launch (1, default) or
  * each in its own (0); defer moves phase-1 transfers that no later stage
- * reads or overwrites into the last stage group (1, default); fuse runs
- * a stage group's copies and its interpolations of one order as one launch
- * of interleaved tiles (0 default). Setting merge, defer, copy_per_thread or
- * fuse rebuilds the plan's tiles (it waits for the device first);
+ * reads or overwrites into the last stage group (1, default); fuse is
+ * refused with TAH_EUNSUPPORTED unless 0. Setting merge, defer or
+ * copy_per_thread rebuilds the plan's tiles (it waits for the device first);
  * batches and graphs made earlier keep the options they were made with. */
 typedef struct tah_plan_options {
   int32_t fork;
   int32_t merge;
   int32_t defer;
   int32_t copy_per_thread; /* the copy kernel's tile: 4, 8 (default) or 16 values per thread */
-  int32_t fuse;            /* a stage group's copies and interpolations in one launch (needs merge) */
+  int32_t fuse;            /* must be 0: the fused copy-and-interpolation launch is switched off */
   int32_t reserved_[3];
 } tah_plan_options;
 tah_status tah_plan_get_options(const tah_plan* p, tah_plan_options* out);

@@ -31,9 +31,7 @@ def test_every_option_combination_fills_alike(which):
     torch.cuda.synchronize()
     ref = fs.work.clone()
     plan = s.plan(fs.layout)
-    for fork, merge, fuse, defer, cpt in itertools.product((0, 1), (0, 1), (0, 1), (0, 1), (4, 8, 16)):
-        if fuse and not merge:
-            continue
+    for fork, merge, fuse, defer, cpt in itertools.product((0, 1), (0, 1), (0,), (0, 1), (4, 8, 16)):
         print(f"[tune] {which} fork={fork} merge={merge} fuse={fuse} defer={defer} copy_per_thread={cpt}",
               flush=True)
         plan.set_options(fork=fork, merge=merge, fuse=fuse, defer=defer, copy_per_thread=cpt)
@@ -64,3 +62,33 @@ def test_tune_caches_by_signature(tmp_path):
     torch.cuda.synchronize()
     assert torch.equal(once.view(torch.int64), fs.work.view(torch.int64))
     del init
+
+
+def test_fused_launch_refused():
+    """The fused copy-and-interpolation launch is switched off: asking for it
+    is an error raised on the host, before any launch."""
+    from treeamr_hip import UnsupportedError
+    fs, s = _case("cell")
+    plan = s.plan(fs.layout)
+    with pytest.raises(UnsupportedError):
+        plan.set_options(fuse=1)
+    assert plan.options["fuse"] == 0
+
+
+@pytest.mark.skip(reason="the fused copy-and-interpolation launch faulted the GPU (illegal address) in a timing "
+                         "run on the large forest; its cause was not found, so the launch is switched off and "
+                         "this case is not run")
+@pytest.mark.parametrize("which", ["cell", "vertex"])
+def test_fused_launch_fills_alike(which):
+    fs, s = _case(which)
+    init = fs.work.clone()
+    fill_ghosts(fs, s)
+    torch.cuda.synchronize()
+    ref = fs.work.clone()
+    plan = s.plan(fs.layout)
+    for defer, cpt in itertools.product((0, 1), (4, 8, 16)):
+        plan.set_options(merge=1, fuse=1, defer=defer, copy_per_thread=cpt)
+        fs.work.copy_(init)
+        fill_ghosts(fs, s)
+        torch.cuda.synchronize()
+        assert torch.equal(fs.work.view(torch.int64), ref.view(torch.int64))

@@ -51,27 +51,6 @@ void launch_interp_r(int rbits, dim3 grid, hipStream_t s, const Job* j, const Ti
 
 }  // namespace
 
-template <int P, int S>
-void launch_fused_p(int cpt, dim3 grid, hipStream_t s, const Job* j, const Tile* t) {
-  dim3 blk(kThreads);
-  if (cpt == 4) hipLaunchKernelGGL((fused_kernel<P, S, 4>), grid, blk, 0, s, j, t);
-  else if (cpt == 16) hipLaunchKernelGGL((fused_kernel<P, S, 16>), grid, blk, 0, s, j, t);
-  else hipLaunchKernelGGL((fused_kernel<P, S, 8>), grid, blk, 0, s, j, t);
-}
-
-void launch_fused(int id, int cpt, dim3 grid, hipStream_t s, const Job* j, const Tile* t) {
-  switch (id) {
-    case 0: launch_fused_p<2, 0>(cpt, grid, s, j, t); break;
-    case 1: launch_fused_p<3, 0>(cpt, grid, s, j, t); break;
-    case 2: launch_fused_p<4, 0>(cpt, grid, s, j, t); break;
-    case 3: launch_fused_p<5, 0>(cpt, grid, s, j, t); break;
-    case 4: launch_fused_p<6, 0>(cpt, grid, s, j, t); break;
-    case 5: launch_fused_p<2, 2>(cpt, grid, s, j, t); break;
-    case 6: launch_fused_p<4, 2>(cpt, grid, s, j, t); break;
-    default: launch_fused_p<6, 2>(cpt, grid, s, j, t); break;
-  }
-}
-
 void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles) {
   dim3 grid(ntiles), blk(kThreads);
   if (cls == kClsGeneric) {
@@ -82,9 +61,6 @@ void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, cons
     hipLaunchKernelGGL(copy_kernel<8>, grid, blk, 0, s, jobs, tiles);
   } else if (cls == copy_launch(16)) {
     hipLaunchKernelGGL(copy_kernel<16>, grid, blk, 0, s, jobs, tiles);
-  } else if (cls >= kClsFused) {
-    const int id = (cls - kClsFused) / 32, cpt = (cls - kClsFused) % 32;
-    launch_fused(id, cpt, grid, s, jobs, tiles);
   } else if (cls >= kClsMerged) {
     switch (cls - kClsMerged) {
       case 0: hipLaunchKernelGGL((interp_all_kernel<2, 0>), grid, blk, 0, s, jobs, tiles); break;
@@ -411,10 +387,6 @@ tah_status tah_kernel_class_name(int32_t cls, char* buf, int32_t len) {
       name = "generic_kernel";
     } else if (cls > kClsCopyLaunch && cls <= kClsCopyLaunch + 16) {
       name = "copy_kernel<" + std::to_string(cls - kClsCopyLaunch) + ">";
-    } else if (cls >= kClsFused && (cls - kClsFused) / 32 < int(kInterpTable.size())) {
-      const auto& e = kInterpTable[(cls - kClsFused) / 32];
-      name = "fused_kernel<" + std::to_string(e[0]) + ", " + std::to_string(e[1]) + ", " +
-             std::to_string((cls - kClsFused) % 32) + ">";
     } else if (cls >= kClsMerged && cls - kClsMerged < int(kInterpTable.size())) {
       const auto& e = kInterpTable[cls - kClsMerged];
       name = "interp_all_kernel<" + std::to_string(e[0]) + ", " + std::to_string(e[1]) + ">";

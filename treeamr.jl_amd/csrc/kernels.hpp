@@ -389,29 +389,6 @@ interp_all_kernel(const Job* __restrict__ jobs, const Tile* __restrict__ tiles) 
   }
 }
 
-// A stage group's copies (tiles marked pad = 1) and its interpolations of
-// one order and shift in one launch, the tile kind picked per workgroup.
-template <int P, int SHIFT, int CPT>
-__global__ void __launch_bounds__(kThreads) fused_kernel(const Job* __restrict__ jobs,
-                                                         const Tile* __restrict__ tiles) {
-  const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
-  const Job& job = jobs[tile.job];
-  if (uniform(tile.pad) == 1) {
-    copy_body<CPT>(job, tile);
-    return;
-  }
-  const DevWClass& wc = job.wclasses[tile.item];
-  switch (uniform(wc.rbits)) {
-    case 0: interp_body<P, SHIFT, 1, 1, 1>(job, tile, wc); break;
-    case 1: interp_body<P, SHIFT, 2, 1, 1>(job, tile, wc); break;
-    case 2: interp_body<P, SHIFT, 1, 2, 1>(job, tile, wc); break;
-    case 3: interp_body<P, SHIFT, 2, 2, 1>(job, tile, wc); break;
-    case 4: interp_body<P, SHIFT, 1, 1, 2>(job, tile, wc); break;
-    case 5: interp_body<P, SHIFT, 2, 1, 2>(job, tile, wc); break;
-    case 6: interp_body<P, SHIFT, 1, 2, 2>(job, tile, wc); break;
-    default: interp_body<P, SHIFT, 2, 2, 2>(job, tile, wc); break;
-  }
-}
 #endif  // TAH_DEFINE_KERNELS
 
 }  // namespace tah

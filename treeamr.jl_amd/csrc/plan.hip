@@ -415,42 +415,6 @@ void build_launches(tah_plan* p) {
                        });
       if (!tl.empty()) built.push_back(Built{cls, std::move(tl), values});
     }
-    // Fused: the copies and the interpolations of one order in one launch,
-    // their tiles dealt out in proportion, so that every CU runs both the
-    // HBM-bound copies and the FP64-bound interpolations at once.
-    if (p->fuse) {
-      int ci = -1, ii = -1, nint = 0;
-      for (size_t b = 0; b < built.size(); ++b) {
-        if (built[b].cls == copy_launch(p->copy_per_thread)) ci = int(b);
-        if (built[b].cls >= kClsMerged && built[b].cls < kClsFused) {
-          ii = int(b);
-          ++nint;
-        }
-      }
-      if (ci >= 0 && ii >= 0 && nint == 1) {
-        Built& C = built[ci];
-        Built& I = built[ii];
-        std::vector<std::pair<int32_t, Tile>> tl;
-        tl.reserve(C.tiles.size() + I.tiles.size());
-        const size_t nc = C.tiles.size(), ni = I.tiles.size();
-        size_t a = 0, b = 0;
-        while (a < nc || b < ni) {
-          // take a copy tile while copies are behind their share
-          if (a < nc && (b >= ni || a * ni <= b * nc)) {
-            Tile t = C.tiles[a++].second;
-            t.pad = 1;
-            tl.push_back({0, t});
-          } else {
-            tl.push_back({0, I.tiles[b++].second});
-          }
-        }
-        Built F{kClsFused + 32 * (I.cls - kClsMerged) + p->copy_per_thread, std::move(tl), C.values + I.values};
-        const int hi = std::max(ci, ii), lo = std::min(ci, ii);
-        built.erase(built.begin() + hi);
-        built.erase(built.begin() + lo);
-        built.push_back(std::move(F));
-      }
-    }
     for (Built& B : built) {
       Launch L{B.cls, int64_t(p->tiles.size()), int64_t(B.tiles.size()), B.values};
       for (const auto& e : B.tiles) p->tiles.push_back(e.second);

@@ -39,7 +39,6 @@ constexpr int kClsCopyLaunch = 900;
 inline int copy_launch(int cpt) { return kClsCopyLaunch + cpt; }
 constexpr int kClsInterp = 2;  // + 8*(index into kInterpTable) + rbits
 constexpr int kClsMerged = 1000;  // + index into kInterpTable: every rbits in one launch
-constexpr int kClsFused = 2000;   // + 32*(index into kInterpTable) + copy values per thread
 
 // (P, SHIFT) of each specialised interpolation kernel family.
 extern const std::array<std::array<int, 2>, 8> kInterpTable;
@@ -113,10 +112,6 @@ struct tah_plan {
   int32_t defer = 1;
   // Values per thread of the copy kernel's tiles: 4, 8 (default) or 16.
   int32_t copy_per_thread = 8;
-  // Run a stage group's copies and its interpolations of one order in one
-  // launch of interleaved tiles (1) or in launches of their own (0, the
-  // default). Needs merge.
-  int32_t fuse = 0;
   std::vector<hipEvent_t> events;  // fork/join events, created on first use
   // The adjoint's inverted schedule, built on first use (adjoint.hip).
   tah::AdjointTables* adj = nullptr;

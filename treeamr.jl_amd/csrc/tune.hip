@@ -1,9 +1,8 @@
 // Plan options and the autotuner.
 //
-// A plan's launches have five knobs: stage groups forked over the
+// A plan's launches have four knobs: stage groups forked over the
 // context's streams or run in order on one; the unit classes of one
-// interpolation order in one launch or a launch each; a stage group's copies
-// fused with its interpolations into one launch or not; phase-1 transfers
+// interpolation order in one launch or a launch each; phase-1 transfers
 // deferred into the last stage group or not; the copy kernel's tile (4, 8 or
 // 16 values per thread). tah_tune times every
 // combination on the caller's working array and keeps the fastest, cached
@@ -75,14 +74,14 @@ void set_options(tah_plan* p, const tah_plan_options& o) {
     fail("plan options fork, merge and defer are 0 or 1");
   if (o.copy_per_thread != 4 && o.copy_per_thread != 8 && o.copy_per_thread != 16)
     fail("copy_per_thread is 4, 8 or 16");
-  if (o.fuse < 0 || o.fuse > 1 || (o.fuse && !o.merge)) fail("fuse is 0 or 1, and 1 needs merge");
-  const bool rebuild = o.merge != p->merge || o.defer != p->defer ||
-                       o.copy_per_thread != p->copy_per_thread || o.fuse != p->fuse;
+  if (o.fuse != 0)
+    fail("the fused copy-and-interpolation launch is switched off in this build (fuse must be 0)",
+         kUnsupported);
+  const bool rebuild = o.merge != p->merge || o.defer != p->defer || o.copy_per_thread != p->copy_per_thread;
   p->fork = o.fork;
   p->merge = o.merge;
   p->defer = o.defer;
   p->copy_per_thread = o.copy_per_thread;
-  p->fuse = o.fuse;
   if (rebuild) build_launches(p);
 }
 
@@ -96,7 +95,8 @@ bool lookup(const std::string& path, const std::string& key, tah_plan_options& o
     std::string tag, k;
     int f, m, d, c, u;
     if (!(ss >> tag >> k >> f >> m >> d >> c >> u) || tag != "tah-tune-1" || k != key) continue;
-    o = tah_plan_options{f, m, d, c, u, {0, 0, 0}};
+    (void)u;  // the fused launch is switched off: never taken from a cache
+    o = tah_plan_options{f, m, d, c, 0, {0, 0, 0}};
     found = true;  // the last entry wins
   }
   return found;
@@ -113,7 +113,7 @@ tah_status tah_plan_get_options(const tah_plan* p, tah_plan_options* out) {
     out->merge = p->merge;
     out->defer = p->defer;
     out->copy_per_thread = p->copy_per_thread;
-    out->fuse = p->fuse;
+    out->fuse = 0;
     out->reserved_[0] = out->reserved_[1] = out->reserved_[2] = 0;
   });
 }
@@ -145,7 +145,7 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
-    tah_plan_options best{p->fork, p->merge, p->defer, p->copy_per_thread, p->fuse, {0, 0, 0}};
+    tah_plan_options best{p->fork, p->merge, p->defer, p->copy_per_thread, 0, {0, 0, 0}};
     float best_t = 1e30f;
     auto run = [&]() {
       tah_status st = tah_fill(p, work, factors, s);
@@ -154,7 +154,7 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
     try {
       for (int f = 0; f <= 1; ++f)
         for (int m = 0; m <= 1; ++m)
-          for (int u = 0; u <= m; ++u)
+          for (int u = 0; u <= 0; ++u)
             for (int d = 0; d <= 1; ++d)
               for (int c : {4, 8, 16}) {
             set_options(p, tah_plan_options{f, m, d, c, u, {0, 0, 0}});
