@@ -182,6 +182,79 @@ __global__ void __launch_bounds__(kThreads) adjoint_pass_kernel(const Job* __res
   }
 }
 
+// The gather with variables innermost and even in number: values 2e and
+// 2e+1 are one point's neighbouring variables, 16 contiguous bytes wherever
+// an entry reads them (copy targets and staged contributions alike have
+// even offsets), so each thread moves both with one access.
+__device__ __forceinline__ void gather_paired(const Job& job, const AdjCell& c, const AdjTile& tile,
+                                              const AdjEntry* __restrict__ entries, double* __restrict__ staged) {
+  typedef double pair_t __attribute__((ext_vector_type(2)));
+  const pair_t zero2 = {0.0, 0.0};
+  constexpr int K = kAdjPerThread / 2;
+  const int32_t s0 = int32_t(job.stride[0]), s1 = int32_t(job.stride[1]), s2 = int32_t(job.stride[2]);
+  const int32_t nvars = job.nvars;
+  const uint32_t total = uint32_t(c.npts) * uint32_t(nvars);
+  double* __restrict__ sb = job.work + job.stride[4] * c.sblock;
+  int32_t p[K][3], var[K], off[K];
+  bool ok[K];
+  pair_t tot[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint32_t e = uint32_t(tile.e0) + 2u * (uint32_t(k * kThreads) + threadIdx.x);
+    ok[k] = e < total;
+    const uint32_t ee = ok[k] ? e : 0;
+    const uint32_t r = job.dnv.div(ee);
+    var[k] = int32_t(ee - r * uint32_t(nvars));
+    const uint32_t q = c.dx[0].div(r);
+    const uint32_t zq = c.dx[1].div(q);
+    p[k][0] = c.lo[0] + int32_t(r - q * uint32_t(c.ext[0]));
+    p[k][1] = c.lo[1] + int32_t(q - zq * uint32_t(c.ext[1]));
+    p[k][2] = c.lo[2] + int32_t(zq);
+    off[k] = var[k] + s0 * p[k][0] + s1 * p[k][1] + s2 * p[k][2];
+    tot[k] = zero2;
+    if (ok[k] && !c.target) tot[k] = *reinterpret_cast<const pair_t*>(sb + off[k]);
+  }
+  const int32_t first = uniform(c.first), count = uniform(c.count);
+  for (int32_t q = 0; q < count; ++q) {
+    const AdjEntry& E = entries[first + q];
+    const bool copy = E.buf == 0;
+    double* __restrict__ src = (copy ? job.work : staged) + E.base;
+    const int32_t c0 = E.c[0], c1 = E.c[1], c2 = E.c[2];
+    const bool single = c0 * c1 * c2 == 1;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      if (!ok[k]) continue;
+      const int32_t o = E.a[0] * p[k][0] + E.a[1] * p[k][1] + E.a[2] * p[k][2] + E.av * var[k];
+      pair_t acc;
+      if (single) {
+        acc = *reinterpret_cast<const pair_t*>(src + o);
+        if (copy) __builtin_nontemporal_store(zero2, reinterpret_cast<pair_t*>(src + o));
+      } else {
+        acc = zero2;
+        for (int32_t r2 = 0; r2 < c2; ++r2)
+          for (int32_t r1 = 0; r1 < c1; ++r1)
+            for (int32_t r0 = 0; r0 < c0; ++r0) {
+              const int32_t oo = o + E.rs[0] * r0 + E.rs[1] * r1 + E.rs[2] * r2;
+              const pair_t x = *reinterpret_cast<const pair_t*>(src + oo);
+              acc.x = __dadd_rn(acc.x, x.x);
+              acc.y = __dadd_rn(acc.y, x.y);
+              if (copy) __builtin_nontemporal_store(zero2, reinterpret_cast<pair_t*>(src + oo));
+            }
+      }
+      if (E.factorcol) {
+        const double* f = job.factors + int64_t(nvars) * (E.factorcol - 1) + var[k];
+        acc.x = __dmul_rn(acc.x, f[0]);
+        acc.y = __dmul_rn(acc.y, f[1]);
+      }
+      tot[k].x = __dadd_rn(tot[k].x, acc.x);
+      tot[k].y = __dadd_rn(tot[k].y, acc.y);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if (ok[k]) *reinterpret_cast<pair_t*>(sb + off[k]) = tot[k];
+}
+
 __global__ void __launch_bounds__(kThreads) adjoint_gather_kernel(const Job* __restrict__ jobs,
                                                                   const AdjCell* __restrict__ cells,
                                                                   const AdjTile* __restrict__ tiles,
@@ -194,6 +267,10 @@ __global__ void __launch_bounds__(kThreads) adjoint_gather_kernel(const Job* __r
   const int32_t s0 = int32_t(job.stride[0]), s1 = int32_t(job.stride[1]),
                 s2 = int32_t(job.stride[2]), s3 = int32_t(job.stride[3]);
   const int32_t nvars = job.nvars;
+  if (job.vfast && s3 == 1 && (nvars & 1) == 0) {
+    gather_paired(job, c, tile, entries, staged);
+    return;
+  }
   const uint32_t npts = uint32_t(c.npts);
   const uint32_t total = npts * uint32_t(nvars);
   double* __restrict__ sb = job.work + job.stride[4] * c.sblock;
