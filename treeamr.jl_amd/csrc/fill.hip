@@ -51,8 +51,19 @@ void launch_interp_r(int rbits, dim3 grid, hipStream_t s, const Job* j, const Ti
 
 }  // namespace
 
-void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles) {
+void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles, int32_t bounded) {
   dim3 grid(ntiles), blk(kThreads);
+  if (bounded > 0) {
+    // a copy class from a bounded grid (a multiple of 8, at most ntiles)
+    if (bounded % 8 != 0 || uint32_t(bounded) > ntiles) fail("bad bounded grid for a copy launch");
+    dim3 g(static_cast<uint32_t>(bounded));
+    if (cls == copy_launch(4)) hipLaunchKernelGGL(copy_loop_kernel<4>, g, blk, 0, s, jobs, tiles, ntiles);
+    else if (cls == copy_launch(8)) hipLaunchKernelGGL(copy_loop_kernel<8>, g, blk, 0, s, jobs, tiles, ntiles);
+    else if (cls == copy_launch(16)) hipLaunchKernelGGL(copy_loop_kernel<16>, g, blk, 0, s, jobs, tiles, ntiles);
+    else fail("a bounded grid is only for copies");
+    HIPCHECK(hipGetLastError());
+    return;
+  }
   if (cls == kClsGeneric) {
     hipLaunchKernelGGL(generic_kernel, grid, blk, 0, s, jobs, tiles);
   } else if (cls == copy_launch(4)) {

@@ -239,6 +239,23 @@ __global__ void __launch_bounds__(kThreads) copy_kernel(const Job* __restrict__ 
   const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   copy_body<CPT>(jobs[tile.job], tile);
 }
+
+// The same copies from a bounded grid (a multiple of eight workgroups): the
+// workgroups of XCD x walk x's contiguous share of the tiles, so that the
+// copies hold only a few wave slots per CU and leave the rest to the
+// interpolations launched beside them on another stream.
+template <int CPT>
+__global__ void __launch_bounds__(kThreads) copy_loop_kernel(const Job* __restrict__ jobs,
+                                                             const Tile* __restrict__ tiles, uint32_t ntiles) {
+  constexpr uint32_t X = 8;
+  const uint32_t x = blockIdx.x % X, slot = blockIdx.x / X, nslots = gridDim.x / X;
+  const uint32_t per = ntiles / X, rem = ntiles % X;
+  const uint32_t lo = x * per + (x < rem ? x : rem), hi = lo + per + (x < rem ? 1u : 0u);
+  for (uint32_t t = lo + slot; t < hi; t += nslots) {
+    const Tile tile = tiles[t];
+    copy_body<CPT>(jobs[tile.job], tile);
+  }
+}
 #endif  // TAH_DEFINE_KERNELS
 
 // ---------------------------------------------------------------------------

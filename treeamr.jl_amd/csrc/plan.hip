@@ -416,7 +416,9 @@ void build_launches(tah_plan* p) {
       if (!tl.empty()) built.push_back(Built{cls, std::move(tl), values});
     }
     for (Built& B : built) {
-      Launch L{B.cls, int64_t(p->tiles.size()), int64_t(B.tiles.size()), B.values};
+      Launch L{B.cls, int64_t(p->tiles.size()), int64_t(B.tiles.size()), B.values, 0};
+      if (p->copy_wgs > 0 && B.cls == copy_launch(p->copy_per_thread) && int64_t(B.tiles.size()) >= 8)
+        L.grid = int32_t(std::min<int64_t>(p->copy_wgs, int64_t(B.tiles.size())) / 8 * 8);
       for (const auto& e : B.tiles) p->tiles.push_back(e.second);
       ls.push_back(L);
     }
@@ -465,7 +467,7 @@ const std::vector<hipStream_t>& ctx_streams(tah_ctx* c) {
 void run_launches(tah_ctx* c, const std::vector<Launch>& ls, hipStream_t s, const Job* jobs,
                   const Tile* tiles, hipEvent_t* ev, bool fork) {
   if (!fork || ls.size() <= 1) {
-    for (const Launch& L : ls) launch_class(L.cls, uint32_t(L.count), s, jobs, tiles + L.first);
+    for (const Launch& L : ls) launch_class(L.cls, uint32_t(L.count), s, jobs, tiles + L.first, L.grid);
     return;
   }
   const auto& streams = ctx_streams(c);
@@ -473,7 +475,7 @@ void run_launches(tah_ctx* c, const std::vector<Launch>& ls, hipStream_t s, cons
   for (size_t i = 0; i < ls.size(); ++i) {
     hipStream_t si = streams[i % streams.size()];
     if (i < streams.size()) HIPCHECK(hipStreamWaitEvent(si, ev[0], 0));
-    launch_class(ls[i].cls, uint32_t(ls[i].count), si, jobs, tiles + ls[i].first);
+    launch_class(ls[i].cls, uint32_t(ls[i].count), si, jobs, tiles + ls[i].first, ls[i].grid);
   }
   for (size_t i = 0; i < std::min(ls.size(), streams.size()); ++i) {
     HIPCHECK(hipEventRecord(ev[1 + i], streams[i]));

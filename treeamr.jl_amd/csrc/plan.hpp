@@ -43,7 +43,6 @@ constexpr int kClsMerged = 1000;  // + index into kInterpTable: every rbits in o
 // (P, SHIFT) of each specialised interpolation kernel family.
 extern const std::array<std::array<int, 2>, 8> kInterpTable;
 int interp_index(int P, int shift);
-void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles);
 int class_tile_elems(int cls);
 
 struct PlanGroup {
@@ -71,6 +70,7 @@ struct Launch {
   int cls;
   int64_t first, count;
   int64_t values = 0;  // values the launch writes
+  int32_t grid = 0;    // copies: workgroups of a bounded grid (0: one per tile)
 };
 
 struct AdjointTables;
@@ -112,6 +112,9 @@ struct tah_plan {
   int32_t defer = 1;
   // Values per thread of the copy kernel's tiles: 4, 8 (default) or 16.
   int32_t copy_per_thread = 8;
+  // Workgroups of the copies' bounded grid (a multiple of 8), or 0 (the
+  // default) for one workgroup per tile.
+  int32_t copy_wgs = 0;
   std::vector<hipEvent_t> events;  // fork/join events, created on first use
   // The adjoint's inverted schedule, built on first use (adjoint.hip).
   tah::AdjointTables* adj = nullptr;
@@ -134,4 +137,6 @@ const std::vector<hipStream_t>& ctx_streams(tah_ctx* c);
 // joined back into `s`. ev must hold launches.size() + 1 events.
 void run_launches(tah_ctx* c, const std::vector<Launch>& ls, hipStream_t s, const Job* jobs,
                   const Tile* tiles, hipEvent_t* ev, bool fork);
+// Launch one class over ntiles tiles (fill.hip); grid > 0: a bounded grid.
+void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles, int32_t grid = 0);
 }  // namespace tah

@@ -1,10 +1,11 @@
 // Plan options and the autotuner.
 //
-// A plan's launches have four knobs: stage groups forked over the
+// A plan's launches have five knobs: stage groups forked over the
 // context's streams or run in order on one; the unit classes of one
 // interpolation order in one launch or a launch each; phase-1 transfers
 // deferred into the last stage group or not; the copy kernel's tile (4, 8 or
-// 16 values per thread). tah_tune times every
+// 16 values per thread); the copies from one workgroup per tile or from a
+// bounded grid that leaves wave slots to the interpolations beside them. tah_tune times every
 // combination on the caller's working array and keeps the fastest, cached
 // in a text file under a signature of the mesh, the layout and the device,
 // so that a second run over the same mesh skips the timing. The reference
@@ -77,11 +78,15 @@ void set_options(tah_plan* p, const tah_plan_options& o) {
   if (o.fuse != 0)
     fail("the fused copy-and-interpolation launch is switched off in this build (fuse must be 0)",
          kUnsupported);
-  const bool rebuild = o.merge != p->merge || o.defer != p->defer || o.copy_per_thread != p->copy_per_thread;
+  const int32_t wgs = o.copy_wgs;
+  if (wgs < 0 || wgs % 8 != 0) fail("copy_wgs is 0 or a positive multiple of 8");
+  const bool rebuild = o.merge != p->merge || o.defer != p->defer || o.copy_per_thread != p->copy_per_thread ||
+                       wgs != p->copy_wgs;
   p->fork = o.fork;
   p->merge = o.merge;
   p->defer = o.defer;
   p->copy_per_thread = o.copy_per_thread;
+  p->copy_wgs = wgs;
   if (rebuild) build_launches(p);
 }
 
@@ -93,13 +98,28 @@ bool lookup(const std::string& path, const std::string& key, tah_plan_options& o
   while (std::getline(in, line)) {
     std::istringstream ss(line);
     std::string tag, k;
-    int f, m, d, c, u;
-    if (!(ss >> tag >> k >> f >> m >> d >> c >> u) || tag != "tah-tune-1" || k != key) continue;
-    (void)u;  // the fused launch is switched off: never taken from a cache
-    o = tah_plan_options{f, m, d, c, 0, {0, 0, 0}};
+    int f, m, d, c, w;
+    if (!(ss >> tag >> k >> f >> m >> d >> c >> w) || tag != "tah-tune-2" || k != key) continue;
+    o = tah_plan_options{f, m, d, c, 0, w, {0, 0}};
     found = true;  // the last entry wins
   }
   return found;
+}
+
+// The candidates: every combination, the bounded copy grid only beside
+// forked interpolations (it has nothing to leave its wave slots to
+// otherwise).
+std::vector<tah_plan_options> candidates() {
+  std::vector<tah_plan_options> out;
+  for (int f = 0; f <= 1; ++f)
+    for (int m = 0; m <= 1; ++m)
+      for (int d = 0; d <= 1; ++d)
+        for (int c : {4, 8, 16})
+          for (int w : {0, 512, 1024, 2048}) {
+            if (w && !f) continue;
+            out.push_back(tah_plan_options{f, m, d, c, 0, w, {0, 0}});
+          }
+  return out;
 }
 
 }  // namespace
@@ -114,7 +134,8 @@ tah_status tah_plan_get_options(const tah_plan* p, tah_plan_options* out) {
     out->defer = p->defer;
     out->copy_per_thread = p->copy_per_thread;
     out->fuse = 0;
-    out->reserved_[0] = out->reserved_[1] = out->reserved_[2] = 0;
+    out->copy_wgs = p->copy_wgs;
+    out->reserved_[0] = out->reserved_[1] = 0;
   });
 }
 
@@ -145,36 +166,32 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
-    tah_plan_options best{p->fork, p->merge, p->defer, p->copy_per_thread, 0, {0, 0, 0}};
+    tah_plan_options best{p->fork, p->merge, p->defer, p->copy_per_thread, 0, p->copy_wgs, {0, 0}};
     float best_t = 1e30f;
     auto run = [&]() {
       tah_status st = tah_fill(p, work, factors, s);
       if (st != TAH_OK) fail(std::string("tah_tune: a trial fill failed: ") + tah_last_error(), st);
     };
     try {
-      for (int f = 0; f <= 1; ++f)
-        for (int m = 0; m <= 1; ++m)
-          for (int u = 0; u <= 0; ++u)
-            for (int d = 0; d <= 1; ++d)
-              for (int c : {4, 8, 16}) {
-            set_options(p, tah_plan_options{f, m, d, c, u, {0, 0, 0}});
-            for (int w = 0; w < 2; ++w) run();
-            std::vector<float> ts;
-            for (int r = 0; r < 5; ++r) {
-              HIPCHECK(hipEventRecord(e0, s));
-              run();
-              HIPCHECK(hipEventRecord(e1, s));
-              HIPCHECK(hipEventSynchronize(e1));
-              float ms = 0.f;
-              HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
-              ts.push_back(ms);
-            }
-            std::sort(ts.begin(), ts.end());
-            if (ts[2] < best_t) {
-              best_t = ts[2];
-              best = tah_plan_options{f, m, d, c, u, {0, 0, 0}};
-            }
-          }
+      for (const tah_plan_options& cand : candidates()) {
+        set_options(p, cand);
+        for (int w = 0; w < 2; ++w) run();
+        std::vector<float> ts;
+        for (int r = 0; r < 5; ++r) {
+          HIPCHECK(hipEventRecord(e0, s));
+          run();
+          HIPCHECK(hipEventRecord(e1, s));
+          HIPCHECK(hipEventSynchronize(e1));
+          float ms = 0.f;
+          HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+          ts.push_back(ms);
+        }
+        std::sort(ts.begin(), ts.end());
+        if (ts[2] < best_t) {
+          best_t = ts[2];
+          best = cand;
+        }
+      }
     } catch (...) {
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
@@ -190,8 +207,8 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
     if (cache_path) {
       std::ofstream out(cache_path, std::ios::app);
       if (!out) fail(std::string("tah_tune: cannot write the cache file ") + cache_path);
-      out << "tah-tune-1 " << key << " " << best.fork << " " << best.merge << " " << best.defer << " "
-          << best.copy_per_thread << " " << best.fuse << " " << best_t << "\n";
+      out << "tah-tune-2 " << key << " " << best.fork << " " << best.merge << " " << best.defer << " "
+          << best.copy_per_thread << " " << best.copy_wgs << " " << best_t << "\n";
     }
   });
 }
