@@ -119,6 +119,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-config2", action="store_true")
     ap.add_argument("--no-reference-meshes", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--config2-repeats", type=int, default=100)
     ap.add_argument("--batched-steps", type=int, default=3)
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
@@ -132,7 +133,7 @@ def main():
 
     import torch
     from treeamr_hip import (FieldSet, FillGraph, GhostSchedule, fill_ghosts, fill_ghosts_adjoint,
-                             fill_ghosts_timed)
+                             fill_ghosts_timed, tune)
     from treeamr_hip import workloads as W
 
     def builder(nvars):
@@ -190,6 +191,18 @@ def main():
     dominant = max(kernels, key=lambda n: kernels[n]["ms"])
     dk = kernels[dominant]
     log(f"[bench] dominant kernel {dominant}: {dk}")
+    tuned = None
+    if not args.no_tune:
+        log("[bench] autotuning the plan's launch options (tah_tune, no cache)")
+        opts, tune_ms = tune(fs, sched, None)
+        tgraph = FillGraph([(fs, sched)])
+        tg = time_fill(tgraph.launch, args.steps, args.warmup, stream)
+        tuned = {"options": opts, "tuner_median_ms": round(tune_ms, 4),
+                 "graph_ms": {k: round(v, 4) for k, v in tg.items()},
+                 "GBs": round(gbs_of(tg["mean_ms"]), 1)}
+        log(f"[bench] tuned: {tuned}")
+        del tgraph
+        sched.plan(fs.layout).set_options(fork=0, merge=1, defer=1, copy_per_thread=8, copy_wgs=0)
     adjoint = None
     if not args.no_adjoint:
         log("[bench] timing the adjoint fill")
@@ -346,6 +359,7 @@ def main():
         "staged_ms": r(staged),
         "plain_ms": r(plain),
         "speedup_vs_plain": round(plain["mean_ms"] / t_ms, 3) if plain else None,
+        "tuned": tuned,
         "adjoint_ms": r(adjoint),
         "adjoint_over_forward": round(adjoint["mean_ms"] / t_ms, 3) if adjoint else None,
         "batched": batched,

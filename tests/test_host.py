@@ -494,6 +494,33 @@ def test_stale_schedule_refused():
         _check_fill(fs, s)
 
 
+def test_check_fill_refusals():
+    """check_fill (src/ghosts.jl:2058-2115): a schedule of another forest, ghost
+    width, centering or parity, a field set of another block count or element
+    type, each refused before any launch."""
+    import torch
+    from treeamr_hip.core import _check_fill
+    f = Forest((2, 2, 2), N=8, periodic=(True,) * 3)
+    fs = FieldSet(f, 2, G=2, device="cpu")
+    ops = Operators(prolongation=4, restriction=4)
+    s = GhostSchedule(fs, ops)
+    _check_fill(fs, s)
+    with pytest.raises(ValueError, match="different forests"):
+        _check_fill(FieldSet(Forest((2, 2, 2), N=8, periodic=(True,) * 3), 2, G=2, device="cpu"), s)
+    with pytest.raises(ValueError, match="G="):
+        _check_fill(FieldSet(f, 2, G=3, device="cpu"), s)
+    with pytest.raises(ValueError, match="centering"):
+        _check_fill(FieldSet(f, 2, G=2, centering="vertex", device="cpu"), s)
+    bad = FieldSet(f, 2, G=2, device="cpu")
+    bad.work = bad.work.to(torch.float32)
+    with pytest.raises(ValueError, match="Float64"):
+        _check_fill(bad, s)
+    bad = FieldSet(f, 2, G=2, device="cpu")
+    bad.layout.nblocks = 7
+    with pytest.raises(ValueError, match="blocks"):
+        _check_fill(bad, s)
+
+
 def test_parity_factor_table():
     par = np.array([[1, 1, 1], [1, 1, 2]], np.int8).ravel()
     out = np.zeros(2 * 27)

@@ -451,6 +451,12 @@ def _check_fill(fs, schedule):
         raise _abi.StaleScheduleError(_abi.TAH_ESTALE, "the schedule is stale: the forest has changed "
                                       "since it was built (refine, coarsen or regrid); rebuild it")
     check(lib().tah_schedule_check(schedule._h))
+    if fs.nblocks != schedule.forest.nleaves:
+        raise ValueError(f"the field set has {fs.nblocks} blocks but the schedule's forest has "
+                         f"{schedule.forest.nleaves}; rebuild both")
+    if str(fs.work.dtype) != "torch.float64":
+        raise ValueError(f"the field set stores {fs.work.dtype} but the fill computes in Float64 "
+                         "(the schedule's weights); store the field set as torch.float64")
     if fs.G != schedule.G:
         raise ValueError(f"the field set has G={fs.G}, the schedule was built for G={schedule.G}")
     if fs.centering != schedule.centering:
