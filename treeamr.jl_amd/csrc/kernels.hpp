@@ -183,7 +183,8 @@ __device__ __forceinline__ void copy_body(const Job& job, const Tile& tile) {
     // Variables innermost and even in number: values 2e and 2e+1 are one
     // point's neighbouring variables, 16 contiguous bytes in source and
     // target alike (offsets even, blocks 256-byte aligned): one 16-byte
-    // load and store for both.
+    // load and store for both. The ghosts are not read again within the
+    // fill: the stores stream past the caches.
     constexpr int CP = CPT / 2;
     typedef double pair_t __attribute__((ext_vector_type(2)));
     int32_t dst2[CP], var2[CP];
@@ -223,12 +224,10 @@ __device__ __forceinline__ void copy_body(const Job& job, const Tile& tile) {
     var[r] = v;
     val[r] = sbase[vo + sb0 + ds0 * i + ds1 * j + ds2 * k];
   }
-  // Ghosts are not read again within the fill: stream them past the caches.
 #pragma unroll
   for (int r = 0; r < CPT; ++r) {
     if (dst[r] < 0) continue;
-    __builtin_nontemporal_store(scaled(__dadd_rn(0.0, val[r]), job.factors, g.factorcol, nvars, var[r]),
-                                tbase + dst[r]);
+    tbase[dst[r]] = scaled(__dadd_rn(0.0, val[r]), job.factors, g.factorcol, nvars, var[r]);
   }
 }
 

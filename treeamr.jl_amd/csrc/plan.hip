@@ -377,8 +377,10 @@ void build_launches(tah_plan* p) {
               nvt = std::min(nvars, kThreads);
               nu = std::max(1, kThreads / nvt);
             } else {
-              nvt = 1;
-              nu = kThreads;
+              // units fastest: as many as the class has per transfer, the
+              // rest of the workgroup over variables
+              nu = std::min(nun, kThreads);
+              nvt = std::max(1, std::min(nvars, kThreads / nu));
             }
             const int32_t nt = int32_t(g.tblocks.size());
             for (int32_t t = 0; t < nt;) {
