@@ -52,12 +52,22 @@ def test_dot_product_identity(which, layout):
     assert abs(lhs - rhs) <= 1e-13 * max(abs(lhs), 1.0), (lhs, rhs)
 
 
-@pytest.mark.parametrize("which", ["config1", "reflecting", "vertex"])
-def test_adjoint_matches_cpu_transpose(which):
-    fs, s = _setup(which)
+@pytest.mark.parametrize("which,layout,offset", [("config1", "dense", 0), ("reflecting", "dense", 0),
+                                                 ("vertex", "dense", 0), ("config1", "arena", 0),
+                                                 ("config1", "arena", 1), ("vertex", "arena", 0),
+                                                 ("reflecting", "arena", 0)])
+def test_adjoint_matches_cpu_transpose(which, layout, offset):
+    """Offset 1 starts the array 8 bytes past a 16-byte boundary: the
+    working-array accesses then move single values, not pairs."""
+    fs, s = _setup(which, layout)
     g = torch.Generator(device="cuda")
     g.manual_seed(2)
     y = torch.rand(fs.work.shape, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    if offset:
+        buf = torch.zeros(fs.nelements + 1, dtype=torch.float64, device="cuda")
+        buf[1:].copy_(y)
+        y = buf[1:]
+        assert y.data_ptr() % 16 == 8
     host = y.cpu().numpy().copy()
     fac = fs.factors.cpu().numpy().copy() if fs.factors is not None else None
     oracle.fill_adjoint(s._descptr, fs.layout, host, fac)

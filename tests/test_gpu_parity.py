@@ -66,6 +66,22 @@ def test_config5_small_bit_exact(method, layout):
     _check(fs, sched, method)
 
 
+@pytest.mark.parametrize("method", ["plain", "staged"])
+@pytest.mark.parametrize("offset", [0, 1])
+def test_arena_even_variables_bit_exact(method, offset):
+    """Arena layout with an even variable count, where the kernels move two
+    variables per 16-byte access; with offset 1 the array starts 8 bytes past
+    a 16-byte boundary, so they fall back to single values."""
+    f, ops, kw, _ = W.config5(nvars=4, roots=2, N=16)
+    fs, sched = _prepare(f, ops, kw, layout="arena", seed=3, data="gaussian")
+    buf = torch.empty(fs.nelements + 1, dtype=torch.float64, device="cuda")
+    view = buf[offset:offset + fs.nelements]
+    view.copy_(fs.work)
+    fs.work = view
+    assert fs.work.data_ptr() % 16 == 8 * offset
+    _check(fs, sched, method)
+
+
 @pytest.mark.parametrize("seed", [2, 3, 4, 5, 6, 12])
 def test_random_forests_bit_exact(seed):
     """Forests of the batched configuration, cell (seeds 2, 4, 5) and vertex

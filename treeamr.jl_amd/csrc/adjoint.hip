@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(kThreads) adjoint_gather_kernel(const Job* __r
   const int32_t s0 = int32_t(job.stride[0]), s1 = int32_t(job.stride[1]),
                 s2 = int32_t(job.stride[2]), s3 = int32_t(job.stride[3]);
   const int32_t nvars = job.nvars;
-  if (job.vfast && s3 == 1 && (nvars & 1) == 0) {
+  if (job.pairs) {
     gather_paired(job, c, tile, entries, staged);
     return;
   }

@@ -75,6 +75,10 @@ struct alignas(16) Job {
   int32_t nvars;
   int32_t vfast;  // stride[3] < stride[0]: variables innermost
   FastDiv dnv;    // division by nvars
+  // Variables innermost (stride[3] == 1), nvars and the other strides even
+  // and the array 16-byte aligned: values 2e, 2e+1 of any point are one
+  // aligned 16-byte pair.
+  int32_t pairs;
 };
 
 // A slice of work. Copy and generic tiles: group `item`, transfer, flat

@@ -179,11 +179,10 @@ __device__ __forceinline__ void copy_body(const Job& job, const Tile& tile) {
   const int32_t tb0 = s0 * g.tfirst[0] + s1 * g.tfirst[1] + s2 * g.tfirst[2];
   const int32_t sb0 = s0 * g.ca[0] + s1 * g.ca[1] + s2 * g.ca[2];
   const int32_t ds0 = s0 * g.cs[0], ds1 = s1 * g.cs[1], ds2 = s2 * g.cs[2];
-  if (job.vfast && s3 == 1 && (nvars & 1) == 0) {
-    // Variables innermost and even in number: values 2e and 2e+1 are one
-    // point's neighbouring variables, 16 contiguous bytes in source and
-    // target alike (offsets even, blocks 256-byte aligned): one 16-byte
-    // load and store for both. The ghosts are not read again within the
+  if (job.pairs) {
+    // Values 2e and 2e+1 are one point's neighbouring variables, an aligned
+    // 16-byte pair in source and target alike: one 16-byte load and store
+    // for both. The ghosts are not read again within the
     // fill: the stores stream past the caches.
     constexpr int CP = CPT / 2;
     typedef double pair_t __attribute__((ext_vector_type(2)));

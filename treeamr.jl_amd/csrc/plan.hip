@@ -451,6 +451,11 @@ Job plan_job(const tah_plan* p, double* work, const double* factors) {
   job.nvars = p->layout.nvars;
   job.vfast = p->layout.stride[3] < p->layout.stride[0] ? 1 : 0;
   job.dnv = FastDiv(uint32_t(p->layout.nvars));
+  const int64_t* st = p->layout.stride;
+  job.pairs = st[3] == 1 && p->layout.nvars % 2 == 0 && st[0] % 2 == 0 && st[1] % 2 == 0 && st[2] % 2 == 0 &&
+                      st[4] % 2 == 0 && reinterpret_cast<uintptr_t>(work) % 16 == 0
+                  ? 1
+                  : 0;
   return job;
 }
 
