@@ -51,7 +51,7 @@ struct alignas(16) DevGroupT {
 // v, at out_off + ((i2*o1 + i1)*o0 + i0)*nvars + v, sums weight * input at
 // in_off + isv*v + is[d]*i[d] (d not contracted) + is[D]*r over the pairs
 // (r, tap) of the contracted dimension D's row i[D]. A pass tile takes rows
-// row0 .. row0+kAdjPerThread-1 of D and plane values pe0 .. pe0+kThreads-1,
+// row0 .. row0+kPassRows-1 of D and plane values pe0 .. pe0+kPassTile-1,
 // a plane value being (v, then the lower, then the upper other dimension).
 struct alignas(16) AdjPass {
   int64_t in_off, out_off;
@@ -98,6 +98,8 @@ struct alignas(8) AdjTile {
 
 constexpr int kAdjPerThread = 4;
 constexpr int kAdjTile = kThreads * kAdjPerThread;
+constexpr int kPassRows = 8;              // output rows of one adjoint pass tile
+constexpr int kPassTile = 2 * kThreads;   // and its plane values
 
 struct AdjStage {
   int64_t pass_tiles[3][2];  // per pass (z, y, x): first tile, count
@@ -129,8 +131,104 @@ void free_adjoint(tah_plan* p) {
 }
 
 // Pass D (2: z, 1: y, 0: x) of every interpolating transfer of a stage.
-// The contracted row is one for the whole workgroup, so its inverted
-// stencil is read through scalar loads; lanes run over the plane.
+// A tile takes kPassRows output rows of D for kPassTile plane values, two
+// per thread: one aligned 16-byte pair (variables v, v+1 of one point)
+// where input and output hold variables innermost in even number, else two
+// values kThreads apart. Scatter form: the tile walks the input rows that
+// reach any of its output rows once each, in ascending order, loads the
+// thread's values of each once, and adds weight * value into every output
+// row of the tile whose stencil holds it (registers). Each output thus sums
+// its (input row, tap) pairs in ascending input row, the order of its
+// inverted stencil, with one load per input row rather than one per pair.
+// The rows, their stencil starts and weights are one for the whole
+// workgroup: scalar loads.
+template <int D, bool PAIR>
+__device__ __forceinline__ void pass_tile(const AdjPass& it, const Job& job, const DevGroup& g, const DevGroupT& T,
+                                          const double* __restrict__ in, double* __restrict__ o, uint32_t pe0,
+                                          int32_t x0) {
+  typedef double pair_t __attribute__((ext_vector_type(2)));
+  constexpr int DL = D == 0 ? 1 : 0;  // the lower other dimension
+  constexpr int DU = D == 2 ? 1 : 2;  // the upper one
+  const int32_t nvars = job.nvars;
+  const uint32_t plane = uint32_t(it.plane);
+  const uint32_t pe[2] = {PAIR ? pe0 + 2u * threadIdx.x : pe0 + threadIdx.x,
+                          PAIR ? pe0 + 2u * threadIdx.x + 1u : pe0 + kThreads + threadIdx.x};
+  if (pe[0] >= plane) return;
+  const bool ok1 = pe[1] < plane;
+  int32_t b[2], ob[2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const uint32_t e = s == 1 && !ok1 ? pe[0] : pe[s];
+    const uint32_t r = job.dnv.div(e);
+    const int32_t v = int32_t(e - r * uint32_t(nvars));
+    const uint32_t q = it.flo.div(r);
+    int32_t i[3];
+    i[DL] = int32_t(r - q * uint32_t(it.o[DL]));
+    i[DU] = int32_t(q);
+    b[s] = it.isv * v + it.is[DL] * i[DL] + it.is[DU] * i[DU];
+    ob[s] = (((D == 2 ? 0 : i[2]) * it.o[1] + (D == 1 ? 0 : i[1])) * it.o[0] + (D == 0 ? 0 : i[0])) * nvars + v;
+  }
+  const int32_t ostep = (D == 0 ? 1 : D == 1 ? it.o[0] : it.o[0] * it.o[1]) * nvars;
+  const auto start = constant(T.start[D]);
+  const auto list = constant(T.list[D]);
+  const auto ss = constant(g.srcstart[D]);
+  const auto w = constant(g.weights[D]);
+  const int32_t order = uniform(g.order[D]);
+  const int32_t isd = it.is[D];
+  const int32_t nr = min(kPassRows, uniform(it.o[D]) - x0);
+  // The input rows that reach the tile: each output row's pairs run in
+  // ascending input row, so its first and last pair bound them.
+  int32_t rlo = INT32_MAX, rhi = -1;
+  for (int k = 0; k < nr; ++k) {
+    const int32_t j0 = start[x0 + k], j1 = start[x0 + k + 1];
+    if (j0 < j1) {
+      rlo = min(rlo, list[j0] >> 4);
+      rhi = max(rhi, list[j1 - 1] >> 4);
+    }
+  }
+  double a0[kPassRows], a1[kPassRows];
+#pragma unroll
+  for (int k = 0; k < kPassRows; ++k) a0[k] = a1[k] = 0.0;
+  // Input row rr reaches output row x0 + k with tap m = mx0 + k - ss[rr].
+  const int32_t mx0 = x0 + uniform(T.lo[D]);
+#pragma unroll 2
+  for (int32_t rr = rlo; rr <= rhi; ++rr) {
+    double u0, u1;
+    if constexpr (PAIR) {
+      const pair_t uu = global(reinterpret_cast<const pair_t*>(in))[(b[0] + isd * rr) >> 1];
+      u0 = uu.x;
+      u1 = uu.y;
+    } else {
+      u0 = global(in)[b[0] + isd * rr];
+      u1 = global(in)[b[1] + isd * rr];
+    }
+    const int32_t m0 = mx0 - ss[rr];
+    const int32_t wr = order * rr;
+#pragma unroll
+    for (int k = 0; k < kPassRows; ++k) {
+      const int32_t m = m0 + k;
+      if (k < nr && m >= 0 && m < order) {
+        const double ww = w[wr + m];
+        a0[k] = __dadd_rn(a0[k], __dmul_rn(ww, u0));
+        a1[k] = __dadd_rn(a1[k], __dmul_rn(ww, u1));
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kPassRows; ++k) {
+    if (k >= nr) break;
+    if constexpr (PAIR) {
+      pair_t res;
+      res.x = a0[k];
+      res.y = a1[k];
+      reinterpret_cast<pair_t*>(o)[(ob[0] + ostep * (x0 + k)) >> 1] = res;
+    } else {
+      o[ob[0] + ostep * (x0 + k)] = a0[k];
+      if (ok1) o[ob[1] + ostep * (x0 + k)] = a1[k];
+    }
+  }
+}
+
 template <int D>
 __global__ void __launch_bounds__(kThreads) adjoint_pass_kernel(const Job* __restrict__ jobs,
                                                                 const DevGroupT* __restrict__ gts,
@@ -139,8 +237,6 @@ __global__ void __launch_bounds__(kThreads) adjoint_pass_kernel(const Job* __res
                                                                 uint32_t ntiles,
                                                                 const double* __restrict__ scratch_in,
                                                                 double* __restrict__ out) {
-  constexpr int DL = D == 0 ? 1 : 0;  // the lower other dimension
-  constexpr int DU = D == 2 ? 1 : 2;  // the upper one
   const AdjPassTile tile = tiles[xcd_tile(blockIdx.x, ntiles)];
   const AdjPass& it = items[uniform(tile.item)];
   const Job& job = jobs[0];
@@ -149,37 +245,12 @@ __global__ void __launch_bounds__(kThreads) adjoint_pass_kernel(const Job* __res
   const DevGroupT& T = gts[gi];
   const double* __restrict__ in = (D == 2 ? job.work : scratch_in) + it.in_off;
   double* __restrict__ o = out + it.out_off;
-  const int32_t nvars = job.nvars;
-  const uint32_t pe = uint32_t(tile.pe0) + threadIdx.x;
-  if (pe >= uint32_t(it.plane)) return;
-  const uint32_t r = job.dnv.div(pe);
-  const int32_t v = int32_t(pe - r * uint32_t(nvars));
-  const uint32_t q = it.flo.div(r);
-  int32_t i[3];
-  i[DL] = int32_t(r - q * uint32_t(it.o[DL]));
-  i[DU] = int32_t(q);
-  const int32_t b = it.isv * v + it.is[DL] * i[DL] + it.is[DU] * i[DU];
-  const int32_t ob = (((D == 2 ? 0 : i[2]) * it.o[1] + (D == 1 ? 0 : i[1])) * it.o[0] + (D == 0 ? 0 : i[0])) * nvars + v;
-  const int32_t ostep = (D == 0 ? 1 : D == 1 ? it.o[0] : it.o[0] * it.o[1]) * nvars;
-  const auto start = constant(T.start[D]);
-  const auto list = constant(T.list[D]);
-  const auto w = constant(g.weights[D]);
-  const auto src = global(in);
-  const int32_t order = uniform(g.order[D]);
-  const int32_t isd = it.is[D];
-  const int32_t row0 = uniform(tile.row0);
-  const int32_t rows = min(kAdjPerThread, uniform(it.o[D]) - row0);
-  for (int k = 0; k < rows; ++k) {
-    const int32_t x = row0 + k;
-    double acc = 0.0;
-    const int32_t j1 = start[x + 1];
-#pragma unroll 4
-    for (int32_t j = start[x]; j < j1; ++j) {
-      const int32_t pk = list[j], rr = pk >> 4, m = pk & 15;
-      acc = __dadd_rn(acc, __dmul_rn(w[order * rr + m], src[b + isd * rr]));
-    }
-    o[ob + ostep * x] = acc;
-  }
+  // The staged passes hold variables innermost; the first reads the
+  // working array, paired where its layout allows.
+  if ((job.nvars & 1) == 0 && (D < 2 || job.pairs))
+    pass_tile<D, true>(it, job, g, T, in, o, uint32_t(tile.pe0), uniform(tile.row0));
+  else
+    pass_tile<D, false>(it, job, g, T, in, o, uint32_t(tile.pe0), uniform(tile.row0));
 }
 
 // The gather with variables innermost and even in number: values 2e and
@@ -551,8 +622,8 @@ void build_adjoint(tah_plan* p) {
         P.flo = FastDiv(uint32_t(std::max(1, P.o[DL])));
         passes.push_back(P);
         const int32_t item = int32_t(passes.size() - 1);
-        for (int32_t r0 = 0; r0 < P.o[D]; r0 += kAdjPerThread)
-          for (int64_t e = 0; e < plane; e += kThreads) ptiles.push_back(AdjPassTile{item, r0, int32_t(e), 0});
+        for (int32_t r0 = 0; r0 < P.o[D]; r0 += kPassRows)
+          for (int64_t e = 0; e < plane; e += kPassTile) ptiles.push_back(AdjPassTile{item, r0, int32_t(e), 0});
       }
       S.pass_tiles[q][1] = int64_t(ptiles.size()) - S.pass_tiles[q][0];
     }
