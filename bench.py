@@ -120,6 +120,7 @@ def main():
     ap.add_argument("--no-config2", action="store_true")
     ap.add_argument("--no-reference-meshes", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-state", action="store_true")
     ap.add_argument("--config2-repeats", type=int, default=100)
     ap.add_argument("--batched-steps", type=int, default=3)
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
@@ -133,7 +134,7 @@ def main():
 
     import torch
     from treeamr_hip import (FieldSet, FillGraph, GhostSchedule, fill_ghosts, fill_ghosts_adjoint,
-                             fill_ghosts_timed, tune)
+                             fill_ghosts_timed, gather, ownextents, scatter, statelength, tune)
     from treeamr_hip import workloads as W
 
     def builder(nvars):
@@ -324,6 +325,42 @@ def main():
             log(f"[bench] {rec}")
             refmeshes.append(rec)
 
+    state = None
+    if not args.no_state:
+        # scatter!/gather! per owned point on the reference's copy meshes
+        # (bench/copies.jl), beside a linear copy of the state, the floor.
+        state = []
+        h200 = {16: (0.182, 0.147, 0.100), 32: (0.140, 0.136, 0.093), 128: (0.116, 0.134, 0.093)}
+        for N, roots in ((16, 8), (32, 8), (128, 2)):
+            fS, _, kwS = W.uniform_vertex(N=N, roots=roots)
+            kwS = dict(kwS)
+            nvS = kwS.pop("nvars")
+            rec = {"workload": f"uniform {roots}^3 roots of {N}^3, vertex, G=3, {nvS} vars (bench/copies.jl)",
+                   "blocks": fS.nleaves,
+                   "h200_ns_per_point": {"scatter": h200[N][0], "gather": h200[N][1], "floor": h200[N][2]},
+                   "h200_source": "CODE.md:2942-2953"}
+            for layout in ("arena", "dense"):
+                fsS = FieldSet(fS, nvS, device="cuda", layout=layout, fill=0.0, **kwS)
+                e = ownextents(fsS)
+                points = fS.nleaves * e[0] * e[1] * e[2]
+                u = torch.rand(statelength(fsS), device="cuda", dtype=torch.float64)
+                tsc = time_fill(lambda: scatter(fsS, u), 20, args.warmup, stream)
+                tga = time_fill(lambda: gather(u, fsS), 20, args.warmup, stream)
+                ns = lambda t: round(t["min_ms"] * 1e6 / points, 4)
+                gb = lambda t: round(16.0 * u.numel() / (t["min_ms"] * 1e-3) / 1e9, 1)
+                rec[layout] = {"scatter_ns_per_point": ns(tsc), "gather_ns_per_point": ns(tga),
+                               "scatter_GBs": gb(tsc), "gather_GBs": gb(tga)}
+                if layout == "arena":
+                    v = torch.empty_like(u)
+                    tfl = time_fill(lambda: v.copy_(u), 20, args.warmup, stream)
+                    rec["floor_ns_per_point"] = ns(tfl)
+                    rec["floor_GBs"] = gb(tfl)
+                    del v
+                del fsS, u
+            rec["points"] = points
+            log(f"[bench] state copies: {rec}")
+            state.append(rec)
+
     cpu = None
     if not args.no_cpu:
         log("[bench] timing the CPU baseline (oracle)")
@@ -365,6 +402,7 @@ def main():
         "batched": batched,
         "config2": nested,
         "reference_meshes": refmeshes,
+        "state_copies": state,
         "schedule_build_s": round(build_s, 3),
         "roofline": {
             "kernel": dominant,

@@ -283,6 +283,25 @@ tah_status tah_plan_set_options(tah_plan* p, const tah_plan_options* o);
 tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char* cache_path,
                     double* best_ms);
 
+/* --- The state vector (src/state.jl) -------------------------------- */
+
+/* A state vector holds every own point of every block, no ghosts, laid out
+ * (x, y, z, variable, block) with x fastest (statearray,
+ * src/state.jl:54-58). The own range of a block is stored points
+ * first[d] .. first[d]+extent[d]-1 per dimension: first = G and extent =
+ * N + c_d (c_d = 1 along a vertex-like dimension) for d < dim, first = 0 and
+ * extent = 1 beyond (ownextents, src/storage.jl:939).
+ * statelength (src/state.jl:25): prod(extent) * nvars * nblocks. */
+tah_status tah_state_length(const tah_layout* layout, const int32_t* extent, int64_t* out);
+/* scatter!(fs, u) (src/state.jl:135-139): state -> the working array's own
+ * range; ghosts untouched. Asynchronous on `stream`. */
+tah_status tah_scatter(const tah_layout* layout, const int32_t* first, const int32_t* extent,
+                       double* work, const double* state, void* stream);
+/* gather!(u, fs) (src/state.jl:150-154): the working array's own range ->
+ * state. Asynchronous on `stream`. */
+tah_status tah_gather(const tah_layout* layout, const int32_t* first, const int32_t* extent,
+                      double* state, const double* work, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,8 @@ Functions:
   fill(desc, layout, work, factors, nthreads=0, skip=True)   run_phase!(::CPU) per stage
   fill_adjoint(desc, layout, adj, factors)                   CPU transpose (scatter form)
   write_counts(desc, layout)                                 writes per stored point
+  scatter(layout, first, extent, work, state)                scatter_kernel! (src/state.jl)
+  gather(layout, first, extent, state, work)                 gather_kernel! (src/state.jl)
 """
 import ctypes as C
 import os
@@ -35,6 +37,9 @@ def lib():
         L.oracle_fill_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_write_counts.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_max_threads.restype = C.c_int
+        for name in ("oracle_scatter", "oracle_gather"):
+            getattr(L, name).restype = C.c_int
+            getattr(L, name).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -72,3 +77,19 @@ def write_counts(desc_ptr, layout):
 
 def max_threads():
     return lib().oracle_max_threads()
+
+
+def _i32(seq):
+    return (C.c_int32 * 3)(*seq)
+
+
+def scatter(layout, first, extent, work, state):
+    """work[own point] = state[n] for every state entry n, in place on host arrays."""
+    lib().oracle_scatter(C.cast(C.pointer(layout), C.c_void_p), _i32(first), _i32(extent), _ptr(work),
+                         _ptr(state))
+
+
+def gather(layout, first, extent, state, work):
+    """state[n] = work[own point] for every state entry n, in place on host arrays."""
+    lib().oracle_gather(C.cast(C.pointer(layout), C.c_void_p), _i32(first), _i32(extent), _ptr(state),
+                        _ptr(work))

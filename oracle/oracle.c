@@ -27,6 +27,12 @@
  * the values read, so that a transfer onto itself -- stage P -- scales). It has
  * no counterpart in the reference (which has no autodiff); it is pinned by
  * the dot-product identity <fill(x), y> = <x, fill^T(y)> in tests/.
+ *
+ * And oracle_scatter / oracle_gather, scatter_kernel! and gather_kernel!
+ * (src/state.jl:101-115) as loops over the state vector's linear index n,
+ * whose position (i1, i2, i3, var, block) in the column-major shape
+ * (extent..., nvars, nblocks) picks the working array's stored point
+ * i_d + first_d (first_d = G_d).
  */
 #include <math.h>
 #include <omp.h>
@@ -239,3 +245,34 @@ int oracle_write_counts(const tah_schedule_desc* d, const tah_layout* L, int32_t
 }
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }
+
+/* The working array's element of state entry n (src/state.jl:101-115). */
+static int64_t state_to_work(const tah_layout* L, const int32_t* first, const int32_t* ext, int64_t n) {
+  int64_t r = n;
+  int64_t i[3];
+  for (int d = 0; d < 3; ++d) {
+    i[d] = r % ext[d];
+    r /= ext[d];
+  }
+  const int64_t v = r % L->nvars, b = r / L->nvars;
+  return L->stride[4] * b + L->stride[3] * v + L->stride[0] * (i[0] + first[0]) +
+         L->stride[1] * (i[1] + first[1]) + L->stride[2] * (i[2] + first[2]);
+}
+
+static int64_t state_length(const tah_layout* L, const int32_t* ext) {
+  return (int64_t)ext[0] * ext[1] * ext[2] * L->nvars * L->nblocks;
+}
+
+int oracle_scatter(const tah_layout* L, const int32_t* first, const int32_t* ext, double* work,
+                   const double* state) {
+  const int64_t n = state_length(L, ext);
+  for (int64_t k = 0; k < n; ++k) work[state_to_work(L, first, ext, k)] = state[k];
+  return 0;
+}
+
+int oracle_gather(const tah_layout* L, const int32_t* first, const int32_t* ext, double* state,
+                  const double* work) {
+  const int64_t n = state_length(L, ext);
+  for (int64_t k = 0; k < n; ++k) state[k] = work[state_to_work(L, first, ext, k)];
+  return 0;
+}

@@ -122,6 +122,18 @@ void validate_layout(const tah_schedule_desc* d, const tah_layout* l) {
     if (l->stored[k] != d->stored[k])
       fail("the layout's stored size differs from the schedule's in dimension " + std::to_string(k + 1),
            kLayout);
+  validate_strides(l);
+}
+
+}  // namespace
+
+namespace tah {
+
+void validate_strides(const tah_layout* l) {
+  if (l->nvars < 1) fail("nvars must be positive", kLayout);
+  if (l->nblocks < 0) fail("the block count must not be negative", kLayout);
+  for (int k = 0; k < 3; ++k)
+    if (l->stored[k] < 1) fail("the layout's stored sizes must be positive", kLayout);
   // Strides positive and the five axes nested without overlap: sorted by
   // stride, each at least the span of the ones inside it.
   int64_t ext[5] = {l->stored[0], l->stored[1], l->stored[2], l->nvars, l->nblocks};
@@ -140,7 +152,7 @@ void validate_layout(const tah_schedule_desc* d, const tah_layout* l) {
     fail("a block larger than 2^31 values is not supported", kUnsupported);
 }
 
-}  // namespace
+}  // namespace tah
 
 extern "C" {
 

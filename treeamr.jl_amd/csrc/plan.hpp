@@ -125,6 +125,9 @@ namespace tah {
 // A unit copy whose source row is affine in the target row along every
 // dimension: ca + cs*r with |cs| <= max_step.
 bool affine_copy(const PlanGroup& g, int32_t ca[3], int32_t cs[3], int max_step);
+// A layout's strides: positive, the five axes nested without overlap,
+// blocks outermost (TAH_ELAYOUT otherwise).
+void validate_strides(const tah_layout* l);
 // Split groups into classes, upload tables, build tiles and launches.
 void build_plan_device(tah_plan* p);
 // (Re)build the tiles and launches for the plan's options.

@@ -17,5 +17,6 @@ from .core import (  # noqa: F401
     Forest, MortonKey, Operators, FieldSet, GhostSchedule, fill_ghosts, fill_ghosts_adjoint,
     PointValue, Conservative, NoParity, EvenParity, OddParity, cellcentered, vertexcentered,
     Context, default_context, Plan, fill_ghosts_batched, FillGraph, fill_ghosts_timed, kernel_class_name, tune,
+    ownextents, statelength, statevector, staterange, stateview, statearray, scatter, gather,
 )
 from ._abi import TahError, StaleScheduleError, LayoutError, UnsupportedError  # noqa: F401

@@ -92,6 +92,9 @@ SIGNATURES = {
     "tah_plan_get_options": [P, C.POINTER(tah_plan_options)],
     "tah_plan_set_options": [P, C.POINTER(tah_plan_options)],
     "tah_tune": [P, P, P, C.c_char_p, C.POINTER(C.c_double)],
+    "tah_state_length": [C.POINTER(tah_layout), I32P, C.POINTER(C.c_int64)],
+    "tah_scatter": [C.POINTER(tah_layout), I32P, I32P, P, P, P],
+    "tah_gather": [C.POINTER(tah_layout), I32P, I32P, P, P, P],
 }
 
 _lib = None

@@ -540,6 +540,130 @@ def fill_ghosts_adjoint(fs, schedule, adj, *, ctx=None):
     return adj
 
 
+# ---------------------------------------------------------------------------
+# The state vector (src/state.jl): every own point of every block, no ghosts,
+# laid out (x, y, z, variable, block) with x fastest.
+
+def ownextents(fs):
+    """Own points per dimension (src/storage.jl:939): N + c_d, c_d = 1 along a
+    vertex-like dimension; 1 beyond the forest's dimension."""
+    D = fs.forest.D
+    return tuple(fs.forest.N + fs.centering[d] for d in range(D)) + (1,) * (3 - D)
+
+
+def _own_first(fs):
+    D = fs.forest.D
+    return tuple(fs.G[d] for d in range(D)) + (0,) * (3 - D)
+
+
+def _check_sets(sets):
+    """check_sets (src/state.jl:283-310)."""
+    if len(sets) == 0:
+        raise ValueError("a state vector over several field sets needs at least one set; for one set, "
+                         "the single-set functions take the field set itself")
+    fs = sets[0]
+    for i, s in enumerate(sets, 1):
+        if s.forest is not fs.forest:
+            raise ValueError(f"field set {i} is over a different forest from field set 1: the sets of one "
+                             "state vector are regridded together, so they must share the forest")
+        if s.work.device != fs.work.device:
+            raise ValueError(f"field set {i} is on {s.work.device} and field set 1 on {fs.work.device}: the "
+                             "state vector is one array, on one device")
+        if s.work.dtype != fs.work.dtype:
+            raise ValueError(f"field set {i} has element type {s.work.dtype} and field set 1 has "
+                             f"{fs.work.dtype}: the state vector is one array, of one element type")
+        if s.nblocks != s.forest.nleaves:
+            raise ValueError(f"field set {i} has {s.nblocks} blocks but the forest has {s.forest.nleaves}: "
+                             "the set was not resized with the forest's last regrid")
+
+
+def statelength(fs):
+    """statelength (src/state.jl:25, 171-174): prod(ownextents) * nvars * nblocks,
+    summed over the sets of a tuple."""
+    if isinstance(fs, tuple):
+        _check_sets(fs)
+        return sum(statelength(s) for s in fs)
+    e = ownextents(fs)
+    return e[0] * e[1] * e[2] * fs.nvars * fs.nblocks
+
+
+def _check_statelength(u, fs):
+    n = statelength(fs)
+    if u.dim() != 1 or u.numel() != n:
+        what = "this field set needs" if not isinstance(fs, tuple) else f"these {len(fs)} field sets need"
+        raise ValueError(f"state vector has {u.numel()} entries but {what} {n}")
+
+
+def statevector(fs):
+    """A zeroed state vector on the field set's device (src/state.jl:37-42, 214-224)."""
+    import torch
+    first = fs[0] if isinstance(fs, tuple) else fs
+    return torch.zeros(statelength(fs), dtype=first.work.dtype, device=first.work.device)
+
+
+def staterange(sets, i):
+    """The entries of the i-th set's segment (0-based here): src/state.jl:181-189."""
+    _check_sets(sets)
+    if not 0 <= i < len(sets):
+        raise ValueError(f"set index {i} is outside 0:{len(sets) - 1}: the state vector has one segment "
+                         "per field set in the tuple")
+    start = sum(statelength(sets[j]) for j in range(i))
+    return range(start, start + statelength(sets[i]))
+
+
+def stateview(u, sets, i):
+    """The i-th set's segment of u, sharing memory (src/state.jl:233-236)."""
+    _check_statelength(u, sets)
+    r = staterange(sets, i)
+    return u[r.start:r.stop]
+
+
+def statearray(u, fs, i=None):
+    """u viewed as (nblocks, nvars, e2, e1, e0) -- the reference's (e0, e1, e2,
+    nvars, nblocks) column-major array (src/state.jl:54-58, 244-248)."""
+    if isinstance(fs, tuple):
+        return statearray(stateview(u, fs, i), fs[i])
+    _check_statelength(u, fs)
+    e = ownextents(fs)
+    return u.view(fs.nblocks, fs.nvars, e[2], e[1], e[0])
+
+
+def _state_copy(fn, fs, u):
+    _check_statelength(u, fs)
+    if str(u.dtype) != "torch.float64" or str(fs.work.dtype) != "torch.float64":
+        raise ValueError("the state vector and the field set hold Float64")
+    if u.device != fs.work.device or fs.work.device.type != "cuda":
+        raise ValueError("the state vector and the field set must be on the same GPU")
+    if not u.is_contiguous():
+        raise ValueError("the state vector must be contiguous")
+    check(fn(C.byref(fs.layout), _i32(_own_first(fs)), _i32(ownextents(fs)), C.c_void_p(fs.work.data_ptr()),
+             C.c_void_p(u.data_ptr()), _stream_ptr(fs)))
+
+
+def scatter(fs, u):
+    """scatter! (src/state.jl:135-139, 257-263): u -> the working array's own
+    points, ghosts untouched; on the field set's current stream."""
+    if isinstance(fs, tuple):
+        _check_statelength(u, fs)
+        for i, s in enumerate(fs):
+            scatter(s, stateview(u, fs, i))
+        return fs
+    _state_copy(lib().tah_scatter, fs, u)
+    return fs
+
+
+def gather(u, fs):
+    """gather! (src/state.jl:150-154, 270-276): the working array's own points
+    -> u; on the field set's current stream."""
+    if isinstance(fs, tuple):
+        _check_statelength(u, fs)
+        for i, s in enumerate(fs):
+            gather(stateview(u, fs, i), s)
+        return u
+    _state_copy(lambda L, f, e, w, st, sp: lib().tah_gather(L, f, e, st, w, sp), fs, u)
+    return u
+
+
 def _batch_args(pairs, ctx):
     ctx = ctx or default_context()
     n = len(pairs)
