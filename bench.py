@@ -121,6 +121,7 @@ def main():
     ap.add_argument("--no-reference-meshes", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-state", action="store_true")
+    ap.add_argument("--no-regrid", action="store_true")
     ap.add_argument("--config2-repeats", type=int, default=100)
     ap.add_argument("--batched-steps", type=int, default=3)
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
@@ -132,9 +133,12 @@ def main():
               f"(asked for --gpus {args.gpus}); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    import ctypes
     import torch
+    from treeamr_hip import _abi
     from treeamr_hip import (FieldSet, FillGraph, GhostSchedule, fill_ghosts, fill_ghosts_adjoint,
-                             fill_ghosts_timed, gather, ownextents, scatter, statelength, tune)
+                             fill_ghosts_timed, gather, ownextents, regrid_transfer, scatter, statelength,
+                             tune, RegridSchedule)
     from treeamr_hip import workloads as W
 
     def builder(nvars):
@@ -361,6 +365,46 @@ def main():
             log(f"[bench] state copies: {rec}")
             state.append(rec)
 
+    regrid_rec = None
+    if not args.no_regrid:
+        # The reference's regrid-transfer benchmark (bench/gpu.jl:159-174):
+        # old leaves = new leaves (every block copied into a fresh array),
+        # on its two-level mesh; scatter! on the same mesh beside it. H200
+        # times from CODE.md:5538-5546.
+        fT, opsT, kwT = W.two_level(N=32)
+        kwT = dict(kwT)
+        nvT = kwT.pop("nvars")
+        regrid_rec = {"workload": "two-level, 960 blocks of 32^3, cell, G=2, 2 vars (bench/gpu.jl), "
+                                  "old leaves = new leaves",
+                      "blocks": fT.nleaves, "h200_ms": {"regrid_transfer": 1.3, "scatter": 1.0},
+                      "h200_source": "CODE.md:5538-5546"}
+        for layout in ("arena", "dense"):
+            fsT = FieldSet(fT, nvT, device="cuda", layout=layout, fill=0.0, **kwT)
+            W.fill_uniform(fsT, 6)
+            rsT = RegridSchedule(fT, fT.leaf_arrays(), opsT, G=fsT.G, centering=fsT.centering, nvars=nvT)
+            fresh, planT = regrid_transfer(fsT, fsT.layout, fsT.work, rsT)
+            lay = fsT.layout
+
+            def move():
+                _abi.check(_abi.lib().tah_regrid_transfer(planT._h, ctypes.byref(lay),
+                                                          ctypes.c_void_p(fsT.work.data_ptr()),
+                                                          ctypes.c_void_p(fresh.data_ptr()),
+                                                          ctypes.c_void_p(stream.cuda_stream)))
+            tT = time_fill(move, 20, args.warmup, stream)
+            vals = statelength(fsT)
+            u = torch.rand(vals, device="cuda", dtype=torch.float64)
+            tS = time_fill(lambda: scatter(fsT, u), 20, args.warmup, stream)
+            regrid_rec[layout] = {
+                "regrid_transfer_ms": round(tT["min_ms"], 4),
+                "regrid_transfer_GBs": round(16.0 * vals / (tT["min_ms"] * 1e-3) / 1e9, 1),
+                "scatter_ms": round(tS["min_ms"], 4),
+                "scatter_GBs": round(16.0 * vals / (tS["min_ms"] * 1e-3) / 1e9, 1)}
+            del fsT, rsT, fresh, planT, u
+        regrid_rec["speedup_vs_h200"] = {
+            "regrid_transfer": round(1.3 / min(regrid_rec[l]["regrid_transfer_ms"] for l in ("arena", "dense")), 2),
+            "scatter": round(1.0 / min(regrid_rec[l]["scatter_ms"] for l in ("arena", "dense")), 2)}
+        log(f"[bench] regrid transfer: {regrid_rec}")
+
     cpu = None
     if not args.no_cpu:
         log("[bench] timing the CPU baseline (oracle)")
@@ -403,6 +447,7 @@ def main():
         "config2": nested,
         "reference_meshes": refmeshes,
         "state_copies": state,
+        "regrid_transfer": regrid_rec,
         "schedule_build_s": round(build_s, 3),
         "roofline": {
             "kernel": dominant,

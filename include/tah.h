@@ -74,6 +74,11 @@ tah_status tah_forest_refine(tah_forest* f, int32_t n, const int32_t* leaves);
 /* balance! (src/forest.jl:1837) and isbalanced (src/forest.jl:1892). */
 tah_status tah_forest_balance(tah_forest* f);
 tah_status tah_forest_isbalanced(const tah_forest* f, int32_t* balanced);
+/* Coarsening, as regrid!'s coarsening flags (src/regrid.jl): every family
+ * whose 2^dim children are all among the given leaves (0-based indices)
+ * becomes its parent, unless the parent would meet a leaf two levels finer
+ * (2:1 balance holds after). coarsened (may be NULL): families merged. */
+tah_status tah_forest_coarsen(tah_forest* f, int32_t n, const int32_t* leaves, int32_t* coarsened);
 
 enum { TAH_POINTVALUE = 0, TAH_CONSERVATIVE = 1 };
 /* Operators(; prolongation, restriction, family, bounded,
@@ -114,6 +119,11 @@ enum { TAH_COPY = 1, TAH_RESTRICT = 2, TAH_PROLONG = 3 };
  * prolongations onto level l (2 + l) (transfer_stage,
  * src/schedule.jl:640-658). */
 enum { TAH_STAGE_P = -1, TAH_STAGE_UNIFY = 0, TAH_STAGE_PHASE1 = 1 };
+/* The regrid transfer's two stages (REGRID_TAG and REGRID_KEPT_TAG,
+ * src/regrid.jl:755-803, 905-933): old array -> new array, then, for a set
+ * with a vertex-like dimension, the kept blocks' copies of shared points
+ * over the created blocks' within the new array. */
+enum { TAH_STAGE_REGRID = 100, TAH_STAGE_REGRID_KEPT = 101 };
 
 /* One dimension of a transfer (Stencil1D, src/schedule.jl:64-93):
  * target rows targetfirst .. targetfirst+ntarget-1 (0-based); row i reads
@@ -154,7 +164,28 @@ typedef struct tah_schedule_desc {
   int32_t ngroups;
   const tah_group_desc* groups; /* sorted by stage */
   int64_t generation;           /* forest generation it was built for */
+  int32_t nsources;             /* regrid: the old leaves the sources index; 0: nblocks */
+  int32_t pad_;
 } tah_schedule_desc;
+
+/* The regrid transfer of one field-set layout onto the forest's current
+ * leaves from its leaves before the regrid (old_root, old_level,
+ * old_coords[3] per old leaf, sorted along the curve, as tah_forest_leaves
+ * gave them): transfer_groups / regrid_stage (src/regrid.jl:755-803,
+ * 935-947) -- each new leaf from the same old leaf (a copy of its own
+ * range), its old parent (a prolongation) or its old children
+ * (restrictions), the ghost builders' stencils at direction zero -- and,
+ * for a set with a vertex-like dimension, the kept-over-created pass
+ * (regrid_kept_pairs / regrid_kept_stage, src/regrid.jl:826-933). Stage
+ * tags TAH_STAGE_REGRID (sources index the old leaves, desc.nsources of
+ * them) and TAH_STAGE_REGRID_KEPT (sources index the new ones). A plan of
+ * it (tah_plan_create, with the new array's layout) runs with
+ * tah_regrid_transfer only. */
+tah_status tah_regrid_schedule_create(const tah_forest* f, int32_t nold, const int32_t* old_root,
+                                      const int32_t* old_level, const uint32_t* old_coords,
+                                      const tah_operators* ops, const int32_t* G,
+                                      const int32_t* centering, int32_t nvars, const int8_t* parity,
+                                      tah_schedule** out);
 
 /* The schedule's flattened form; pointers stay valid while s lives. */
 tah_status tah_schedule_desc_get(const tah_schedule* s, const tah_schedule_desc** out);
@@ -217,6 +248,17 @@ tah_status tah_plan_values(const tah_plan* p, int64_t* values);
  * (launch_transfer!, src/ghosts.jl:507-521). factors: device table
  * nvars x 3^dim, or NULL when no group is mirrored. */
 tah_status tah_fill_plain(tah_plan* p, double* work, const double* factors, void* stream);
+
+/* The regrid transfer (run_stages! of regrid!, src/regrid.jl:1337-1350 and
+ * 1368-1384): the plan's first stage reads old_work (the set's array
+ * before the regrid, old_layout: the same layout per block, nsources
+ * blocks) and writes new_work's own ranges; the second stage copies within
+ * new_work. new_work's ghosts are not written (the next fill does), and
+ * whatever else it holds stays. As in the reference, the old array's
+ * ghosts must be current (a fill first): a prolongation reads the parent's
+ * ghost layers. Asynchronous on `stream`. */
+tah_status tah_regrid_transfer(tah_plan* p, const tah_layout* old_layout, const double* old_work,
+                               double* new_work, void* stream);
 
 /* The staged fill: one launch per stage group over a tile list of every
  * group of its stages, class-specialised kernels. Same results, bit for bit. */

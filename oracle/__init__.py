@@ -7,6 +7,7 @@ Functions:
   fill(desc, layout, work, factors, nthreads=0, skip=True)   run_phase!(::CPU) per stage
   fill_adjoint(desc, layout, adj, factors)                   CPU transpose (scatter form)
   write_counts(desc, layout)                                 writes per stored point
+  regrid(desc, layout, old, new)                             the regrid transfer (src/regrid.jl)
   scatter(layout, first, extent, work, state)                scatter_kernel! (src/state.jl)
   gather(layout, first, extent, state, work)                 gather_kernel! (src/state.jl)
 """
@@ -37,6 +38,8 @@ def lib():
         L.oracle_fill_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_write_counts.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_max_threads.restype = C.c_int
+        L.oracle_regrid.restype = C.c_int
+        L.oracle_regrid.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         for name in ("oracle_scatter", "oracle_gather"):
             getattr(L, name).restype = C.c_int
             getattr(L, name).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -93,3 +96,12 @@ def gather(layout, first, extent, state, work):
     """state[n] = work[own point] for every state entry n, in place on host arrays."""
     lib().oracle_gather(C.cast(C.pointer(layout), C.c_void_p), _i32(first), _i32(extent), _ptr(state),
                         _ptr(work))
+
+
+def regrid(desc_ptr, layout, old, new, nthreads=0):
+    """The regrid transfer of schedule desc_ptr from host array old into new
+    (layout: the new array's; old has desc.nsources blocks of the same layout)."""
+    rc = lib().oracle_regrid(C.cast(desc_ptr, C.c_void_p), C.cast(C.pointer(layout), C.c_void_p), _ptr(old),
+                             _ptr(new), int(nthreads))
+    if rc:
+        raise RuntimeError(f"oracle_regrid failed ({rc})")

@@ -81,7 +81,7 @@ static int single_row(const tah_stencil1d* s, int32_t col) {
   return nz == 1 ? row : -1;
 }
 
-static void run_transfers(const tah_group_desc* g, const tah_layout* L, double* work,
+static void run_transfers(const tah_group_desc* g, const tah_layout* L, double* work, const double* src,
                           const double* factors, int32_t t0, int32_t t1, int skip) {
   const tah_stencil1d* S = g->stencils;
   const int64_t* st = L->stride;
@@ -90,7 +90,7 @@ static void run_transfers(const tah_group_desc* g, const tah_layout* L, double* 
   int32_t n0 = S[0].ntarget, n1 = S[1].ntarget, n2 = S[2].ntarget;
   for (int32_t t = t0; t < t1; ++t) {
     double* tb = work + st[4] * g->targetblocks[t];
-    const double* sb = work + st[4] * g->sourceblocks[t];
+    const double* sb = src + st[4] * g->sourceblocks[t];
     for (int32_t v = 0; v < L->nvars; ++v) {
       double f = g->factorcol ? factors[v + (int64_t)L->nvars * (g->factorcol - 1)] : 1.0;
       for (int32_t k = 0; k < n2; ++k)
@@ -146,7 +146,35 @@ int oracle_fill(const tah_schedule_desc* d, const tah_layout* L, double* work,
         const tah_group_desc* g = &d->groups[gi];
         if (g->stage != stage) continue;
         span_t tr = owned_transfers(g, own.lo, own.hi);
-        if (tr.lo < tr.hi) run_transfers(g, L, work, factors, tr.lo, tr.hi, skip);
+        if (tr.lo < tr.hi) run_transfers(g, L, work, work, factors, tr.lo, tr.hi, skip);
+      }
+    }
+  }
+  return 0;
+}
+
+/* The regrid transfer (run_stages! in regrid!, src/regrid.jl:1337-1384):
+ * stage REGRID reads old_work (the same layout per block as L, the
+ * schedule's nsources blocks) and writes new_work; stage REGRID_KEPT copies
+ * within new_work. Threads own the new blocks, as the reference threads the
+ * transfer by owner of the new block. */
+int oracle_regrid(const tah_schedule_desc* d, const tah_layout* L, const double* old_work, double* new_work,
+                  int nthreads) {
+  if (!d || !L || !old_work || !new_work) return 1;
+  int T = nthreads > 0 ? nthreads : omp_get_max_threads();
+  if (T > d->nblocks) T = d->nblocks > 0 ? d->nblocks : 1;
+  for (int32_t stage = 0; stage < d->nstages; ++stage) {
+    const double* src = d->stage_tags[stage] == TAH_STAGE_REGRID ? old_work : new_work;
+#pragma omp parallel num_threads(T)
+    {
+      int t = omp_get_thread_num();
+      int nt = omp_get_num_threads();
+      span_t own = equalsplit(d->nblocks, nt, t);
+      for (int32_t gi = 0; gi < d->ngroups; ++gi) {
+        const tah_group_desc* g = &d->groups[gi];
+        if (g->stage != stage) continue;
+        span_t tr = owned_transfers(g, own.lo, own.hi);
+        if (tr.lo < tr.hi) run_transfers(g, L, new_work, src, NULL, tr.lo, tr.hi, 1);
       }
     }
   }

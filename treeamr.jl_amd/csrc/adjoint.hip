@@ -758,6 +758,7 @@ using namespace tah;
 extern "C" tah_status tah_fill_adjoint(tah_plan* p, double* adj, const double* factors, void* stream) {
   return guard([&] {
     if (!p || !adj) fail("null argument");
+    if (p->regrid) fail("the adjoint of a regrid transfer is not provided");
     if (p->needs_factors && !factors)
       fail("this schedule mirrors ghosts across a reflecting face, but no factor table was given");
     HIPCHECK(hipSetDevice(p->ctx->device));

@@ -28,6 +28,59 @@ struct tah_schedule {
   tah_schedule_desc desc{};
 };
 
+namespace {
+
+// The flattened, 0-based descriptor of a built schedule.
+void flatten(tah_schedule* S, int nblocks) {
+  GhostSchedule& gs = *S->s;
+  S->srcstart0.resize(gs.groups.size());
+  S->gdesc.resize(gs.groups.size());
+  for (const Stage& st : gs.stages) S->tags.push_back(st.tag);
+  for (size_t i = 0; i < gs.groups.size(); ++i) {
+    const TransferGroup& tg = gs.groups[i];
+    tah_group_desc& gd = S->gdesc[i];
+    gd.kind = tg.kind;
+    gd.stage = -1;
+    for (size_t k = 0; k < gs.stages.size(); ++k)
+      if (gs.stages[k].tag == tg.stage) gd.stage = int32_t(k);
+    gd.factorcol = tg.factorcol;
+    gd.ntransfers = int32_t(tg.targetblocks.size());
+    gd.targetblocks = tg.targetblocks.data();
+    gd.sourceblocks = tg.sourceblocks.data();
+    for (int d = 0; d < kMaxDim; ++d) {
+      const Stencil1D& s = tg.stencils[d];
+      auto& v = S->srcstart0[i][d];
+      v.resize(s.srcstart.size());
+      for (size_t r = 0; r < v.size(); ++r) v[r] = s.srcstart[r] - 1;
+      gd.stencils[d].targetfirst = s.targetfirst - 1;
+      gd.stencils[d].ntarget = s.ntarget();
+      gd.stencils[d].order = s.order;
+      gd.stencils[d].unit = s.unit ? 1 : 0;
+      gd.stencils[d].srcstart = v.data();
+      gd.stencils[d].weights = s.weights.data();
+    }
+  }
+  for (auto& sg : gs.stagegroups) {
+    S->sgfirst.push_back(sg[0]);
+    S->sglast.push_back(sg[1]);
+  }
+  tah_schedule_desc& d = S->desc;
+  d.dim = gs.D;
+  d.nblocks = nblocks;
+  for (int k = 0; k < kMaxDim; ++k) d.stored[k] = gs.stored[k];
+  d.nstages = int32_t(S->tags.size());
+  d.stage_tags = S->tags.data();
+  d.nstagegroups = int32_t(S->sgfirst.size());
+  d.stagegroup_first = S->sgfirst.data();
+  d.stagegroup_last = S->sglast.data();
+  d.ngroups = int32_t(S->gdesc.size());
+  d.groups = S->gdesc.data();
+  d.generation = gs.generation;
+  d.nsources = gs.nsources;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* tah_last_error(void) { return tah::last_error.c_str(); }
@@ -166,51 +219,52 @@ tah_status tah_schedule_create(const tah_forest* f, const tah_operators* ops, co
     auto par = parities_of(F.D, nvars, parity);
     auto S = std::make_unique<tah_schedule>();
     S->s = std::make_unique<GhostSchedule>(F, o, g, c, par);
-    GhostSchedule& gs = *S->s;
-    S->srcstart0.resize(gs.groups.size());
-    S->gdesc.resize(gs.groups.size());
-    for (const Stage& st : gs.stages) S->tags.push_back(st.tag);
-    for (size_t i = 0; i < gs.groups.size(); ++i) {
-      const TransferGroup& tg = gs.groups[i];
-      tah_group_desc& gd = S->gdesc[i];
-      gd.kind = tg.kind;
-      gd.stage = -1;
-      for (size_t k = 0; k < gs.stages.size(); ++k)
-        if (gs.stages[k].tag == tg.stage) gd.stage = int32_t(k);
-      gd.factorcol = tg.factorcol;
-      gd.ntransfers = int32_t(tg.targetblocks.size());
-      gd.targetblocks = tg.targetblocks.data();
-      gd.sourceblocks = tg.sourceblocks.data();
-      for (int d = 0; d < kMaxDim; ++d) {
-        const Stencil1D& s = tg.stencils[d];
-        auto& v = S->srcstart0[i][d];
-        v.resize(s.srcstart.size());
-        for (size_t r = 0; r < v.size(); ++r) v[r] = s.srcstart[r] - 1;
-        gd.stencils[d].targetfirst = s.targetfirst - 1;
-        gd.stencils[d].ntarget = s.ntarget();
-        gd.stencils[d].order = s.order;
-        gd.stencils[d].unit = s.unit ? 1 : 0;
-        gd.stencils[d].srcstart = v.data();
-        gd.stencils[d].weights = s.weights.data();
-      }
-    }
-    for (auto& sg : gs.stagegroups) {
-      S->sgfirst.push_back(sg[0]);
-      S->sglast.push_back(sg[1]);
-    }
-    tah_schedule_desc& d = S->desc;
-    d.dim = gs.D;
-    d.nblocks = F.nleaves();
-    for (int k = 0; k < kMaxDim; ++k) d.stored[k] = gs.stored[k];
-    d.nstages = int32_t(S->tags.size());
-    d.stage_tags = S->tags.data();
-    d.nstagegroups = int32_t(S->sgfirst.size());
-    d.stagegroup_first = S->sgfirst.data();
-    d.stagegroup_last = S->sglast.data();
-    d.ngroups = int32_t(S->gdesc.size());
-    d.groups = S->gdesc.data();
-    d.generation = gs.generation;
+    flatten(S.get(), F.nleaves());
     *out = S.release();
+  });
+}
+
+tah_status tah_regrid_schedule_create(const tah_forest* f, int32_t nold, const int32_t* old_root,
+                                      const int32_t* old_level, const uint32_t* old_coords,
+                                      const tah_operators* ops, const int32_t* G, const int32_t* centering,
+                                      int32_t nvars, const int8_t* parity, tah_schedule** out) {
+  return guard([&] {
+    if (!f || !ops || !G || !out || !old_root || !old_level || !old_coords) fail("null argument");
+    const Forest& F = *f->f;
+    if (nvars <= 0) fail("nvars must be positive");
+    if (nold <= 0) fail("a regrid needs the leaves before it");
+    std::vector<Key> old(nold);
+    for (int32_t i = 0; i < nold; ++i) {
+      old[i].root = old_root[i];
+      old[i].level = old_level[i];
+      for (int d = 0; d < kMaxDim; ++d) old[i].c[d] = d < F.D ? old_coords[3 * i + d] : 0;
+    }
+    Operators o(ops->family, ops->prolongation, ops->restriction, ops->bounded != 0,
+                ops->vertex_prolongation);
+    int g[kMaxDim] = {0, 0, 0}, c[kMaxDim] = {0, 0, 0};
+    for (int d = 0; d < F.D; ++d) {
+      g[d] = G[d];
+      c[d] = centering ? centering[d] : 0;
+    }
+    auto par = parities_of(F.D, nvars, parity);
+    auto S = std::make_unique<tah_schedule>();
+    S->s = std::make_unique<GhostSchedule>(RegridFrom{old}, F, o, g, c, par);
+    flatten(S.get(), F.nleaves());
+    *out = S.release();
+  });
+}
+
+tah_status tah_forest_coarsen(tah_forest* f, int32_t n, const int32_t* leaves, int32_t* coarsened) {
+  return guard([&] {
+    if (!f || (n > 0 && !leaves)) fail("null argument");
+    Forest& F = *f->f;
+    std::vector<Key> keys;
+    for (int32_t i = 0; i < n; ++i) {
+      if (leaves[i] < 0 || leaves[i] >= F.nleaves()) fail("leaf index out of range");
+      keys.push_back(F.leaves[leaves[i]]);
+    }
+    const int m = F.coarsen(keys);
+    if (coarsened) *coarsened = m;
   });
 }
 

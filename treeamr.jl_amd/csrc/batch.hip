@@ -48,6 +48,7 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
   for (int32_t i = 0; i < n; ++i) {
     tah_plan* p = plans[i];
     if (!p || !works[i]) fail("null plan or working array in the batch");
+    if (p->regrid) fail("plan " + std::to_string(i) + " is a regrid transfer's, not a fill's");
     if (p->ctx->device != ctx->device) fail("every plan of a batch must be on the context's device");
     const double* f = factors ? factors[i] : nullptr;
     if (p->needs_factors && !f)

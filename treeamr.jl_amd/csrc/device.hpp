@@ -69,14 +69,15 @@ struct alignas(16) DevWClass {
 struct alignas(16) Job {
   const DevGroup* groups;
   const DevWClass* wclasses;
-  double* work;
+  double* work;        // the targets' array
+  const double* src;   // the sources' array: work, or a regrid's old array
   const double* factors;
   int64_t stride[5];
   int32_t nvars;
   int32_t vfast;  // stride[3] < stride[0]: variables innermost
   FastDiv dnv;    // division by nvars
   // Variables innermost (stride[3] == 1), nvars and the other strides even
-  // and the array 16-byte aligned: values 2e, 2e+1 of any point are one
+  // and both arrays 16-byte aligned: values 2e, 2e+1 of any point are one
   // aligned 16-byte pair.
   int32_t pairs;
 };

@@ -238,6 +238,22 @@ tah_status tah_plan_create(tah_ctx* ctx, const tah_schedule_desc* d, const tah_l
     }
     if (d->nstages > 0 && (p->stagegroups.empty() || p->stagegroups.back()[1] != d->nstages - 1))
       fail("stage groups must cover every stage");
+    // A regrid transfer: stage REGRID (sources: the old leaves), then at
+    // most stage REGRID_KEPT, each a stage group of its own.
+    p->regrid = d->nstages > 0 && d->stage_tags[0] == TAH_STAGE_REGRID;
+    if (p->regrid) {
+      if (d->nsources < 1) fail("a regrid schedule needs the old leaf count (nsources)");
+      if (d->nstages > 2 || (d->nstages == 2 && d->stage_tags[1] != TAH_STAGE_REGRID_KEPT) ||
+          int(p->stagegroups.size()) != d->nstages)
+        fail("a regrid schedule has stage REGRID, then at most REGRID_KEPT, each in a stage group of its own");
+      p->nsources = d->nsources;
+    } else {
+      for (int32_t k = 0; k < d->nstages; ++k)
+        if (d->stage_tags[k] == TAH_STAGE_REGRID || d->stage_tags[k] == TAH_STAGE_REGRID_KEPT)
+          fail("regrid stages out of place");
+      if (d->nsources != 0 && d->nsources != d->nblocks)
+        fail("only a regrid schedule's sources index another block count");
+    }
     int ncols = 1;
     for (int k = 0; k < d->dim; ++k) ncols *= 3;
     int prev_stage = -1;
@@ -258,8 +274,9 @@ tah_status tah_plan_create(tah_ctx* ctx, const tah_schedule_desc* d, const tah_l
       p->needs_factors = p->needs_factors || g.factorcol != 0;
       h.tblocks.assign(g.targetblocks, g.targetblocks + g.ntransfers);
       h.sblocks.assign(g.sourceblocks, g.sourceblocks + g.ntransfers);
+      const int32_t nsrc = p->regrid && g.stage == 0 ? d->nsources : d->nblocks;
       for (int32_t t = 0; t < g.ntransfers; ++t) {
-        if (h.tblocks[t] < 0 || h.tblocks[t] >= d->nblocks || h.sblocks[t] < 0 || h.sblocks[t] >= d->nblocks)
+        if (h.tblocks[t] < 0 || h.tblocks[t] >= d->nblocks || h.sblocks[t] < 0 || h.sblocks[t] >= nsrc)
           fail(where + "block index out of range");
         if (t > 0 && h.tblocks[t] < h.tblocks[t - 1]) fail(where + "targetblocks must be non-decreasing");
       }
@@ -309,6 +326,7 @@ tah_status tah_plan_values(const tah_plan* p, int64_t* values) {
 tah_status tah_fill_plain(tah_plan* p, double* work, const double* factors, void* stream) {
   return guard([&] {
     if (!p || !work) fail("null argument");
+    if (p->regrid) fail("this plan is a regrid transfer's: run it with tah_regrid_transfer");
     if (p->needs_factors && !factors)
       fail("this schedule mirrors ghosts across a reflecting face, but no factor table was given");
     HIPCHECK(hipSetDevice(p->ctx->device));
@@ -332,9 +350,48 @@ tah_status tah_fill_plain(tah_plan* p, double* work, const double* factors, void
   });
 }
 
+tah_status tah_regrid_transfer(tah_plan* p, const tah_layout* old_layout, const double* old_work,
+                               double* new_work, void* stream) {
+  return guard([&] {
+    if (!p || !old_layout || !old_work || !new_work) fail("null argument");
+    if (!p->regrid) fail("this plan is a fill's: run it with tah_fill");
+    const tah_layout& L = p->layout;
+    if (old_layout->nblocks != p->nsources)
+      fail("the old array holds " + std::to_string(old_layout->nblocks) + " blocks, the regrid's sources " +
+               std::to_string(p->nsources),
+           kLayout);
+    bool same = old_layout->nvars == L.nvars;
+    for (int k = 0; k < 3; ++k) same = same && old_layout->stored[k] == L.stored[k];
+    for (int k = 0; k < 5; ++k) same = same && old_layout->stride[k] == L.stride[k];
+    if (!same) fail("the old and the new array must share one layout per block", kLayout);
+    HIPCHECK(hipSetDevice(p->ctx->device));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    // Stage REGRID reads the old array; stage REGRID_KEPT the new one.
+    const Job j0 = plan_job(p, new_work, nullptr, old_work), j1 = plan_job(p, new_work, nullptr);
+    if (std::memcmp(&j0, &p->last_job, sizeof(Job)) != 0) {
+      p->last_job = j0;
+      HIPCHECK(hipMemcpyAsync(p->djob, &p->last_job, sizeof(Job), hipMemcpyHostToDevice, s));
+    }
+    if (std::memcmp(&j1, &p->last_job2, sizeof(Job)) != 0) {
+      p->last_job2 = j1;
+      HIPCHECK(hipMemcpyAsync(p->djob2, &p->last_job2, sizeof(Job), hipMemcpyHostToDevice, s));
+    }
+    if (p->events.empty())
+      for (int i = 0; i < 8; ++i) {
+        hipEvent_t e;
+        HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        p->events.push_back(e);
+      }
+    for (size_t k = 0; k < p->launches.size(); ++k)
+      run_launches(p->ctx, p->launches[k], s, k == 0 ? p->djob : p->djob2, p->dtiles, p->events.data(),
+                   p->fork != 0);
+  });
+}
+
 tah_status tah_fill(tah_plan* p, double* work, const double* factors, void* stream) {
   return guard([&] {
     if (!p || !work) fail("null argument");
+    if (p->regrid) fail("this plan is a regrid transfer's: run it with tah_regrid_transfer");
     if (p->needs_factors && !factors)
       fail("this schedule mirrors ghosts across a reflecting face, but no factor table was given");
     HIPCHECK(hipSetDevice(p->ctx->device));
@@ -360,6 +417,7 @@ tah_status tah_fill_timed(tah_plan* p, double* work, const double* factors, void
                           int32_t* cls, int64_t* values, float* ms) {
   return guard([&] {
     if (!p || !n) fail("null argument");
+    if (p->regrid) fail("this plan is a regrid transfer's: run it with tah_regrid_transfer");
     int32_t total = 0;
     for (const auto& sg : p->launches) total += int32_t(sg.size());
     if (!cls && !values && !ms) {

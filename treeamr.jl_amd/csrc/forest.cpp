@@ -253,6 +253,69 @@ void Forest::refine(const std::vector<Key>& keys) {
   generation += 1;
 }
 
+int Forest::coarsen(const std::vector<Key>& keys) {
+  std::vector<char> flag(leaves.size(), 0);
+  for (const Key& k : keys) {
+    const int i = find_leaf(k);
+    if (i < 0) fail("cannot coarsen a key that is not a leaf of the forest");
+    flag[i] = 1;
+  }
+  auto parent_of = [this](const Key& k) {
+    Key p = k;
+    p.level -= 1;
+    for (int d = 0; d < D; ++d) p.c[d] = k.c[d] >> 1;
+    return p;
+  };
+  // Candidate families: all children leaves and flagged. Children of one
+  // parent are consecutive along the curve.
+  const int nkids = 1 << D;
+  std::vector<Key> parents;
+  for (size_t i = 0; i + nkids <= leaves.size(); ++i) {
+    const Key& k = leaves[i];
+    if (k.level == 0 || !flag[i]) continue;
+    const Key p = parent_of(k);
+    bool family = true;
+    for (int o = 0; o < nkids && family; ++o) {
+      const Key& x = leaves[i + o];
+      family = x.level == k.level && parent_of(x) == p && flag[i + o];
+    }
+    if (!family) continue;
+    parents.push_back(p);
+    i += nkids - 1;
+  }
+  const std::vector<Key> before = leaves;
+  // Drop families whose parent would break 2:1 balance, until none does.
+  while (true) {
+    leaves.clear();
+    size_t q = 0;
+    for (size_t i = 0; i < before.size();) {
+      if (q < parents.size() && before[i].level > 0 && parent_of(before[i]) == parents[q]) {
+        leaves.push_back(parents[q++]);
+        i += nkids;
+      } else {
+        leaves.push_back(before[i++]);
+      }
+    }
+    std::vector<Key> keep;
+    bool dropped = false;
+    auto dirs = alldirections(D);
+    for (const Key& p : parents) {
+      bool ok = true;
+      for (const Dir& delta : dirs) {
+        if (!ok) break;
+        for (int j : neighbors(p, delta))
+          if (leaves[j].level > p.level + 1) ok = false;
+      }
+      if (ok) keep.push_back(p);
+      else dropped = true;
+    }
+    if (!dropped) break;
+    parents.swap(keep);
+  }
+  if (!parents.empty()) generation += 1;
+  return int(parents.size());
+}
+
 void Forest::balance() {
   auto dirs = alldirections(D);
   while (true) {

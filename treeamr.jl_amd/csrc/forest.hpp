@@ -95,6 +95,12 @@ struct Forest {
 
   // src/forest.jl:1761-1781: replace each key by its 2^D children.
   void refine(const std::vector<Key>& keys);
+  // Replace every family whose 2^D children are all listed leaves by its
+  // parent (the coarsening flags of regrid!, src/regrid.jl), keeping 2:1
+  // balance: a family whose parent would meet a leaf two levels finer
+  // stays, as the reference's completed leaves keep it. Other keys are
+  // ignored. Returns the number of families coarsened.
+  int coarsen(const std::vector<Key>& keys);
   // src/forest.jl:1837-1870: 2:1 across faces, edges and corners.
   void balance();
   bool isbalanced() const;

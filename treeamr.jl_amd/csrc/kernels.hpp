@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(kThreads) generic_kernel(const Job* __restrict
                 s4 = job.stride[4];
   const int32_t nvars = job.nvars;
   const uint32_t total = uint32_t(g.points) * uint32_t(nvars);
-  const double* sbase = job.work + s4 * g.sblocks[tile.transfer];
+  const double* sbase = job.src + s4 * g.sblocks[tile.transfer];
   double* tbase = job.work + s4 * g.tblocks[tile.transfer];
   for (int r = 0; r < kGenericPerThread; ++r) {
     uint32_t e = uint32_t(tile.e0) + uint32_t(r * kThreads) + threadIdx.x;
@@ -174,7 +174,7 @@ __device__ __forceinline__ void copy_body(const Job& job, const Tile& tile) {
   const int64_t s4 = job.stride[4];
   const int32_t nvars = job.nvars;
   const uint32_t total = uint32_t(g.points) * uint32_t(nvars);
-  const double* __restrict__ sbase = job.work + s4 * g.sblocks[tile.transfer];
+  const double* __restrict__ sbase = job.src + s4 * g.sblocks[tile.transfer];
   double* __restrict__ tbase = job.work + s4 * g.tblocks[tile.transfer];
   const int32_t tb0 = s0 * g.tfirst[0] + s1 * g.tfirst[1] + s2 * g.tfirst[2];
   const int32_t sb0 = s0 * g.ca[0] + s1 * g.ca[1] + s2 * g.ca[2];
@@ -296,7 +296,7 @@ __device__ __forceinline__ void interp_body(const Job& job, const Tile& tile, co
                 s2 = int32_t(job.stride[2]), s3 = int32_t(job.stride[3]);
   const int64_t s4 = job.stride[4];
   const int32_t ra[2] = {un[0], un[1]}, rb[2] = {un[2], un[3]}, rc[2] = {un[4], un[5]};
-  const double* __restrict__ src = job.work + s4 * g.sblocks[tr] +
+  const double* __restrict__ src = job.src + s4 * g.sblocks[tr] +
                                    (s3 * v + s0 * g.srcstart[0][ra[0]] + s1 * g.srcstart[1][rb[0]] +
                                     s2 * g.srcstart[2][rc[0]]);
   // The table is the same for every lane: read it through the constant

@@ -243,6 +243,10 @@ void build_plan_device(tah_plan* p) {
   HIPCHECK(hipMemcpy(p->dtables, blob.host.data(), blob.host.size(), hipMemcpyHostToDevice));
   HIPCHECK(hipMalloc(&p->djob, sizeof(Job)));
   std::memset(&p->last_job, 0, sizeof(Job));
+  if (p->regrid) {
+    HIPCHECK(hipMalloc(&p->djob2, sizeof(Job)));
+    std::memset(&p->last_job2, 0, sizeof(Job));
+  }
   build_launches(p);
 }
 
@@ -291,7 +295,8 @@ void build_launches(tah_plan* p) {
           const PlanGroup& g = p->groups[gi];
           if (g.points == 0 || sg_of_stage[g.stage] != k) continue;
           const Box3 wb = wbox(g), rb = rbox(g);
-          for (size_t t = 0; t < g.tblocks.size() && p->defer; ++t) {
+          // (a regrid's stages read different arrays: nothing moves)
+          for (size_t t = 0; t < g.tblocks.size() && p->defer && !p->regrid; ++t) {
             bool ok = true;
             auto ir = later_reads.find(g.tblocks[t]);
             if (ir != later_reads.end())
@@ -440,12 +445,13 @@ void build_launches(tah_plan* p) {
   }
 }
 
-Job plan_job(const tah_plan* p, double* work, const double* factors) {
+Job plan_job(const tah_plan* p, double* work, const double* factors, const double* src) {
   Job job;
   std::memset(&job, 0, sizeof(Job));
   job.groups = p->dgroups;
   job.wclasses = p->dwclasses;
   job.work = work;
+  job.src = src ? src : work;
   job.factors = factors;
   for (int k = 0; k < 5; ++k) job.stride[k] = p->layout.stride[k];
   job.nvars = p->layout.nvars;
@@ -453,7 +459,8 @@ Job plan_job(const tah_plan* p, double* work, const double* factors) {
   job.dnv = FastDiv(uint32_t(p->layout.nvars));
   const int64_t* st = p->layout.stride;
   job.pairs = st[3] == 1 && p->layout.nvars % 2 == 0 && st[0] % 2 == 0 && st[1] % 2 == 0 && st[2] % 2 == 0 &&
-                      st[4] % 2 == 0 && reinterpret_cast<uintptr_t>(work) % 16 == 0
+                      st[4] % 2 == 0 && reinterpret_cast<uintptr_t>(work) % 16 == 0 &&
+                      reinterpret_cast<uintptr_t>(job.src) % 16 == 0
                   ? 1
                   : 0;
   return job;
@@ -506,4 +513,5 @@ tah_plan::~tah_plan() {
   if (dtables) (void)hipFree(dtables);
   if (dtiles) (void)hipFree(dtiles);
   if (djob) (void)hipFree(djob);
+  if (djob2) (void)hipFree(djob2);
 }

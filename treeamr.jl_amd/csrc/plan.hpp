@@ -98,6 +98,13 @@ struct tah_plan {
   tah::Tile* dtiles = nullptr;
   tah::Job* djob = nullptr;
   tah::Job last_job{};
+  // A regrid transfer (stage tags TAH_STAGE_REGRID, then
+  // TAH_STAGE_REGRID_KEPT): its first stage reads the old array, whose
+  // nsources blocks its sources index, through the second Job.
+  bool regrid = false;
+  int32_t nsources = 0;
+  tah::Job* djob2 = nullptr;
+  tah::Job last_job2{};
   std::vector<std::vector<tah::Launch>> launches;  // per stage group
   int64_t deferred = 0;            // transfers moved to run with the last stage group
   // Run a stage group's launches forked over the context's streams (1) or
@@ -132,8 +139,9 @@ void validate_strides(const tah_layout* l);
 void build_plan_device(tah_plan* p);
 // (Re)build the tiles and launches for the plan's options.
 void build_launches(tah_plan* p);
-// The Job of one fill of `work` by plan p.
-Job plan_job(const tah_plan* p, double* work, const double* factors);
+// The Job of one fill of `work` by plan p; sources from `src` (a regrid's
+// old array) where given, else from work.
+Job plan_job(const tah_plan* p, double* work, const double* factors, const double* src = nullptr);
 // Streams of the context (created on first use).
 const std::vector<hipStream_t>& ctx_streams(tah_ctx* c);
 // Run a stage group's launches forked over the context's streams and

@@ -620,46 +620,19 @@ struct Builder {
     return true;
   }
 
-  // Stage 0's transfers into leaf b (unify_transfers and merged_transfers,
-  // src/schedule.jl:1990-2021, 2077-2122), recorded; returns b's primary
-  // bits, one per piece of its closed box.
-  uint32_t unify_sources(int b) {
-    Table t(F, b);
-    const int npieces = [&] {
-      int n = 1;
-      for (int d = 0; d < D; ++d) n *= 3;
-      return n;
-    }();
-    std::vector<int> shared(npieces, 0);
-    std::vector<PieceCopy> prim(npieces);
-    uint32_t bits = 0;
-    struct Entry {
-      Dir sub;
-      int kind, source;
-      Dir delta, offset;
-    };
-    std::vector<Entry> entries;
-    for (const Dir& sub : subpieces()) {
-      Dir cls = piece_of(sub);
-      int i = piece_index(cls);
-      if (shared[i] == 0) {
-        auto copies = piece_copies(t, representative(cls));
-        prim[i] = primary(copies);
-        bool onelevel = true;
-        for (const auto& x : copies) onelevel = onelevel && x.level == t.level;
-        shared[i] = onelevel ? 1 : 2;
-        if (prim[i].leaf == b && prim[i].cls == cls) bits |= 1u << i;
-      }
-      bool interior = true;
-      for (int d = 0; d < D; ++d) interior = interior && cls[d] == 1;
-      if (interior) continue;
-      PieceCopy p = shared[i] == 1 ? prim[i] : primary(piece_copies(t, sub));
-      if (p.leaf == b && p.cls == cls) continue;
-      entries.push_back({sub, p.level == t.level ? KindCopy : KindRestrict, p.leaf, p.delta, p.offset});
-    }
-    // merged_transfers: by (kind, source, direction, offset), sorted.
+  // One sub-piece of leaf b and the copy it takes its value from.
+  struct SubEntry {
+    Dir sub;
+    int kind, source;
+    Dir delta, offset;
+  };
+
+  // merged_transfers (src/schedule.jl:2077-2122): b's entries by (kind,
+  // source, direction, offset), each set recorded as one box of spans
+  // where it fills one, else per piece, else per sub-piece.
+  void record_merged(int b, const std::vector<SubEntry>& entries) {
     std::map<std::tuple<int, int, Dir, Dir>, std::vector<Dir>> bysource;
-    for (const Entry& e : entries) {
+    for (const SubEntry& e : entries) {
       auto& v = bysource[std::make_tuple(e.kind, e.source, e.delta, e.offset)];
       if (std::find(v.begin(), v.end(), e.sub) == v.end()) v.push_back(e.sub);
     }
@@ -692,6 +665,87 @@ struct Builder {
         }
       }
     }
+  }
+
+  // The kept-over-created pass into created leaf j (regrid_kept_pairs,
+  // src/regrid.jl:826-877): every sub-piece of j's closed box off its open
+  // interior that j holds at the finest level holding it takes the value
+  // of the primary among the kept blocks holding it there. kept[x]: leaf
+  // x was a leaf before the regrid.
+  void kept_sources(int j, const std::vector<char>& kept) {
+    Table t(F, j);
+    // Most created blocks have no kept neighbour on their level across a
+    // face, edge or corner of a vertex-like dimension: skip them whole.
+    bool any = false;
+    int n = 1;
+    for (int d = 0; d < D; ++d) n *= 3;
+    for (int q = 0; q < n && !any; ++q) {
+      Dir delta{0, 0, 0};
+      bool zero = true, ok = true;
+      for (int d = 0, r = q; d < D; ++d, r /= 3) {
+        delta[d] = r % 3 - 1;
+        zero = zero && delta[d] == 0;
+        ok = ok && (c[d] == 1 || delta[d] == 0);
+      }
+      if (zero || !ok) continue;
+      const Table::Entry& e = t.at(delta);
+      if (e.level != t.level) continue;
+      for (const auto& nb : e.nbrs) any = any || kept[nb.first];
+    }
+    if (!any) return;
+    std::vector<SubEntry> entries;
+    for (const Dir& sub : subpieces()) {
+      Dir cls = piece_of(sub);
+      bool interior = true;
+      for (int d = 0; d < D; ++d) interior = interior && cls[d] == 1;
+      if (interior) continue;
+      auto copies = piece_copies(t, sub);
+      int finest = -1;
+      for (const auto& x : copies) finest = std::max(finest, x.level);
+      if (t.level != finest) continue;
+      std::vector<PieceCopy> holders;
+      for (const auto& x : copies)
+        if (x.level == finest && kept[x.leaf]) holders.push_back(x);
+      if (holders.empty()) continue;
+      PieceCopy p = primary(holders);
+      entries.push_back({sub, KindCopy, p.leaf, p.delta, p.offset});
+    }
+    record_merged(j, entries);
+  }
+
+  // Stage 0's transfers into leaf b (unify_transfers and merged_transfers,
+  // src/schedule.jl:1990-2021, 2077-2122), recorded; returns b's primary
+  // bits, one per piece of its closed box.
+  uint32_t unify_sources(int b) {
+    Table t(F, b);
+    const int npieces = [&] {
+      int n = 1;
+      for (int d = 0; d < D; ++d) n *= 3;
+      return n;
+    }();
+    std::vector<int> shared(npieces, 0);
+    std::vector<PieceCopy> prim(npieces);
+    uint32_t bits = 0;
+    std::vector<SubEntry> entries;
+    for (const Dir& sub : subpieces()) {
+      Dir cls = piece_of(sub);
+      int i = piece_index(cls);
+      if (shared[i] == 0) {
+        auto copies = piece_copies(t, representative(cls));
+        prim[i] = primary(copies);
+        bool onelevel = true;
+        for (const auto& x : copies) onelevel = onelevel && x.level == t.level;
+        shared[i] = onelevel ? 1 : 2;
+        if (prim[i].leaf == b && prim[i].cls == cls) bits |= 1u << i;
+      }
+      bool interior = true;
+      for (int d = 0; d < D; ++d) interior = interior && cls[d] == 1;
+      if (interior) continue;
+      PieceCopy p = shared[i] == 1 ? prim[i] : primary(piece_copies(t, sub));
+      if (p.leaf == b && p.cls == cls) continue;
+      entries.push_back({sub, p.level == t.level ? KindCopy : KindRestrict, p.leaf, p.delta, p.offset});
+    }
+    record_merged(b, entries);
     return bits;
   }
 
@@ -858,9 +912,7 @@ struct Builder {
 
 }  // namespace
 
-GhostSchedule::GhostSchedule(const Forest& f, const Operators& ops_, const int* G_, const int* c_,
-                             const std::vector<std::array<int8_t, kMaxDim>>& par)
-    : forest(&f), generation(f.generation), D(f.D), N(f.N), ops(ops_), parities(par) {
+void GhostSchedule::check_layout(const int* G_, const int* c_) {
   for (int d = 0; d < kMaxDim; ++d) {
     G[d] = d < D ? G_[d] : 0;
     c[d] = d < D ? c_[d] : 0;
@@ -872,6 +924,135 @@ GhostSchedule::GhostSchedule(const Forest& f, const Operators& ops_, const int* 
     stored[d] = d < D ? N + 2 * G[d] + c[d] : 1;
   }
   check_operators(N, G.data(), c.data(), D, ops);
+}
+
+// One group per key (src/schedule.jl:3193-3232, 2970-2992).
+void GhostSchedule::add_group(const GroupKey& key, const std::vector<int32_t>& tb, const std::vector<int32_t>& sb,
+                              int stage) {
+  TransferGroup g;
+  g.key = key;
+  g.kind = key.kind;
+  g.targetblocks = tb;
+  g.sourceblocks = sb;
+  for (int d = 0; d < kMaxDim; ++d) {
+    if (d >= D) {
+      g.stencils[d] = unit_point();
+      continue;
+    }
+    int dd = key.direction[d], od = key.offset[d], st = key.mirror[d];
+    // A span of stage 0 or P: a copy or a restriction into the target's
+    // closed box (unify_stencil, src/schedule.jl:973-981).
+    if (key.isunify(D)) {
+      g.stencils[d] = key.kind == KindCopy
+                          ? copy_stencil(N, G[d], c[d], dd, key.span[d])
+                          : restrict_stencil_over(piece_range(N, G[d], c[d], key.span[d]), N, G[d], c[d], dd,
+                                                  od, ops.restriction);
+      continue;
+    }
+    int ds = st == 0 ? dd : 0;
+    Stencil1D s = (st != 0 && key.kind == KindRestrict && c[d] == 1)
+                      ? mirror_restriction(N, G[d], od, ops.restriction)
+                      : kind_stencil(N, G[d], c[d], key.kind, ds, od, ops, key.span[d]);
+    if (st == 3) s = clamp_rows(s, N, G[d], c[d], dd);
+    else if (st != 0) s = mirror_rows(s, N, G[d], c[d], dd);
+    g.stencils[d] = std::move(s);
+  }
+  // src/schedule.jl:3218-3222: clamped dimensions carry no sign.
+  bool signed_ = false;
+  int col = 1, p3 = 1;
+  for (int d = 0; d < D; ++d) {
+    int s = key.mirror[d] == 3 ? 0 : key.mirror[d];
+    signed_ = signed_ || s != 0;
+    col += s * p3;
+    p3 *= 3;
+  }
+  g.factorcol = signed_ ? col : 0;
+  g.stage = stage;
+  groups.push_back(std::move(g));
+}
+
+GhostSchedule::GhostSchedule(const RegridFrom& from, const Forest& f, const Operators& ops_, const int* G_,
+                             const int* c_, const std::vector<std::array<int8_t, kMaxDim>>& par)
+    : forest(&f), generation(f.generation), D(f.D), N(f.N), ops(ops_), parities(par) {
+  check_layout(G_, c_);
+  const std::vector<Key>& old = from.oldleaves;
+  if (old.empty()) fail("a regrid needs the leaves before it");
+  nsources = int32_t(old.size());
+  auto less = [&](const Key& a, const Key& b) { return curve_less(a, b, D); };
+  for (size_t i = 1; i < old.size(); ++i)
+    if (!less(old[i - 1], old[i])) fail("the old leaves must be sorted along the curve, without repeats");
+  auto find_old = [&](const Key& k) {
+    auto it = std::lower_bound(old.begin(), old.end(), k, less);
+    return it != old.end() && *it == k ? int(it - old.begin()) : -1;
+  };
+  // Where each new leaf's data come from (classify_regrid!,
+  // src/regrid.jl:620-652): the old leaf itself, its parent, or each of its
+  // children, batched by (kind, child offset) with direction zero.
+  Builder B(f, G, c);
+  const Dir zero{0, 0, 0};
+  const std::array<int8_t, kMaxDim> nomirror{0, 0, 0};
+  std::vector<char> kept(f.nleaves(), 0);
+  for (int j = 0; j < f.nleaves(); ++j) {
+    const Key& k = f.leaves[j];
+    const int same = find_old(k);
+    if (same >= 0) {
+      B.record(KindCopy, zero, zero, 0, nomirror, j, same);
+      kept[j] = 1;
+      continue;
+    }
+    if (k.level > 0) {
+      Key parent = k;
+      parent.level -= 1;
+      for (int d = 0; d < D; ++d) parent.c[d] = k.c[d] >> 1;
+      const int p = find_old(parent);
+      if (p >= 0) {
+        B.record(KindProlong, zero, B.childoffset(k), 0, nomirror, j, p);
+        continue;
+      }
+    }
+    if (k.level >= kMaxLevel) fail("cannot rebuild a new leaf: it is neither an old leaf, a child of one, nor a parent");
+    for (int o = 0; o < (1 << D); ++o) {
+      Key ch = k;
+      ch.level += 1;
+      for (int d = 0; d < D; ++d) ch.c[d] = 2 * k.c[d] + ((o >> d) & 1u);
+      const int x = find_old(ch);
+      if (x < 0)
+        fail("cannot rebuild a new leaf: neither it, its parent, nor its child was a leaf before the regrid. "
+             "A single regrid may move a block by at most one level, which holds when the previous tree was "
+             "2:1 balanced.");
+      B.record(KindRestrict, zero, B.childoffset(ch), 0, nomirror, j, x);
+    }
+  }
+  // Every target is the new block's own range: the ghost builders at
+  // direction zero (regrid_stage, src/regrid.jl:755-803).
+  for (auto& kv : B.pairs) add_group(kv.first, kv.second.first, kv.second.second, kRegridTag);
+  std::vector<int> tags{kRegridTag};
+  bool closed = false;
+  for (int d = 0; d < D; ++d) closed = closed || c[d] == 1;
+  if (closed) {
+    // The kept blocks' copies of shared points over the created blocks'
+    // (regrid_kept_pairs, regrid_kept_stage, src/regrid.jl:826-933).
+    Builder K(f, G, c);
+    for (int j = 0; j < f.nleaves(); ++j)
+      if (!kept[j]) K.kept_sources(j, kept);
+    for (auto& kv : K.pairs) add_group(kv.first, kv.second.first, kv.second.second, kRegridKeptTag);
+    tags.push_back(kRegridKeptTag);
+  }
+  std::stable_sort(groups.begin(), groups.end(), [](const TransferGroup& a, const TransferGroup& b) {
+    if (a.stage != b.stage) return a.stage < b.stage;
+    return key_less(a.key, b.key);
+  });
+  for (int t : tags) stages.push_back(Stage{t, {}});
+  for (int i = 0; i < (int)groups.size(); ++i)
+    for (auto& st : stages)
+      if (st.tag == groups[i].stage) st.groups.push_back(i);
+  for (int i = 0; i < (int)tags.size(); ++i) stagegroups.push_back({i, i});
+}
+
+GhostSchedule::GhostSchedule(const Forest& f, const Operators& ops_, const int* G_, const int* c_,
+                             const std::vector<std::array<int8_t, kMaxDim>>& par)
+    : forest(&f), generation(f.generation), D(f.D), N(f.N), ops(ops_), parities(par) {
+  check_layout(G_, c_);
   bool closed = false;
   for (int d = 0; d < D; ++d) closed = closed || c[d] == 1;
   // A variable needs a parity in every dimension that has a reflecting face
@@ -903,50 +1084,6 @@ GhostSchedule::GhostSchedule(const Forest& f, const Operators& ops_, const int* 
   }
   boundaries = std::move(B.boundaries);
 
-  // One group per key (src/schedule.jl:3193-3232, 2970-2992).
-  auto add_group = [&](const GroupKey& key, const std::vector<int32_t>& tb, const std::vector<int32_t>& sb,
-                       int stage) {
-    TransferGroup g;
-    g.key = key;
-    g.kind = key.kind;
-    g.targetblocks = tb;
-    g.sourceblocks = sb;
-    for (int d = 0; d < kMaxDim; ++d) {
-      if (d >= D) {
-        g.stencils[d] = unit_point();
-        continue;
-      }
-      int dd = key.direction[d], od = key.offset[d], st = key.mirror[d];
-      // A span of stage 0 or P: a copy or a restriction into the target's
-      // closed box (unify_stencil, src/schedule.jl:973-981).
-      if (key.isunify(D)) {
-        g.stencils[d] = key.kind == KindCopy
-                            ? copy_stencil(N, G[d], c[d], dd, key.span[d])
-                            : restrict_stencil_over(piece_range(N, G[d], c[d], key.span[d]), N, G[d], c[d],
-                                                    dd, od, ops.restriction);
-        continue;
-      }
-      int ds = st == 0 ? dd : 0;
-      Stencil1D s = (st != 0 && key.kind == KindRestrict && c[d] == 1)
-                        ? mirror_restriction(N, G[d], od, ops.restriction)
-                        : kind_stencil(N, G[d], c[d], key.kind, ds, od, ops, key.span[d]);
-      if (st == 3) s = clamp_rows(s, N, G[d], c[d], dd);
-      else if (st != 0) s = mirror_rows(s, N, G[d], c[d], dd);
-      g.stencils[d] = std::move(s);
-    }
-    // src/schedule.jl:3218-3222: clamped dimensions carry no sign.
-    bool signed_ = false;
-    int col = 1, p3 = 1;
-    for (int d = 0; d < D; ++d) {
-      int s = key.mirror[d] == 3 ? 0 : key.mirror[d];
-      signed_ = signed_ || s != 0;
-      col += s * p3;
-      p3 *= 3;
-    }
-    g.factorcol = signed_ ? col : 0;
-    g.stage = stage;
-    groups.push_back(std::move(g));
-  };
   for (auto& kv : B.pairs) {
     const GroupKey& key = kv.first;
     // src/schedule.jl:653-656

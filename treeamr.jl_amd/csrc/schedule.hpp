@@ -28,6 +28,11 @@ constexpr int kStageP = -1;
 constexpr int kUnifyTag = 0;
 constexpr int kPhase1Tag = 1;
 inline int prolongation_tag(int level) { return 2 + level; }
+// The regrid transfer (REGRID_TAG, src/regrid.jl:755-803): old array -> new
+// array; and its kept-over-created pass within the new array
+// (REGRID_KEPT_TAG, src/regrid.jl:810-933).
+constexpr int kRegridTag = 100;
+constexpr int kRegridKeptTag = 101;
 
 // Parities per variable and dimension (src/storage.jl, NoParity /
 // EvenParity / OddParity).
@@ -96,9 +101,14 @@ struct Stage {
   std::vector<int> groups;  // indices into GhostSchedule::groups
 };
 
+struct RegridFrom {
+  const std::vector<Key>& oldleaves;  // the leaves before the regrid, sorted
+};
+
 struct GhostSchedule {
   const Forest* forest = nullptr;
   int64_t generation = 0;
+  int32_t nsources = 0;  // a regrid's old leaves (sources); 0: the forest's leaves
   int D = 3, N = 8;
   std::array<int, kMaxDim> G{0, 0, 0}, c{0, 0, 0}, stored{1, 1, 1};
   Operators ops;
@@ -110,6 +120,18 @@ struct GhostSchedule {
 
   GhostSchedule(const Forest& f, const Operators& ops, const int* G, const int* c,
                 const std::vector<std::array<int8_t, kMaxDim>>& parities);
+  // The regrid transfer onto f's leaves from `from.oldleaves`
+  // (transfer_groups and regrid_stage, src/regrid.jl:755-803, 935-947; the
+  // kept-over-created pass, regrid_kept_pairs and regrid_kept_stage,
+  // :826-933): stage kRegridTag, and kRegridKeptTag where the set has a
+  // vertex-like dimension. Targets index f's leaves, sources of the first
+  // stage the old leaves, of the second f's.
+  GhostSchedule(const RegridFrom& from, const Forest& f, const Operators& ops, const int* G, const int* c,
+                const std::vector<std::array<int8_t, kMaxDim>>& parities);
+
+ private:
+  void check_layout(const int* G, const int* c);
+  void add_group(const GroupKey& key, const std::vector<int32_t>& tb, const std::vector<int32_t>& sb, int stage);
 };
 
 // src/storage.jl:536-554: factors[v + nvars*(col-1)] for col in 1..3^D.

@@ -151,6 +151,7 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
                     double* best_ms) {
   return guard([&] {
     if (!p || !work) fail("null argument");
+    if (p->regrid) fail("tah_tune times fills; this plan is a regrid transfer's");
     if (p->needs_factors && !factors)
       fail("this schedule mirrors ghosts across a reflecting face, but no factor table was given");
     HIPCHECK(hipSetDevice(p->ctx->device));

@@ -34,7 +34,8 @@ class tah_schedule_desc(C.Structure):
                 ("nstages", C.c_int32), ("stage_tags", C.POINTER(C.c_int32)),
                 ("nstagegroups", C.c_int32), ("stagegroup_first", C.POINTER(C.c_int32)),
                 ("stagegroup_last", C.POINTER(C.c_int32)), ("ngroups", C.c_int32),
-                ("groups", C.POINTER(tah_group_desc)), ("generation", C.c_int64)]
+                ("groups", C.POINTER(tah_group_desc)), ("generation", C.c_int64),
+                ("nsources", C.c_int32), ("pad_", C.c_int32)]
 
 
 class tah_layout(C.Structure):
@@ -93,6 +94,10 @@ SIGNATURES = {
     "tah_plan_set_options": [P, C.POINTER(tah_plan_options)],
     "tah_tune": [P, P, P, C.c_char_p, C.POINTER(C.c_double)],
     "tah_state_length": [C.POINTER(tah_layout), I32P, C.POINTER(C.c_int64)],
+    "tah_forest_coarsen": [P, C.c_int32, I32P, I32P],
+    "tah_regrid_schedule_create": [P, C.c_int32, I32P, I32P, C.POINTER(C.c_uint32), C.POINTER(tah_operators),
+                                   I32P, I32P, C.c_int32, C.POINTER(C.c_int8), C.POINTER(P)],
+    "tah_regrid_transfer": [P, C.POINTER(tah_layout), P, P, P],
     "tah_scatter": [C.POINTER(tah_layout), I32P, I32P, P, P, P],
     "tah_gather": [C.POINTER(tah_layout), I32P, I32P, P, P, P],
 }

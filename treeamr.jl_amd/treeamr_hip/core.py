@@ -96,6 +96,15 @@ class Forest:
         check(lib().tah_forest_refine(self._h, len(idx), idx.ctypes.data_as(_abi.I32P)))
         return self
 
+    def coarsen(self, leaf_indices):
+        """Merge every family whose children are all among the given (0-based)
+        leaves into its parent, where 2:1 balance allows; returns the number
+        of families merged (regrid!'s coarsening flags)."""
+        idx = np.ascontiguousarray(np.asarray(list(leaf_indices), dtype=np.int32))
+        m = C.c_int32()
+        check(lib().tah_forest_coarsen(self._h, len(idx), idx.ctypes.data_as(_abi.I32P), C.byref(m)))
+        return m.value
+
     def balance(self):
         check(lib().tah_forest_balance(self._h))
         return self
@@ -381,16 +390,7 @@ class FieldSet:
         self.stored = tuple(forest.N + 2 * self.G[d] + self.centering[d] for d in range(D)) + (1,) * (3 - D)
         self.device = device
         self.layout_kind = layout
-        # The layout needs only the stored size and the block count.
-        desc = _abi.tah_schedule_desc()
-        desc.dim = D
-        desc.nblocks = forest.nleaves
-        for d in range(3):
-            desc.stored[d] = self.stored[d]
-        self.layout = _layout_of(C.byref(desc), self.nvars, layout)
-        n = C.c_int64()
-        check(lib().tah_layout_size(C.byref(self.layout), C.byref(n)))
-        self.nelements = n.value
+        self._relayout()
         self.work = torch.full((self.nelements,), float(fill), dtype=torch.float64, device=device)
         self.factors = None
         if parity is not None:
@@ -401,6 +401,20 @@ class FieldSet:
                                            f.ctypes.data_as(_abi.DP)))
             self.factors = torch.tensor(f, dtype=torch.float64, device=device)
         self.generation = forest.generation
+
+    def _relayout(self):
+        """The layout of the forest's current leaves (the stored size and
+        the block count are all it needs)."""
+        desc = _abi.tah_schedule_desc()
+        desc.dim = self.forest.D
+        desc.nblocks = self.forest.nleaves
+        for d in range(3):
+            desc.stored[d] = self.stored[d]
+        self.layout = _layout_of(C.byref(desc), self.nvars, self.layout_kind)
+        n = C.c_int64()
+        check(lib().tah_layout_size(C.byref(self.layout), C.byref(n)))
+        self.nelements = n.value
+        self.generation = self.forest.generation
 
     @property
     def nblocks(self):
@@ -662,6 +676,104 @@ def gather(u, fs):
         return u
     _state_copy(lambda L, f, e, w, st, sp: lib().tah_gather(L, f, e, st, w, sp), fs, u)
     return u
+
+
+# ---------------------------------------------------------------------------
+# Regrid (src/regrid.jl): refine and coarsen the forest and move every field
+# set's own points onto the new leaves.
+
+class RegridSchedule:
+    """The regrid transfer of one field-set layout from the leaves `old`
+    (root, level, coords arrays as Forest.leaf_arrays gives them) onto the
+    forest's current leaves (transfer_groups / regrid_stage and the
+    kept-over-created pass, src/regrid.jl:755-960)."""
+
+    def __init__(self, forest, old, operators, *, G, centering, nvars, parity=None):
+        root, level, coords = (np.ascontiguousarray(a) for a in old)
+        D = forest.D
+        par = None
+        if parity is not None:
+            flat = [int(x) for p in parity for x in p]
+            par = (C.c_int8 * len(flat))(*flat)
+        h = C.c_void_p()
+        check(lib().tah_regrid_schedule_create(
+            forest._h, len(root), root.ctypes.data_as(_abi.I32P), level.ctypes.data_as(_abi.I32P),
+            coords.astype(np.uint32).ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(operators._c()),
+            _i32(G if hasattr(G, "__len__") else (G,) * D), _i32(_centering_codes(centering, D)), int(nvars),
+            par, C.byref(h)))
+        self._h = h
+        dp = C.POINTER(_abi.tah_schedule_desc)()
+        check(lib().tah_schedule_desc_get(h, C.byref(dp)))
+        self.desc = dp.contents
+        self._descptr = dp
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _abi is not None and _abi._lib is not None:
+            _abi._lib.tah_schedule_destroy(h)
+
+    def stage_tags(self):
+        return [self.desc.stage_tags[i] for i in range(self.desc.nstages)]
+
+    def groups(self):
+        return GhostSchedule.groups(self)
+
+
+def regrid_transfer(fs, old_layout, old_work, schedule, *, ctx=None):
+    """The transfer alone: a fresh zeroed array of fs's layout (fs.layout
+    already the new one) filled from old_work over `schedule` (a
+    RegridSchedule). Returns the new array; ghosts are left for the next fill."""
+    import torch
+    ctx = ctx or default_context()
+    plan = Plan(ctx, schedule, fs.layout)
+    fresh = torch.zeros(fs.nelements, dtype=torch.float64, device=old_work.device)
+    check(lib().tah_regrid_transfer(plan._h, C.byref(old_layout), C.c_void_p(old_work.data_ptr()),
+                                    C.c_void_p(fresh.data_ptr()),
+                                    C.c_void_p(torch.cuda.current_stream(old_work.device).cuda_stream)))
+    return fresh, plan
+
+
+def regrid(forest, pairs, *, refine=(), coarsen=(), ctx=None):
+    """regrid! (src/regrid.jl:1269-1395) with refinement and coarsening given
+    as leaf indices: fill every set's ghosts (a prolongation reads its
+    parent's), coarsen the families whose children are all listed (where
+    2:1 balance allows), refine the listed leaves and rebalance, then move
+    each set's own points onto the new leaves -- kept blocks copied, new
+    ones prolongated from their parent or restricted from their children --
+    and, for a set with a vertex-like dimension, give every created copy of
+    a shared point the kept block's value. Each set's storage is replaced
+    in place (fs.work, fs.layout); schedules built before are stale.
+    Returns True if the mesh changed."""
+    import torch
+    pairs = [pairs] if isinstance(pairs, tuple) and len(pairs) == 2 and isinstance(pairs[0], FieldSet) else \
+        list(pairs)
+    for fs, sched in pairs:
+        if fs.forest is not forest:
+            raise ValueError("every field set must be over the forest being regridded")
+        _check_fill(fs, sched)
+    refine, coarsen = list(refine), list(coarsen)
+    if set(refine) & set(coarsen):
+        raise ValueError("a leaf flagged both for refinement and for coarsening")
+    old = forest.leaf_arrays()
+    keys = forest.leaves
+    for fs, sched in pairs:
+        fill_ghosts(fs, sched, ctx=ctx)
+    if coarsen:
+        forest.coarsen(coarsen)
+    if refine:
+        now = {k: i for i, k in enumerate(forest.leaves)}
+        forest.refine([now[keys[i]] for i in refine])
+    forest.balance()
+    if forest.leaves == keys:
+        return False
+    for fs, sched in pairs:
+        rs = RegridSchedule(forest, old, sched.operators, G=fs.G, centering=fs.centering, nvars=fs.nvars,
+                            parity=fs.parity)
+        old_layout, old_work = fs.layout, fs.work
+        fs._relayout()
+        fs.work, _ = regrid_transfer(fs, old_layout, old_work, rs, ctx=ctx)
+        torch.cuda.current_stream(old_work.device).synchronize()
+    return True
 
 
 def _batch_args(pairs, ctx):
