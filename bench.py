@@ -122,6 +122,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-state", action="store_true")
     ap.add_argument("--no-regrid", action="store_true")
+    ap.add_argument("--no-unify", action="store_true")
     ap.add_argument("--config2-repeats", type=int, default=100)
     ap.add_argument("--batched-steps", type=int, default=3)
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
@@ -138,7 +139,7 @@ def main():
     from treeamr_hip import _abi
     from treeamr_hip import (FieldSet, FillGraph, GhostSchedule, fill_ghosts, fill_ghosts_adjoint,
                              fill_ghosts_timed, gather, ownextents, regrid_transfer, scatter, statelength,
-                             tune, RegridSchedule)
+                             tune, Operators, RegridSchedule, unify)
     from treeamr_hip import workloads as W
 
     def builder(nvars):
@@ -405,6 +406,35 @@ def main():
             "scatter": round(1.0 / min(regrid_rec[l]["scatter_ms"] for l in ("arena", "dense")), 2)}
         log(f"[bench] regrid transfer: {regrid_rec}")
 
+    unify_rec = None
+    if not args.no_unify:
+        # unify! alone, as bench/rhs.jl times it (unify!(du, fs, schedule) on a
+        # state-shaped du; the reference's uniform mesh, N=16, vertex, G=3,
+        # 20 variables, order-6 operators), and on the working array. The
+        # H200 figure is derived: unify! alone is 10-23% of the 0.540 ns per
+        # own point of the every-point right-hand side (CODE.md:1296-1308).
+        fU, _, kwU = W.uniform_vertex(N=16, roots=8)
+        kwU = dict(kwU)
+        nvU = kwU.pop("nvars")
+        opsU = Operators(prolongation=6, restriction=6)
+        unify_rec = {"workload": "uniform 8^3 roots of 16^3, vertex, G=3, 20 vars, PointValue(6,6) (bench/rhs.jl)",
+                     "h200_ns_per_point_derived": [round(0.10 * 0.540, 3), round(0.23 * 0.540, 3)],
+                     "h200_source": "CODE.md:1296-1308 (10-23% of 0.540 ns per point)"}
+        for layout in ("arena", "dense"):
+            fsU = FieldSet(fU, nvU, device="cuda", layout=layout, fill=0.0, **kwU)
+            W.fill_uniform(fsU, 9)
+            sU = GhostSchedule(fsU, opsU)
+            e = ownextents(fsU)
+            pts = fU.nleaves * e[0] * e[1] * e[2]
+            du = torch.rand(statelength(fsU), device="cuda", dtype=torch.float64)
+            tv = time_fill(lambda: unify(du, fsU, sU), 20, args.warmup, stream)
+            tw = time_fill(lambda: unify(fsU, sU), 20, args.warmup, stream)
+            unify_rec[layout] = {"state_ns_per_point": round(tv["min_ms"] * 1e6 / pts, 4),
+                                 "work_ns_per_point": round(tw["min_ms"] * 1e6 / pts, 4),
+                                 "state_ms": round(tv["min_ms"], 4), "work_ms": round(tw["min_ms"], 4)}
+            del fsU, sU, du
+        log(f"[bench] unify: {unify_rec}")
+
     cpu = None
     if not args.no_cpu:
         log("[bench] timing the CPU baseline (oracle)")
@@ -448,6 +478,7 @@ def main():
         "reference_meshes": refmeshes,
         "state_copies": state,
         "regrid_transfer": regrid_rec,
+        "unify": unify_rec,
         "schedule_build_s": round(build_s, 3),
         "roofline": {
             "kernel": dominant,

@@ -187,6 +187,21 @@ tah_status tah_regrid_schedule_create(const tah_forest* f, int32_t nold, const i
                                       const int32_t* centering, int32_t nvars, const int8_t* parity,
                                       tah_schedule** out);
 
+/* UnifySchedule(fs) (src/unify.jl:76-170): stage P and stage 0 alone, for a
+ * set with a vertex-like dimension (refused otherwise), with the
+ * restriction the geometry fixes -- injection along a vertex-like
+ * dimension, the exact two-cell average along a cell-like one -- and any
+ * ghost width G >= 0 (the set needs no ghosts). */
+tah_status tah_unify_schedule_create(const tah_forest* f, const int32_t* G, const int32_t* centering,
+                                     int32_t nvars, const int8_t* parity, tah_schedule** out);
+/* What unify! replays (unify_arrays!, src/ghosts.jl:1874-1906): stages P and
+ * 0 of a ghost or unify schedule as a schedule of their own, in the working
+ * array's indices (state = 0) or a state vector's (state = 1: every index
+ * shifted by -G, stored = the own extents, for a dense layout of the state
+ * vector). A plan of it filled with tah_fill is unify!. Valid while s
+ * lives. */
+tah_status tah_unify_desc_get(tah_schedule* s, int32_t state, const tah_schedule_desc** out);
+
 /* The schedule's flattened form; pointers stay valid while s lives. */
 tah_status tah_schedule_desc_get(const tah_schedule* s, const tah_schedule_desc** out);
 

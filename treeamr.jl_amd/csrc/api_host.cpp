@@ -19,6 +19,14 @@ struct tah_forest {
   std::unique_ptr<Forest> f;
 };
 
+// A flattened, 0-based descriptor and the arrays it points into.
+struct FlatDesc {
+  std::vector<std::array<std::vector<int32_t>, kMaxDim>> srcstart0;
+  std::vector<tah_group_desc> gdesc;
+  std::vector<int32_t> tags, sgfirst, sglast;
+  tah_schedule_desc desc{};
+};
+
 struct tah_schedule {
   std::unique_ptr<GhostSchedule> s;
   // The flattened, 0-based form handed out through tah_schedule_desc_get.
@@ -26,6 +34,9 @@ struct tah_schedule {
   std::vector<tah_group_desc> gdesc;
   std::vector<int32_t> tags, sgfirst, sglast;
   tah_schedule_desc desc{};
+  // Stages P and 0 alone, in the working array's indices (0) or a state
+  // vector's (1), built on first request (tah_unify_desc_get).
+  std::unique_ptr<FlatDesc> unify[2];
 };
 
 namespace {
@@ -77,6 +88,76 @@ void flatten(tah_schedule* S, int nblocks) {
   d.groups = S->gdesc.data();
   d.generation = gs.generation;
   d.nsources = gs.nsources;
+}
+
+// Stages P and 0 of a schedule (unify_arrays!, src/ghosts.jl:1874-1906): on
+// the working array, or with `state` on a state-shaped array, every index
+// shifted by -G and the stored size the own extent (state_unify_stages,
+// src/unify.jl).
+std::unique_ptr<FlatDesc> unify_desc(const tah_schedule* S, bool state) {
+  const GhostSchedule& gs = *S->s;
+  auto U = std::make_unique<FlatDesc>();
+  std::vector<int> map(gs.stages.size(), -1);
+  for (size_t k = 0; k < gs.stages.size(); ++k)
+    if (gs.stages[k].tag == kStageP || gs.stages[k].tag == kUnifyTag) {
+      map[k] = int(U->tags.size());
+      U->tags.push_back(gs.stages[k].tag);
+    }
+  if (U->tags.empty() || U->tags.back() != kUnifyTag)
+    fail("this schedule has no stage 0: its set has no vertex-like dimension, so nothing to unify");
+  int own[kMaxDim];
+  for (int d = 0; d < kMaxDim; ++d) own[d] = d < gs.D ? gs.N + gs.c[d] : 1;
+  for (size_t i = 0; i < gs.groups.size(); ++i) {
+    const TransferGroup& tg = gs.groups[i];
+    int k = -1;
+    for (size_t q = 0; q < gs.stages.size(); ++q)
+      if (gs.stages[q].tag == tg.stage) k = map[q];
+    if (k < 0) continue;
+    tah_group_desc gd{};
+    gd.kind = tg.kind;
+    gd.stage = k;
+    gd.factorcol = tg.factorcol;
+    gd.ntransfers = int32_t(tg.targetblocks.size());
+    gd.targetblocks = tg.targetblocks.data();
+    gd.sourceblocks = tg.sourceblocks.data();
+    U->srcstart0.emplace_back();
+    auto& ss = U->srcstart0.back();
+    for (int d = 0; d < kMaxDim; ++d) {
+      const Stencil1D& s = tg.stencils[d];
+      const int shift = state && d < gs.D ? gs.G[d] : 0;
+      ss[d].resize(s.srcstart.size());
+      for (size_t r = 0; r < ss[d].size(); ++r) {
+        ss[d][r] = s.srcstart[r] - 1 - shift;
+        if (state && (ss[d][r] < 0 || ss[d][r] + s.order > own[d]))
+          fail("a stage-0 transfer reads outside the own range; this is a bug");
+      }
+      gd.stencils[d].targetfirst = s.targetfirst - 1 - shift;
+      if (state && (gd.stencils[d].targetfirst < 0 || gd.stencils[d].targetfirst + s.ntarget() > own[d]))
+        fail("a stage-0 transfer writes outside the own range; this is a bug");
+      gd.stencils[d].ntarget = s.ntarget();
+      gd.stencils[d].order = s.order;
+      gd.stencils[d].unit = s.unit ? 1 : 0;
+      gd.stencils[d].weights = s.weights.data();
+    }
+    U->gdesc.push_back(gd);
+  }
+  for (size_t i = 0; i < U->gdesc.size(); ++i)
+    for (int d = 0; d < kMaxDim; ++d) U->gdesc[i].stencils[d].srcstart = U->srcstart0[i][d].data();
+  for (size_t k = 0; k < U->tags.size(); ++k) {
+    U->sgfirst.push_back(int32_t(k));
+    U->sglast.push_back(int32_t(k));
+  }
+  tah_schedule_desc& d = U->desc;
+  d = S->desc;
+  for (int k = 0; k < kMaxDim; ++k) d.stored[k] = state ? own[k] : gs.stored[k];
+  d.nstages = int32_t(U->tags.size());
+  d.stage_tags = U->tags.data();
+  d.nstagegroups = int32_t(U->sgfirst.size());
+  d.stagegroup_first = U->sgfirst.data();
+  d.stagegroup_last = U->sglast.data();
+  d.ngroups = int32_t(U->gdesc.size());
+  d.groups = U->gdesc.data();
+  return U;
 }
 
 }  // namespace
@@ -251,6 +332,34 @@ tah_status tah_regrid_schedule_create(const tah_forest* f, int32_t nold, const i
     S->s = std::make_unique<GhostSchedule>(RegridFrom{old}, F, o, g, c, par);
     flatten(S.get(), F.nleaves());
     *out = S.release();
+  });
+}
+
+tah_status tah_unify_schedule_create(const tah_forest* f, const int32_t* G, const int32_t* centering,
+                                     int32_t nvars, const int8_t* parity, tah_schedule** out) {
+  return guard([&] {
+    if (!f || !G || !centering || !out) fail("null argument");
+    const Forest& F = *f->f;
+    if (nvars <= 0) fail("nvars must be positive");
+    int g[kMaxDim] = {0, 0, 0}, c[kMaxDim] = {0, 0, 0};
+    for (int d = 0; d < F.D; ++d) {
+      g[d] = G[d];
+      c[d] = centering[d];
+    }
+    auto par = parities_of(F.D, nvars, parity);
+    auto S = std::make_unique<tah_schedule>();
+    S->s = std::make_unique<GhostSchedule>(UnifyOnly{}, F, g, c, par);
+    flatten(S.get(), F.nleaves());
+    *out = S.release();
+  });
+}
+
+tah_status tah_unify_desc_get(tah_schedule* s, int32_t state, const tah_schedule_desc** out) {
+  return guard([&] {
+    if (!s || !out) fail("null argument");
+    const int k = state ? 1 : 0;
+    if (!s->unify[k]) s->unify[k] = unify_desc(s, state != 0);
+    *out = &s->unify[k]->desc;
   });
 }
 

@@ -912,7 +912,7 @@ struct Builder {
 
 }  // namespace
 
-void GhostSchedule::check_layout(const int* G_, const int* c_) {
+void GhostSchedule::check_layout(const int* G_, const int* c_, bool operators) {
   for (int d = 0; d < kMaxDim; ++d) {
     G[d] = d < D ? G_[d] : 0;
     c[d] = d < D ? c_[d] : 0;
@@ -923,7 +923,7 @@ void GhostSchedule::check_layout(const int* G_, const int* c_) {
       fail("N must be >= 2G in every dimension, got N=" + std::to_string(N) + ", G=" + std::to_string(G[d]));
     stored[d] = d < D ? N + 2 * G[d] + c[d] : 1;
   }
-  check_operators(N, G.data(), c.data(), D, ops);
+  if (operators) check_operators(N, G.data(), c.data(), D, ops);
 }
 
 // One group per key (src/schedule.jl:3193-3232, 2970-2992).
@@ -969,6 +969,46 @@ void GhostSchedule::add_group(const GroupKey& key, const std::vector<int32_t>& t
   g.factorcol = signed_ ? col : 0;
   g.stage = stage;
   groups.push_back(std::move(g));
+}
+
+GhostSchedule::GhostSchedule(UnifyOnly, const Forest& f, const int* G_, const int* c_,
+                             const std::vector<std::array<int8_t, kMaxDim>>& par)
+    : forest(&f), generation(f.generation), D(f.D), N(f.N), ops(PointValue, 2, 2, false, 0), parities(par) {
+  check_layout(G_, c_, false);
+  bool closed = false;
+  for (int d = 0; d < D; ++d) closed = closed || c[d] == 1;
+  if (!closed)
+    fail("a cell-centered field set has no shared point to unify: its values sit half a cell in from every "
+         "face. Build it over the set that holds the face quantity, vertex-like in the face dimension.");
+  bool reflecting = false;
+  for (int d = 0; d < D; ++d) reflecting = reflecting || f.faces[d][0] == ReflectingFace || f.faces[d][1] == ReflectingFace;
+  for (size_t v = 0; v < par.size(); ++v)
+    for (int d = 0; d < D; ++d)
+      if ((f.faces[d][0] == ReflectingFace || f.faces[d][1] == ReflectingFace) && par[v][d] == NoParity)
+        fail("variable " + std::to_string(v + 1) + " has NoParity in dimension " + std::to_string(d + 1) +
+             ", which has a reflecting face");
+  // Stage 0 (unify_sources!) and stage P (selfimage_sources!) of every
+  // block, as the fill builds them (src/unify.jl:105-160).
+  Builder B(f, G, c);
+  std::map<GroupKey, std::pair<std::vector<int32_t>, std::vector<int32_t>>, KeyLess> ppairs;
+  for (int b = 0; b < f.nleaves(); ++b) {
+    const uint32_t bits = B.unify_sources(b);
+    B.selfimage_sources(b, bits, reflecting ? par : std::vector<std::array<int8_t, kMaxDim>>{}, ppairs);
+  }
+  for (auto& kv : B.pairs) add_group(kv.first, kv.second.first, kv.second.second, kUnifyTag);
+  for (auto& kv : ppairs) add_group(kv.first, kv.second.first, kv.second.second, kStageP);
+  std::stable_sort(groups.begin(), groups.end(), [](const TransferGroup& a, const TransferGroup& b) {
+    if (a.stage != b.stage) return a.stage < b.stage;
+    return key_less(a.key, b.key);
+  });
+  std::vector<int> tags;
+  if (!ppairs.empty()) tags.push_back(kStageP);
+  tags.push_back(kUnifyTag);
+  for (int t : tags) stages.push_back(Stage{t, {}});
+  for (int i = 0; i < (int)groups.size(); ++i)
+    for (auto& st : stages)
+      if (st.tag == groups[i].stage) st.groups.push_back(i);
+  for (int i = 0; i < (int)tags.size(); ++i) stagegroups.push_back({i, i});
 }
 
 GhostSchedule::GhostSchedule(const RegridFrom& from, const Forest& f, const Operators& ops_, const int* G_,

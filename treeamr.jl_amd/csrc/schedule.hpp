@@ -105,6 +105,10 @@ struct RegridFrom {
   const std::vector<Key>& oldleaves;  // the leaves before the regrid, sorted
 };
 
+// Stages P and 0 alone, with the restriction the geometry fixes (a
+// UnifySchedule, src/unify.jl:76-170).
+struct UnifyOnly {};
+
 struct GhostSchedule {
   const Forest* forest = nullptr;
   int64_t generation = 0;
@@ -128,9 +132,15 @@ struct GhostSchedule {
   // stage the old leaves, of the second f's.
   GhostSchedule(const RegridFrom& from, const Forest& f, const Operators& ops, const int* G, const int* c,
                 const std::vector<std::array<int8_t, kMaxDim>>& parities);
+  // UnifySchedule(fs) (src/unify.jl:76-170): stage P and stage 0 of a set
+  // with a vertex-like dimension, its restriction injection along a
+  // vertex-like dimension and the exact two-cell average (order 2) along a
+  // cell-like one, for any ghost width (none needed).
+  GhostSchedule(UnifyOnly, const Forest& f, const int* G, const int* c,
+                const std::vector<std::array<int8_t, kMaxDim>>& parities);
 
  private:
-  void check_layout(const int* G, const int* c);
+  void check_layout(const int* G, const int* c, bool operators = true);
   void add_group(const GroupKey& key, const std::vector<int32_t>& tb, const std::vector<int32_t>& sb, int stage);
 };
 

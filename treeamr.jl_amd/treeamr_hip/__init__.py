@@ -18,6 +18,6 @@ from .core import (  # noqa: F401
     PointValue, Conservative, NoParity, EvenParity, OddParity, cellcentered, vertexcentered,
     Context, default_context, Plan, fill_ghosts_batched, FillGraph, fill_ghosts_timed, kernel_class_name, tune,
     ownextents, statelength, statevector, staterange, stateview, statearray, scatter, gather,
-    RegridSchedule, regrid, regrid_transfer,
+    RegridSchedule, regrid, regrid_transfer, UnifySchedule, unify, unify_desc,
 )
 from ._abi import TahError, StaleScheduleError, LayoutError, UnsupportedError  # noqa: F401

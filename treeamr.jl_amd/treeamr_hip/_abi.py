@@ -95,6 +95,8 @@ SIGNATURES = {
     "tah_tune": [P, P, P, C.c_char_p, C.POINTER(C.c_double)],
     "tah_state_length": [C.POINTER(tah_layout), I32P, C.POINTER(C.c_int64)],
     "tah_forest_coarsen": [P, C.c_int32, I32P, I32P],
+    "tah_unify_schedule_create": [P, I32P, I32P, C.c_int32, C.POINTER(C.c_int8), C.POINTER(P)],
+    "tah_unify_desc_get": [P, C.c_int32, C.POINTER(C.POINTER(tah_schedule_desc))],
     "tah_regrid_schedule_create": [P, C.c_int32, I32P, I32P, C.POINTER(C.c_uint32), C.POINTER(tah_operators),
                                    I32P, I32P, C.c_int32, C.POINTER(C.c_int8), C.POINTER(P)],
     "tah_regrid_transfer": [P, C.POINTER(tah_layout), P, P, P],

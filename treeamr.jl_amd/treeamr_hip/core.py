@@ -679,6 +679,124 @@ def gather(u, fs):
 
 
 # ---------------------------------------------------------------------------
+# unify! (src/ghosts.jl:1836-1906, src/unify.jl): stages P and 0 alone.
+
+class UnifySchedule:
+    """UnifySchedule(fs) (src/unify.jl:76-170): stage P and stage 0 of a set
+    with a vertex-like dimension, alone, with the restriction the geometry
+    fixes (injection along a vertex-like dimension, the exact two-cell
+    average along a cell-like one) and any ghost width."""
+
+    def __init__(self, fs):
+        self.forest = fs.forest
+        self.G, self.centering, self.nvars, self.parity = fs.G, fs.centering, fs.nvars, fs.parity
+        par = None
+        if self.parity is not None:
+            flat = [x for p in self.parity for x in p]
+            par = (C.c_int8 * len(flat))(*flat)
+        h = C.c_void_p()
+        check(lib().tah_unify_schedule_create(fs.forest._h, _i32(self.G), _i32(self.centering), self.nvars, par,
+                                              C.byref(h)))
+        self._h = h
+        self.generation = fs.forest.generation
+        dp = C.POINTER(_abi.tah_schedule_desc)()
+        check(lib().tah_schedule_desc_get(h, C.byref(dp)))
+        self.desc = dp.contents
+        self._descptr = dp
+        self._plans = {}
+
+    def __del__(self):
+        self._plans = {}
+        h = getattr(self, "_h", None)
+        if h and _abi is not None and _abi._lib is not None:
+            _abi._lib.tah_schedule_destroy(h)
+
+    def isstale(self):
+        return self.forest.generation != self.generation
+
+    def stage_tags(self):
+        return [self.desc.stage_tags[i] for i in range(self.desc.nstages)]
+
+    def groups(self):
+        return GhostSchedule.groups(self)
+
+
+class _Desc:
+    """A derived descriptor (stages P and 0 of a schedule), kept with it."""
+
+    def __init__(self, owner, dp):
+        self.owner = owner
+        self._descptr = dp
+        self.desc = dp.contents
+
+    def groups(self):
+        return GhostSchedule.groups(self)
+
+
+def unify_desc(schedule, state=False):
+    """Stages P and 0 of a GhostSchedule or UnifySchedule as a descriptor of
+    their own: in the working array's indices, or with state=True a state
+    vector's (tah_unify_desc_get)."""
+    dp = C.POINTER(_abi.tah_schedule_desc)()
+    check(lib().tah_unify_desc_get(schedule._h, 1 if state else 0, C.byref(dp)))
+    return _Desc(schedule, dp)
+
+
+def _check_unify(fs, schedule):
+    """check_unify (src/unify.jl): the schedule is current and built for this
+    set's forest and layout."""
+    if isinstance(schedule, GhostSchedule):
+        return _check_fill(fs, schedule)
+    if fs.forest is not schedule.forest:
+        raise ValueError("the field set and the unify schedule were built over different forests")
+    if schedule.isstale():
+        raise _abi.StaleScheduleError(_abi.TAH_ESTALE, "the unify schedule is stale: the forest has changed "
+                                      "since it was built; rebuild it")
+    if fs.nblocks != schedule.forest.nleaves:
+        raise ValueError(f"the field set has {fs.nblocks} blocks but the forest has {schedule.forest.nleaves}")
+    if fs.G != schedule.G or fs.centering != schedule.centering or fs.nvars != schedule.nvars:
+        raise ValueError("the unify schedule was built for another layout (G, centering or nvars)")
+    if schedule.parity is not None and fs.parity != schedule.parity:
+        raise ValueError("the unify schedule was built for other parities than the field set's")
+
+
+def unify(*args, ctx=None):
+    """unify!(fs, schedule) and unify!(v, fs, schedule) (src/ghosts.jl:1836-1860):
+    stages P and 0 of the schedule (a GhostSchedule or a UnifySchedule) on
+    the field set's working array, or on the state vector v, whose every
+    secondary copy of a shared point then holds its primary's value. On the
+    field set's current stream."""
+    import torch
+    if len(args) == 2:
+        (fs, schedule), v = args, None
+    elif len(args) == 3:
+        v, fs, schedule = args
+    else:
+        raise TypeError("unify(fs, schedule) or unify(v, fs, schedule)")
+    _check_unify(fs, schedule)
+    ctx = ctx or default_context()
+    state = v is not None
+    if state:
+        _check_statelength(v, fs)
+        if str(v.dtype) != "torch.float64" or v.device != fs.work.device or not v.is_contiguous():
+            raise ValueError("the state vector must be a contiguous Float64 tensor on the field set's GPU")
+    dsc = unify_desc(schedule, state)
+    layout = fs.layout
+    if state:
+        layout = _abi.tah_layout()
+        check(lib().tah_layout_dense(dsc._descptr, fs.nvars, C.byref(layout)))
+    key = (state, tuple(layout.stride), layout.nvars, ctx.device)
+    plans = schedule._plans.setdefault("unify", {})
+    if key not in plans:
+        plans[key] = (Plan(ctx, dsc, layout), dsc)
+    plan = plans[key][0]
+    fptr = C.c_void_p(fs.factors.data_ptr()) if fs.factors is not None else None
+    target = v if state else fs.work
+    check(lib().tah_fill(plan._h, C.c_void_p(target.data_ptr()), fptr, _stream_ptr(fs)))
+    return v if state else fs
+
+
+# ---------------------------------------------------------------------------
 # Regrid (src/regrid.jl): refine and coarsen the forest and move every field
 # set's own points onto the new leaves.
 
