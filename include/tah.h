@@ -316,7 +316,10 @@ tah_status tah_fill_adjoint(tah_plan* p, double* adj, const double* factors, voi
  * reads or overwrites into the last stage group (1, default); fuse is
  * refused with TAH_EUNSUPPORTED unless 0; copy_wgs > 0 launches the copies
  * from a bounded grid, leaving wave slots to interpolations forked beside
- * them. Setting merge, defer, copy_per_thread or copy_wgs rebuilds the
+ * them; interp_pairs gives each thread of a merged interpolation two
+ * neighbouring variables (layouts with variables innermost and an even
+ * count; 16-byte loads where the array is 16-byte aligned) or one (0).
+ * Setting merge, defer, copy_per_thread, copy_wgs or interp_pairs rebuilds the
  * plan's tiles (it waits for the device first);
  * batches and graphs made earlier keep the options they were made with. */
 typedef struct tah_plan_options {
@@ -326,7 +329,8 @@ typedef struct tah_plan_options {
   int32_t copy_per_thread; /* the copy kernel's tile: 4, 8 (default) or 16 values per thread */
   int32_t fuse;            /* must be 0: the fused copy-and-interpolation launch is switched off */
   int32_t copy_wgs;        /* copies from a bounded grid of this many workgroups (a multiple of 8), or 0 */
-  int32_t reserved_[2];
+  int32_t interp_pairs;    /* merged interpolations over variable pairs where the layout allows (1, default) */
+  int32_t reserved_[1];
 } tah_plan_options;
 tah_status tah_plan_get_options(const tah_plan* p, tah_plan_options* out);
 tah_status tah_plan_set_options(tah_plan* p, const tah_plan_options* o);

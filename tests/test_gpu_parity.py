@@ -82,6 +82,35 @@ def test_arena_even_variables_bit_exact(method, offset):
     _check(fs, sched, method)
 
 
+@pytest.mark.parametrize("family,orders", [("pointvalue", (2, 2)), ("pointvalue", (4, 4)), ("pointvalue", (6, 6)),
+                                            ("pointvalue", (6, 2)), ("conservative", (3, 2)),
+                                            ("conservative", (5, 2))])
+@pytest.mark.parametrize("centering", ["cell", "vertex"])
+@pytest.mark.parametrize("offset", [0, 1])
+def test_variable_pair_interpolation_every_order(family, orders, centering, offset):
+    """Arena layout, 6 variables: the merged interpolations take two
+    neighbouring variables per thread, with 16-byte accesses (offset 0) or
+    single ones (the array 8 bytes past a 16-byte boundary), for every
+    specialised order of prolongation and restriction."""
+    from treeamr_hip import Conservative, Forest, Operators
+    if family == "conservative" and centering == "vertex":
+        pytest.skip("the conservative family is cell-centred")
+    f = Forest((2, 2, 2), N=8, periodic=(True,) * 3)
+    f.refine([0])
+    f.balance()
+    f.refine([3])
+    f.balance()
+    ops = Operators(prolongation=orders[0], restriction=orders[1],
+                    **({"family": Conservative} if family == "conservative" else {}))
+    fs, sched = _prepare(f, ops, dict(nvars=6, G=3, centering=centering), layout="arena", seed=sum(orders))
+    buf = torch.empty(fs.nelements + 1, dtype=torch.float64, device="cuda")
+    view = buf[offset:offset + fs.nelements]
+    view.copy_(fs.work)
+    fs.work = view
+    assert sched.plan(fs.layout).options["interp_pairs"] == 1
+    _check(fs, sched, "staged")
+
+
 @pytest.mark.parametrize("seed", [2, 3, 4, 5, 6, 12])
 def test_random_forests_bit_exact(seed):
     """Forests of the batched configuration, cell (seeds 2, 4, 5) and vertex

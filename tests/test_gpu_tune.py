@@ -13,8 +13,8 @@ pytestmark = pytest.mark.gpu
 
 
 def _case(which):
-    if which == "cell":
-        f, ops, kw, _ = W.config5(nvars=3, roots=4, N=16)
+    if which in ("cell", "cell-even"):
+        f, ops, kw, _ = W.config5(nvars=3 if which == "cell" else 4, roots=4, N=16)
     else:
         f, ops, kw, _ = W.config2(roots=2, N=8, radii=(0.5, 0.3))
     kw = dict(kw)
@@ -23,23 +23,27 @@ def _case(which):
     return fs, GhostSchedule(fs, ops)
 
 
-@pytest.mark.parametrize("which", ["cell", "vertex"])
+@pytest.mark.parametrize("which", ["cell", "cell-even", "vertex"])
 def test_every_option_combination_fills_alike(which):
+    """Odd (3) and even (4, 8) variable counts: the even ones take the
+    variable-pair interpolations when interp_pairs is on."""
     fs, s = _case(which)
     init = fs.work.clone()
     fill_ghosts(fs, s)
     torch.cuda.synchronize()
     ref = fs.work.clone()
     plan = s.plan(fs.layout)
-    for fork, merge, defer, cpt, wgs in itertools.product((0, 1), (0, 1), (0, 1), (4, 8, 16), (0, 64, 512)):
-        print(f"[tune] {which} fork={fork} merge={merge} defer={defer} copy_per_thread={cpt} copy_wgs={wgs}",
-              flush=True)
-        plan.set_options(fork=fork, merge=merge, fuse=0, defer=defer, copy_per_thread=cpt, copy_wgs=wgs)
+    for fork, merge, defer, cpt, wgs, ip in itertools.product((0, 1), (0, 1), (0, 1), (4, 8, 16), (0, 64, 512),
+                                                              (0, 1)):
+        print(f"[tune] {which} fork={fork} merge={merge} defer={defer} copy_per_thread={cpt} copy_wgs={wgs} "
+              f"interp_pairs={ip}", flush=True)
+        plan.set_options(fork=fork, merge=merge, fuse=0, defer=defer, copy_per_thread=cpt, copy_wgs=wgs,
+                         interp_pairs=ip)
         fs.work.copy_(init)
         fill_ghosts(fs, s)
         torch.cuda.synchronize()
-        assert torch.equal(fs.work.view(torch.int64), ref.view(torch.int64)), (fork, merge, defer, cpt, wgs)
-    plan.set_options(fork=0, merge=1, fuse=0, defer=1, copy_per_thread=8, copy_wgs=0)
+        assert torch.equal(fs.work.view(torch.int64), ref.view(torch.int64)), (fork, merge, defer, cpt, wgs, ip)
+    plan.set_options(fork=0, merge=1, fuse=0, defer=1, copy_per_thread=8, copy_wgs=0, interp_pairs=1)
 
 
 def test_tune_caches_by_signature(tmp_path):
@@ -48,7 +52,7 @@ def test_tune_caches_by_signature(tmp_path):
     opts, ms = tune(fs, s, path)
     assert ms is not None and ms > 0
     lines = path.read_text().splitlines()
-    assert len(lines) == 1 and lines[0].startswith("tah-tune-2 ")
+    assert len(lines) == 1 and lines[0].startswith("tah-tune-3 ")
     s.plan(fs.layout).set_options(fork=1 - opts["fork"])
     opts2, ms2 = tune(fs, s, path)
     assert ms2 is None and opts2 == opts

@@ -72,6 +72,17 @@ void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, cons
     hipLaunchKernelGGL(copy_kernel<8>, grid, blk, 0, s, jobs, tiles);
   } else if (cls == copy_launch(16)) {
     hipLaunchKernelGGL(copy_kernel<16>, grid, blk, 0, s, jobs, tiles);
+  } else if (cls >= kClsMergedPair) {
+    switch (cls - kClsMergedPair) {
+      case 0: hipLaunchKernelGGL((interp_pair_kernel<2, 0>), grid, blk, 0, s, jobs, tiles); break;
+      case 1: hipLaunchKernelGGL((interp_pair_kernel<3, 0>), grid, blk, 0, s, jobs, tiles); break;
+      case 2: hipLaunchKernelGGL((interp_pair_kernel<4, 0>), grid, blk, 0, s, jobs, tiles); break;
+      case 3: hipLaunchKernelGGL((interp_pair_kernel<5, 0>), grid, blk, 0, s, jobs, tiles); break;
+      case 4: hipLaunchKernelGGL((interp_pair_kernel<6, 0>), grid, blk, 0, s, jobs, tiles); break;
+      case 5: hipLaunchKernelGGL((interp_pair_kernel<2, 2>), grid, blk, 0, s, jobs, tiles); break;
+      case 6: hipLaunchKernelGGL((interp_pair_kernel<4, 2>), grid, blk, 0, s, jobs, tiles); break;
+      default: hipLaunchKernelGGL((interp_pair_kernel<6, 2>), grid, blk, 0, s, jobs, tiles); break;
+    }
   } else if (cls >= kClsMerged) {
     switch (cls - kClsMerged) {
       case 0: hipLaunchKernelGGL((interp_all_kernel<2, 0>), grid, blk, 0, s, jobs, tiles); break;
@@ -468,6 +479,9 @@ tah_status tah_kernel_class_name(int32_t cls, char* buf, int32_t len) {
       name = "generic_kernel";
     } else if (cls > kClsCopyLaunch && cls <= kClsCopyLaunch + 16) {
       name = "copy_kernel<" + std::to_string(cls - kClsCopyLaunch) + ">";
+    } else if (cls >= kClsMergedPair && cls - kClsMergedPair < int(kInterpTable.size())) {
+      const auto& e = kInterpTable[cls - kClsMergedPair];
+      name = "interp_pair_kernel<" + std::to_string(e[0]) + ", " + std::to_string(e[1]) + ">";
     } else if (cls >= kClsMerged && cls - kClsMerged < int(kInterpTable.size())) {
       const auto& e = kInterpTable[cls - kClsMerged];
       name = "interp_all_kernel<" + std::to_string(e[0]) + ", " + std::to_string(e[1]) + ">";

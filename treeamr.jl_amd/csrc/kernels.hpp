@@ -14,6 +14,10 @@
 
 #include <hip/hip_runtime.h>
 
+#ifndef TAH_PAIR_WAVES
+#define TAH_PAIR_WAVES 5
+#endif
+
 #include "device.hpp"
 
 namespace tah {
@@ -266,23 +270,31 @@ __global__ void __launch_bounds__(kThreads) copy_loop_kernel(const Job* __restri
 // taps in the reference's order (m3, m2, m1 ascending); the weight product
 // of every (target, tap) is the class's table W, read in visit order through
 // uniform loads, identical for every lane.
-template <int P, int SHIFT, int R0, int R1, int R2>
+//
+// MODE 0: one variable per thread. MODE 1 and 2 (variables innermost, an
+// even count): two neighbouring variables per thread, which share every
+// weight and every address; MODE 2 loads and stores the two as one aligned
+// 16-byte pair, MODE 1 (an array not 16-byte aligned) one at a time.
+template <int P, int SHIFT, int R0, int R1, int R2, int MODE = 0>
 __device__ __forceinline__ void interp_body(const Job& job, const Tile& tile, const DevWClass& wc) {
   constexpr int U0 = P + (R0 == 2 ? SHIFT : 0);
   constexpr int U1 = P + (R1 == 2 ? SHIFT : 0);
   constexpr int U2 = P + (R2 == 2 ? SHIFT : 0);
+  constexpr int NV = MODE == 0 ? 1 : 2;
+  typedef double pair_t __attribute__((ext_vector_type(2)));
   const DevGroup& g = job.groups[wc.group];
   const int32_t t = int32_t(threadIdx.x);
-  if (t >= tile.nu * tile.nvt) return;
+  const int32_t lanes = MODE == 0 ? tile.nvt : tile.nvt / 2;  // threads per unit
+  if (t >= tile.nu * lanes) return;
   int32_t ul, vl;
-  if (job.vfast) {
-    ul = t / tile.nvt;
-    vl = t - ul * tile.nvt;
+  if (MODE != 0 || job.vfast) {
+    ul = t / lanes;
+    vl = t - ul * lanes;
   } else {
     vl = t / tile.nu;
     ul = t - vl * tile.nu;
   }
-  const int32_t v = tile.v0 + vl;
+  const int32_t v = tile.v0 + NV * vl;
   // The tile's units run on across consecutive transfers of the group.
   int32_t u = tile.e0 + ul, tr = tile.transfer;
   const int32_t nun = wc.nunits;
@@ -302,13 +314,15 @@ __device__ __forceinline__ void interp_body(const Job& job, const Tile& tile, co
   // The table is the same for every lane: read it through the constant
   // address space, so that it arrives by scalar loads into SGPRs.
   uint64_t wbase = reinterpret_cast<uint64_t>(uniform(wc.W));
-  double acc[R0][R1][R2];
+  double acc[NV][R0][R1][R2];
 #pragma unroll
-  for (int a = 0; a < R0; ++a)
+  for (int n = 0; n < NV; ++n)
 #pragma unroll
-    for (int b = 0; b < R1; ++b)
+    for (int a = 0; a < R0; ++a)
 #pragma unroll
-      for (int c = 0; c < R2; ++c) acc[a][b][c] = 0.0;
+      for (int b = 0; b < R1; ++b)
+#pragma unroll
+        for (int c = 0; c < R2; ++c) acc[n][a][b][c] = 0.0;
   int idx = 0;
 #pragma unroll
   for (int m2 = 0; m2 < U2; ++m2) {
@@ -321,11 +335,20 @@ __device__ __forceinline__ void interp_body(const Job& job, const Tile& tile, co
       const __attribute__((address_space(4))) double* __restrict__ W =
           (const __attribute__((address_space(4))) double*)(wbase) + idx;
       int widx = 0;
-      double row[U0];
+      double row[NV][U0];
       const __attribute__((address_space(1))) double* __restrict__ p =
           (const __attribute__((address_space(1))) double*)(src + (s2 * m2 + s1 * m1));
 #pragma unroll
-      for (int m0 = 0; m0 < U0; ++m0) row[m0] = p[s0 * m0];
+      for (int m0 = 0; m0 < U0; ++m0) {
+        if (MODE == 2) {
+          const pair_t x = *(const __attribute__((address_space(1))) pair_t*)(p + s0 * m0);
+          row[0][m0] = x.x;
+          row[NV - 1][m0] = x.y;
+        } else {
+          row[0][m0] = p[s0 * m0];
+          if (MODE == 1) row[NV - 1][m0] = p[s0 * m0 + 1];
+        }
+      }
 #pragma unroll
       for (int c = 0; c < R2; ++c) {
         const int k2 = m2 - c * SHIFT;
@@ -340,7 +363,9 @@ __device__ __forceinline__ void interp_body(const Job& job, const Tile& tile, co
             for (int m0 = 0; m0 < U0; ++m0) {
               const int k0 = m0 - a * SHIFT;
               if (k0 < 0 || k0 >= P) continue;
-              acc[a][b][c] = __dadd_rn(acc[a][b][c], __dmul_rn(W[widx], row[m0]));
+              const double w = W[widx];
+#pragma unroll
+              for (int n = 0; n < NV; ++n) acc[n][a][b][c] = __dadd_rn(acc[n][a][b][c], __dmul_rn(w, row[n][m0]));
               ++widx;
             }
           }
@@ -351,11 +376,13 @@ __device__ __forceinline__ void interp_body(const Job& job, const Tile& tile, co
       // the compiler defers whole chains to the end and keeps every row's
       // source values and weights live until then, spilling them.
 #pragma unroll
-      for (int a = 0; a < R0; ++a)
+      for (int n = 0; n < NV; ++n)
 #pragma unroll
-        for (int b = 0; b < R1; ++b)
+        for (int a = 0; a < R0; ++a)
 #pragma unroll
-          for (int c = 0; c < R2; ++c) asm volatile("" : "+v"(acc[a][b][c]));
+          for (int b = 0; b < R1; ++b)
+#pragma unroll
+            for (int c = 0; c < R2; ++c) asm volatile("" : "+v"(acc[n][a][b][c]));
     }
   }
   double* __restrict__ dst = job.work + s4 * g.tblocks[tr] + s3 * v;
@@ -365,9 +392,19 @@ __device__ __forceinline__ void interp_body(const Job& job, const Tile& tile, co
 #pragma unroll
     for (int b = 0; b < R1; ++b)
 #pragma unroll
-      for (int a = 0; a < R0; ++a)
-        dst[s0 * (g.tfirst[0] + ra[a]) + s1 * (g.tfirst[1] + rb[b]) + s2 * (g.tfirst[2] + rc[c])] =
-            scaled(acc[a][b][c], job.factors, g.factorcol, nvars, v);
+      for (int a = 0; a < R0; ++a) {
+        double* q = dst + (s0 * (g.tfirst[0] + ra[a]) + s1 * (g.tfirst[1] + rb[b]) + s2 * (g.tfirst[2] + rc[c]));
+        const double x = scaled(acc[0][a][b][c], job.factors, g.factorcol, nvars, v);
+        if (MODE == 2) {
+          pair_t out;
+          out.x = x;
+          out.y = scaled(acc[NV - 1][a][b][c], job.factors, g.factorcol, nvars, v + 1);
+          *reinterpret_cast<pair_t*>(q) = out;
+        } else {
+          q[0] = x;
+          if (MODE == 1) q[1] = scaled(acc[NV - 1][a][b][c], job.factors, g.factorcol, nvars, v + 1);
+        }
+      }
 }
 
 #ifdef TAH_DEFINE_KERNELS
@@ -402,6 +439,35 @@ interp_all_kernel(const Job* __restrict__ jobs, const Tile* __restrict__ tiles) 
     case 6: interp_body<P, SHIFT, 1, 2, 2>(job, tile, wc); break;
     default: interp_body<P, SHIFT, 2, 2, 2>(job, tile, wc); break;
   }
+}
+
+// The merged launch over tiles of variable pairs (variables innermost, an
+// even count): 16-byte loads and stores where the arrays are 16-byte
+// aligned. A pair halves the addresses the vector memory unit processes per
+// value, the unit that bounds the single-variable kernel, and each weight
+// read serves both variables. Registers capped for five waves per SIMD.
+template <int MODE, int P, int SHIFT>
+__device__ __forceinline__ void interp_pair_dispatch(const Job& job, const Tile& tile, const DevWClass& wc) {
+  switch (uniform(wc.rbits)) {
+    case 0: interp_body<P, SHIFT, 1, 1, 1, MODE>(job, tile, wc); break;
+    case 1: interp_body<P, SHIFT, 2, 1, 1, MODE>(job, tile, wc); break;
+    case 2: interp_body<P, SHIFT, 1, 2, 1, MODE>(job, tile, wc); break;
+    case 3: interp_body<P, SHIFT, 2, 2, 1, MODE>(job, tile, wc); break;
+    case 4: interp_body<P, SHIFT, 1, 1, 2, MODE>(job, tile, wc); break;
+    case 5: interp_body<P, SHIFT, 2, 1, 2, MODE>(job, tile, wc); break;
+    case 6: interp_body<P, SHIFT, 1, 2, 2, MODE>(job, tile, wc); break;
+    default: interp_body<P, SHIFT, 2, 2, 2, MODE>(job, tile, wc); break;
+  }
+}
+
+template <int P, int SHIFT>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(TAH_PAIR_WAVES, 8)))
+interp_pair_kernel(const Job* __restrict__ jobs, const Tile* __restrict__ tiles) {
+  const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  const Job& job = jobs[tile.job];
+  const DevWClass& wc = job.wclasses[tile.item];
+  if (job.pairs) interp_pair_dispatch<2, P, SHIFT>(job, tile, wc);
+  else interp_pair_dispatch<1, P, SHIFT>(job, tile, wc);
 }
 
 #endif  // TAH_DEFINE_KERNELS

@@ -332,8 +332,17 @@ void build_launches(tah_plan* p) {
   p->launches.clear();
   // The launch an interpolation class runs in: its own, or with every
   // class of its order and shift (tah_plan::merge).
+  // Merged prolongations take tiles of variable pairs where the layout
+  // allows: variables innermost, an even count, every other stride even.
+  // (Restriction over pairs measured slower than over single variables.)
+  const int64_t* st = p->layout.stride;
+  const bool pair_layout = p->interp_pairs && st[3] == 1 && nvars % 2 == 0 && st[0] % 2 == 0 &&
+                           st[1] % 2 == 0 && st[2] % 2 == 0 && st[4] % 2 == 0;
   auto launch_of = [&](int cls) {
-    return p->merge && cls >= kClsInterp && cls < kClsMerged ? kClsMerged + (cls - kClsInterp) / 8 : cls;
+    if (!p->merge || cls < kClsInterp || cls >= kClsMerged) return cls;
+    const int index = (cls - kClsInterp) / 8;
+    const bool pair = pair_layout && kInterpTable[size_t(index)][1] == 0;
+    return (pair ? kClsMergedPair : kClsMerged) + index;
   };
   // The launch a copy or generic group runs in.
   auto group_launch = [&](int cls) { return cls == kClsCopy ? copy_launch(p->copy_per_thread) : cls; };
@@ -379,8 +388,10 @@ void build_launches(tah_plan* p) {
             const int32_t nun = c.nunits();
             int32_t nu, nvt;
             if (vfast) {
-              nvt = std::min(nvars, kThreads);
-              nu = std::max(1, kThreads / nvt);
+              // a pair tile's thread holds two variables
+              const int32_t per_wg = cls >= kClsMergedPair ? 2 * kThreads : kThreads;
+              nvt = std::min(nvars, per_wg);
+              nu = std::max(1, per_wg / nvt);
             } else {
               // units fastest: as many as the class has per transfer, the
               // rest of the workgroup over variables

@@ -39,6 +39,7 @@ constexpr int kClsCopyLaunch = 900;
 inline int copy_launch(int cpt) { return kClsCopyLaunch + cpt; }
 constexpr int kClsInterp = 2;  // + 8*(index into kInterpTable) + rbits
 constexpr int kClsMerged = 1000;  // + index into kInterpTable: every rbits in one launch
+constexpr int kClsMergedPair = 1100;  // + index: the same over tiles of variable pairs
 
 // (P, SHIFT) of each specialised interpolation kernel family.
 extern const std::array<std::array<int, 2>, 8> kInterpTable;
@@ -122,6 +123,10 @@ struct tah_plan {
   // Workgroups of the copies' bounded grid (a multiple of 8), or 0 (the
   // default) for one workgroup per tile.
   int32_t copy_wgs = 0;
+  // Merged interpolation launches over tiles of two neighbouring variables
+  // per thread where the layout has variables innermost and an even count
+  // (1, the default), or of one variable per thread (0).
+  int32_t interp_pairs = 1;
   std::vector<hipEvent_t> events;  // fork/join events, created on first use
   // The adjoint's inverted schedule, built on first use (adjoint.hip).
   tah::AdjointTables* adj = nullptr;

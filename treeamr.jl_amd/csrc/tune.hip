@@ -1,11 +1,12 @@
 // Plan options and the autotuner.
 //
-// A plan's launches have five knobs: stage groups forked over the
+// A plan's launches have six knobs: stage groups forked over the
 // context's streams or run in order on one; the unit classes of one
 // interpolation order in one launch or a launch each; phase-1 transfers
 // deferred into the last stage group or not; the copy kernel's tile (4, 8 or
 // 16 values per thread); the copies from one workgroup per tile or from a
-// bounded grid that leaves wave slots to the interpolations beside them. tah_tune times every
+// bounded grid that leaves wave slots to the interpolations beside them;
+// merged interpolations over variable pairs or single variables. tah_tune times every
 // combination on the caller's working array and keeps the fastest, cached
 // in a text file under a signature of the mesh, the layout and the device,
 // so that a second run over the same mesh skips the timing. The reference
@@ -80,13 +81,15 @@ void set_options(tah_plan* p, const tah_plan_options& o) {
          kUnsupported);
   const int32_t wgs = o.copy_wgs;
   if (wgs < 0 || wgs % 8 != 0) fail("copy_wgs is 0 or a positive multiple of 8");
+  if (o.interp_pairs < 0 || o.interp_pairs > 1) fail("plan option interp_pairs is 0 or 1");
   const bool rebuild = o.merge != p->merge || o.defer != p->defer || o.copy_per_thread != p->copy_per_thread ||
-                       wgs != p->copy_wgs;
+                       wgs != p->copy_wgs || o.interp_pairs != p->interp_pairs;
   p->fork = o.fork;
   p->merge = o.merge;
   p->defer = o.defer;
   p->copy_per_thread = o.copy_per_thread;
   p->copy_wgs = wgs;
+  p->interp_pairs = o.interp_pairs;
   if (rebuild) build_launches(p);
 }
 
@@ -98,9 +101,9 @@ bool lookup(const std::string& path, const std::string& key, tah_plan_options& o
   while (std::getline(in, line)) {
     std::istringstream ss(line);
     std::string tag, k;
-    int f, m, d, c, w;
-    if (!(ss >> tag >> k >> f >> m >> d >> c >> w) || tag != "tah-tune-2" || k != key) continue;
-    o = tah_plan_options{f, m, d, c, 0, w, {0, 0}};
+    int f, m, d, c, w, ip;
+    if (!(ss >> tag >> k >> f >> m >> d >> c >> w >> ip) || tag != "tah-tune-3" || k != key) continue;
+    o = tah_plan_options{f, m, d, c, 0, w, ip, {0}};
     found = true;  // the last entry wins
   }
   return found;
@@ -108,17 +111,19 @@ bool lookup(const std::string& path, const std::string& key, tah_plan_options& o
 
 // The candidates: every combination, the bounded copy grid only beside
 // forked interpolations (it has nothing to leave its wave slots to
-// otherwise).
+// otherwise), variable pairs only for merged launches (the only ones that
+// take them).
 std::vector<tah_plan_options> candidates() {
   std::vector<tah_plan_options> out;
   for (int f = 0; f <= 1; ++f)
     for (int m = 0; m <= 1; ++m)
-      for (int d = 0; d <= 1; ++d)
-        for (int c : {4, 8, 16})
-          for (int w : {0, 512, 1024, 2048}) {
-            if (w && !f) continue;
-            out.push_back(tah_plan_options{f, m, d, c, 0, w, {0, 0}});
-          }
+      for (int ip = m; ip >= 0; --ip)
+        for (int d = 0; d <= 1; ++d)
+          for (int c : {4, 8, 16})
+            for (int w : {0, 512, 1024, 2048}) {
+              if (w && !f) continue;
+              out.push_back(tah_plan_options{f, m, d, c, 0, w, m ? ip : 1, {0}});
+            }
   return out;
 }
 
@@ -135,7 +140,8 @@ tah_status tah_plan_get_options(const tah_plan* p, tah_plan_options* out) {
     out->copy_per_thread = p->copy_per_thread;
     out->fuse = 0;
     out->copy_wgs = p->copy_wgs;
-    out->reserved_[0] = out->reserved_[1] = 0;
+    out->interp_pairs = p->interp_pairs;
+    out->reserved_[0] = 0;
   });
 }
 
@@ -167,7 +173,7 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
-    tah_plan_options best{p->fork, p->merge, p->defer, p->copy_per_thread, 0, p->copy_wgs, {0, 0}};
+    tah_plan_options best{p->fork, p->merge, p->defer, p->copy_per_thread, 0, p->copy_wgs, p->interp_pairs, {0}};
     float best_t = 1e30f;
     auto run = [&]() {
       tah_status st = tah_fill(p, work, factors, s);
@@ -208,8 +214,8 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
     if (cache_path) {
       std::ofstream out(cache_path, std::ios::app);
       if (!out) fail(std::string("tah_tune: cannot write the cache file ") + cache_path);
-      out << "tah-tune-2 " << key << " " << best.fork << " " << best.merge << " " << best.defer << " "
-          << best.copy_per_thread << " " << best.copy_wgs << " " << best_t << "\n";
+      out << "tah-tune-3 " << key << " " << best.fork << " " << best.merge << " " << best.defer << " "
+          << best.copy_per_thread << " " << best.copy_wgs << " " << best.interp_pairs << " " << best_t << "\n";
     }
   });
 }
