@@ -79,9 +79,7 @@ void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, cons
       case 2: hipLaunchKernelGGL((interp_pair_kernel<4, 0>), grid, blk, 0, s, jobs, tiles); break;
       case 3: hipLaunchKernelGGL((interp_pair_kernel<5, 0>), grid, blk, 0, s, jobs, tiles); break;
       case 4: hipLaunchKernelGGL((interp_pair_kernel<6, 0>), grid, blk, 0, s, jobs, tiles); break;
-      case 5: hipLaunchKernelGGL((interp_pair_kernel<2, 2>), grid, blk, 0, s, jobs, tiles); break;
-      case 6: hipLaunchKernelGGL((interp_pair_kernel<4, 2>), grid, blk, 0, s, jobs, tiles); break;
-      default: hipLaunchKernelGGL((interp_pair_kernel<6, 2>), grid, blk, 0, s, jobs, tiles); break;
+      default: fail("no variable-pair kernel for restriction (class " + std::to_string(cls) + ")");
     }
   } else if (cls >= kClsMerged) {
     switch (cls - kClsMerged) {

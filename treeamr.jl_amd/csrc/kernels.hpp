@@ -14,13 +14,22 @@
 
 #include <hip/hip_runtime.h>
 
-#ifndef TAH_PAIR_WAVES
-#define TAH_PAIR_WAVES 5
-#endif
+
 
 #include "device.hpp"
 
 namespace tah {
+
+// Waves per SIMD the merged interpolation kernels are compiled for, and how
+// many rows ahead restriction loads its window. Measured on the large forest
+// at order 6: restriction, whose rows wait on HBM, takes 3.3 ms with its
+// loads three rows ahead at four waves against 3.7 ms with none at seven;
+// prolongation gained nothing from loads ahead at any depth, single (seven
+// waves) or over variable pairs (five).
+constexpr int kRestrictWaves = 4;
+constexpr int kRestrictRowsAhead = 3;
+constexpr int kProlongWaves = 7;
+constexpr int kPairWaves = 5;
 
 // A value every lane holds alike, moved into scalar registers, so that what
 // is loaded through it is loaded by the scalar unit.
@@ -323,6 +332,30 @@ __device__ __forceinline__ void interp_body(const Job& job, const Tile& tile, co
       for (int b = 0; b < R1; ++b)
 #pragma unroll
         for (int c = 0; c < R2; ++c) acc[n][a][b][c] = 0.0;
+  // One row of the window (m2, m1): U0 values of each variable.
+  auto load_row = [&](int m2, int m1, double (&row)[NV][U0]) {
+    const __attribute__((address_space(1))) double* __restrict__ p =
+        (const __attribute__((address_space(1))) double*)(src + (s2 * m2 + s1 * m1));
+#pragma unroll
+    for (int m0 = 0; m0 < U0; ++m0) {
+      if (MODE == 2) {
+        const pair_t x = *(const __attribute__((address_space(1))) pair_t*)(p + s0 * m0);
+        row[0][m0] = x.x;
+        row[NV - 1][m0] = x.y;
+      } else {
+        row[0][m0] = p[s0 * m0];
+        if (MODE == 1) row[NV - 1][m0] = p[s0 * m0 + 1];
+      }
+    }
+  };
+  // PF rows ahead (restriction): the loads of row r + PF are issued before
+  // row r's taps, into a ring of PF + 1 row buffers, so that each row's
+  // memory latency overlaps the rows before it within the wave.
+  constexpr int PF = SHIFT != 0 ? kRestrictRowsAhead : 0;
+  constexpr int NR = U1 * U2;
+  double rows[PF + 1][NV][U0];
+#pragma unroll
+  for (int q = 0; q < PF && q < NR; ++q) load_row(q / U1, q % U1, rows[q]);
   int idx = 0;
 #pragma unroll
   for (int m2 = 0; m2 < U2; ++m2) {
@@ -335,20 +368,9 @@ __device__ __forceinline__ void interp_body(const Job& job, const Tile& tile, co
       const __attribute__((address_space(4))) double* __restrict__ W =
           (const __attribute__((address_space(4))) double*)(wbase) + idx;
       int widx = 0;
-      double row[NV][U0];
-      const __attribute__((address_space(1))) double* __restrict__ p =
-          (const __attribute__((address_space(1))) double*)(src + (s2 * m2 + s1 * m1));
-#pragma unroll
-      for (int m0 = 0; m0 < U0; ++m0) {
-        if (MODE == 2) {
-          const pair_t x = *(const __attribute__((address_space(1))) pair_t*)(p + s0 * m0);
-          row[0][m0] = x.x;
-          row[NV - 1][m0] = x.y;
-        } else {
-          row[0][m0] = p[s0 * m0];
-          if (MODE == 1) row[NV - 1][m0] = p[s0 * m0 + 1];
-        }
-      }
+      const int r = m2 * U1 + m1;
+      if (r + PF < NR) load_row((r + PF) / U1, (r + PF) % U1, rows[(r + PF) % (PF + 1)]);
+      double(&row)[NV][U0] = rows[r % (PF + 1)];
 #pragma unroll
       for (int c = 0; c < R2; ++c) {
         const int k2 = m2 - c * SHIFT;
@@ -421,10 +443,12 @@ __global__ void __launch_bounds__(kThreads) interp_kernel(const Job* __restrict_
 // per tile (a branch uniform over the workgroup), so that the tiles of all
 // classes of one transfer -- which read the same source windows -- can run
 // side by side and meet in one L2.
-// Registers capped for seven waves per SIMD (72 VGPRs: no spills at order 6),
-// where the unconstrained octet body takes 94 and leaves room for five.
+// Prolongation's registers are capped for seven waves per SIMD (72 VGPRs:
+// no spills at order 6, where the unconstrained octet body takes 94);
+// restriction's for four (its ring of row buffers: 115 VGPRs at order 6,
+// no spills).
 template <int P, int SHIFT>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(7, 8)))
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(SHIFT != 0 ? kRestrictWaves : kProlongWaves, 8)))
 interp_all_kernel(const Job* __restrict__ jobs, const Tile* __restrict__ tiles) {
   const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const Job& job = jobs[tile.job];
@@ -446,6 +470,7 @@ interp_all_kernel(const Job* __restrict__ jobs, const Tile* __restrict__ tiles) 
 // aligned. A pair halves the addresses the vector memory unit processes per
 // value, the unit that bounds the single-variable kernel, and each weight
 // read serves both variables. Registers capped for five waves per SIMD.
+// Prolongation only (SHIFT 0): over pairs, restriction was slower.
 template <int MODE, int P, int SHIFT>
 __device__ __forceinline__ void interp_pair_dispatch(const Job& job, const Tile& tile, const DevWClass& wc) {
   switch (uniform(wc.rbits)) {
@@ -461,8 +486,9 @@ __device__ __forceinline__ void interp_pair_dispatch(const Job& job, const Tile&
 }
 
 template <int P, int SHIFT>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(TAH_PAIR_WAVES, 8)))
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kPairWaves, 8)))
 interp_pair_kernel(const Job* __restrict__ jobs, const Tile* __restrict__ tiles) {
+  static_assert(SHIFT == 0, "variable pairs are for prolongation");
   const Tile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
   const Job& job = jobs[tile.job];
   const DevWClass& wc = job.wclasses[tile.item];
