@@ -120,6 +120,18 @@ def test_random_forests_bit_exact(seed):
     _check(fs, sched, "staged")
 
 
+@pytest.mark.parametrize("seed", [2, 3, 4, 5, 6, 12])
+def test_random_forests_arena_even_variables(seed):
+    """The same forests in the arena layout with an even variable count
+    (the set's own count rounded up), so that prolongation runs over
+    variable pairs, beside restriction with its rows loaded ahead."""
+    f, ops, G, nvars, centering = W.random_forest(seed, max_blocks=600)
+    nv = nvars[0] + nvars[0] % 2
+    fs, sched = _prepare(f, ops, dict(nvars=nv, G=G, centering=centering), layout="arena", seed=seed)
+    assert sched.plan(fs.layout).options["interp_pairs"] == 1
+    _check(fs, sched, "staged")
+
+
 @pytest.mark.parametrize("method", ["plain", "staged"])
 @pytest.mark.parametrize("layout", ["dense", "arena"])
 def test_config2_vertex_bit_exact(method, layout):
