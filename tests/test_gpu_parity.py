@@ -111,6 +111,21 @@ def test_variable_pair_interpolation_every_order(family, orders, centering, offs
     _check(fs, sched, "staged")
 
 
+@pytest.mark.parametrize("nvars", [2, 514])
+def test_variable_pair_tiles_at_the_extremes(nvars):
+    """Pair tiles with one thread per unit (2 variables: 256 units a tile)
+    and with a variable count above one tile's 512 (514: a tile of 512
+    variables and one of 2), cell-centred, arena layout."""
+    from treeamr_hip import Forest, Operators
+    f = Forest((2, 2, 2), N=8, periodic=(True,) * 3)
+    f.refine([0])
+    f.balance()
+    ops = Operators(prolongation=4, restriction=4)
+    fs, sched = _prepare(f, ops, dict(nvars=nvars, G=2, centering="cell"), layout="arena", seed=nvars)
+    assert sched.plan(fs.layout).options["interp_pairs"] == 1
+    _check(fs, sched, "staged")
+
+
 @pytest.mark.parametrize("seed", [2, 3, 4, 5, 6, 12])
 def test_random_forests_bit_exact(seed):
     """Forests of the batched configuration, cell (seeds 2, 4, 5) and vertex
