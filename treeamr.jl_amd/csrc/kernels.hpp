@@ -14,8 +14,6 @@
 
 #include <hip/hip_runtime.h>
 
-
-
 #include "device.hpp"
 
 namespace tah {
