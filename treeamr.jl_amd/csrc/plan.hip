@@ -325,7 +325,8 @@ void build_launches(tah_plan* p) {
   }
 
   // Tiles: per stage group, per class; within a launch ordered by target
-  // block, so that the tiles of one block run close together in time.
+  // block (restriction: by source block), so that the tiles of one block run
+  // close together in time.
   const int32_t nvars = p->layout.nvars;
   const bool vfast = p->layout.stride[3] < p->layout.stride[0];
   p->tiles.clear();
@@ -411,7 +412,11 @@ void build_launches(tah_plan* p) {
                 for (int64_t i = 0; i < count; i += nu) {
                   const int32_t tt = t + int32_t(i / nun), uu = int32_t(i % nun);
                   const int32_t n = int32_t(std::min<int64_t>(nu, count - i));
-                  tl.push_back({g.tblocks[tt], Tile{0, ci, tt, uu, n, v0, std::min(nvt, nvars - v0), 0}});
+                  // restriction by source block: the several small
+                  // transfers that read one fine block (faces, edges,
+                  // corners) run together and meet in one L2
+                  const int32_t key = g.kind == 2 ? g.sblocks[tt] : g.tblocks[tt];
+                  tl.push_back({key, Tile{0, ci, tt, uu, n, v0, std::min(nvt, nvars - v0), 0}});
                   values += int64_t(n) * std::min(nvt, nvars - v0) * per_unit;
                 }
               t = t1;
