@@ -99,6 +99,7 @@ struct alignas(8) AdjTile {
 constexpr int kAdjPerThread = 4;
 constexpr int kAdjTile = kThreads * kAdjPerThread;
 constexpr int kPassRows = 8;              // output rows of one adjoint pass tile
+constexpr int kPassRowsAhead = 4;         // input rows loaded ahead of their taps
 constexpr int kPassTile = 2 * kThreads;   // and its plane values
 
 struct AdjStage {
@@ -191,9 +192,10 @@ __device__ __forceinline__ void pass_tile(const AdjPass& it, const Job& job, con
   for (int k = 0; k < kPassRows; ++k) a0[k] = a1[k] = 0.0;
   // Input row rr reaches output row x0 + k with tap m = mx0 + k - ss[rr].
   const int32_t mx0 = x0 + uniform(T.lo[D]);
-#pragma unroll 2
-  for (int32_t rr = rlo; rr <= rhi; ++rr) {
-    double u0, u1;
+  // Input rows are loaded kPassRowsAhead rows ahead of their taps, into a
+  // ring (one slot per row of a group of kPassRowsAhead), so that a row's
+  // load latency overlaps the rows before it.
+  auto load = [&](int32_t rr, double& u0, double& u1) {
     if constexpr (PAIR) {
       const pair_t uu = global(reinterpret_cast<const pair_t*>(in))[(b[0] + isd * rr) >> 1];
       u0 = uu.x;
@@ -202,15 +204,30 @@ __device__ __forceinline__ void pass_tile(const AdjPass& it, const Job& job, con
       u0 = global(in)[b[0] + isd * rr];
       u1 = global(in)[b[1] + isd * rr];
     }
-    const int32_t m0 = mx0 - ss[rr];
-    const int32_t wr = order * rr;
+  };
+  double ring0[kPassRowsAhead], ring1[kPassRowsAhead];
+  if (rlo <= rhi) {  // (no input rows: rlo is INT32_MAX)
 #pragma unroll
-    for (int k = 0; k < kPassRows; ++k) {
-      const int32_t m = m0 + k;
-      if (k < nr && m >= 0 && m < order) {
-        const double ww = w[wr + m];
-        a0[k] = __dadd_rn(a0[k], __dmul_rn(ww, u0));
-        a1[k] = __dadd_rn(a1[k], __dmul_rn(ww, u1));
+    for (int q = 0; q < kPassRowsAhead; ++q)
+      if (q <= rhi - rlo) load(rlo + q, ring0[q], ring1[q]);
+  }
+  for (int32_t r0 = rlo; r0 <= rhi; r0 += kPassRowsAhead) {
+#pragma unroll
+    for (int q = 0; q < kPassRowsAhead; ++q) {
+      const int32_t rr = r0 + q;
+      if (rr > rhi) break;
+      const double u0 = ring0[q], u1 = ring1[q];
+      if (kPassRowsAhead <= rhi - rr) load(rr + kPassRowsAhead, ring0[q], ring1[q]);
+      const int32_t m0 = mx0 - ss[rr];
+      const int32_t wr = order * rr;
+#pragma unroll
+      for (int k = 0; k < kPassRows; ++k) {
+        const int32_t m = m0 + k;
+        if (k < nr && m >= 0 && m < order) {
+          const double ww = w[wr + m];
+          a0[k] = __dadd_rn(a0[k], __dmul_rn(ww, u0));
+          a1[k] = __dadd_rn(a1[k], __dmul_rn(ww, u1));
+        }
       }
     }
   }
