@@ -87,6 +87,31 @@ def cpu_baseline(sched_builder, seconds=12.0):
                        f"{reps} fills, {dt * 1e3:.1f} ms per fill, OpenMP threads pinned by block owner")
 
 
+def device_copy_rate(gib=2.0, reps=10):
+    """GB/s (bytes read plus bytes written) of torch's device-to-device copy of a
+    gib-GiB buffer on this GPU, median of reps: what a plain streaming copy
+    reaches here, beside the fill's copy kernel."""
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(2):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        y.copy_(x)
+        b.record()
+        b.synchronize()
+        ts.append(a.elapsed_time(b))
+    ms = sorted(ts)[len(ts) // 2]
+    del x, y
+    return round(2 * n * 8 / (ms * 1e-3) / 1e9, 1)
+
+
 def traffic_of(kernel, rec, layout, nvars):
     """HBM bytes per launch of `kernel` from the committed PMC summary of this
     workload (profiles/hbm_traffic_large_forest_arena.json: FETCH_SIZE and WRITE_SIZE per dispatch,
@@ -448,6 +473,9 @@ def main():
     def r(d):
         return {k: round(v, 4) for k, v in d.items()} if d else None
 
+    copy_ref = device_copy_rate()
+    log(f"[bench] device copy of 2 GiB: {copy_ref} GB/s")
+
     out = {
         "metric": METRIC,
         "value": round(gbs, 1),
@@ -496,6 +524,8 @@ def main():
             "traffic": traffic_of(dominant, dk, args.layout, nvars),
             "algorithmic_bytes_per_launch": dk["bytes_per_launch"],
             "avg_launch_ms": dk["avg_launch_ms"],
+            "device_copy_GBs": copy_ref,
+            "frac_of_device_copy": round(dk["GBs"] / copy_ref, 4),
         },
         "kernels": kernels,
         "cpu_baseline": cpu,
