@@ -176,11 +176,17 @@ def main():
     f, ops, kw = builder(args.nvars)
     kw = dict(kw)
     nvars = kw.pop("nvars")
-    sched_t0 = time.perf_counter()
     fs = FieldSet(f, nvars, device="cuda", layout=args.layout, fill=0.0, **kw)
     W.fill_gaussians(fs, 5)
+    torch.cuda.synchronize()
+    # the host schedule alone, then its device plan (classes, tiles, upload)
+    sched_t0 = time.perf_counter()
     sched = GhostSchedule(fs, ops)
     build_s = time.perf_counter() - sched_t0
+    plan_t0 = time.perf_counter()
+    sched.plan(fs.layout)
+    torch.cuda.synchronize()
+    plan_s = time.perf_counter() - plan_t0
     values = sched.points() * nvars
     bytes_eff = 16.0 * values
     log(f"[bench] {f.nleaves} blocks, {values} values/fill, {bytes_eff / 1e9:.2f} GB effective; "
@@ -514,6 +520,7 @@ def main():
         "regrid_transfer": regrid_rec,
         "unify": unify_rec,
         "schedule_build_s": round(build_s, 3),
+        "plan_build_s": round(plan_s, 3),
         "roofline": {
             "kernel": dominant,
             "bound": "hbm",
