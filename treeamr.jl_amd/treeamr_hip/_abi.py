@@ -47,13 +47,13 @@ P = C.c_void_p
 I32P = C.POINTER(C.c_int32)
 DP = C.POINTER(C.c_double)
 
-# (name, argtypes) for every entry point of include/tah.h.
 class tah_plan_options(C.Structure):
     _fields_ = [("fork", C.c_int32), ("merge", C.c_int32), ("defer", C.c_int32),
                 ("copy_per_thread", C.c_int32), ("fuse", C.c_int32), ("copy_wgs", C.c_int32),
                 ("interp_pairs", C.c_int32), ("reserved_", C.c_int32 * 1)]
 
 
+# (name, argtypes) for every entry point of include/tah.h.
 SIGNATURES = {
     "tah_lagrange_weights": [C.c_int32, C.c_int64, C.c_int64, DP],
     "tah_conservative_weights": [C.c_int32, DP, DP],
