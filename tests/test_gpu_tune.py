@@ -79,6 +79,19 @@ def test_fused_launch_refused():
     assert plan.options["fuse"] == 0
 
 
+def test_option_values_refused():
+    """Out-of-range options are refused on the host, before any launch, and
+    leave the plan's options as they were."""
+    fs, s = _case("cell-even")
+    plan = s.plan(fs.layout)
+    before = plan.options
+    for bad in (dict(interp_pairs=2), dict(interp_pairs=-1), dict(copy_per_thread=5), dict(copy_wgs=12),
+                dict(fork=2)):
+        with pytest.raises(ValueError):
+            plan.set_options(**bad)
+        assert plan.options == before, bad
+
+
 @pytest.mark.skip(reason="the fused copy-and-interpolation launch faulted the GPU (illegal address) in a timing "
                          "run on the large forest; its cause was not found, so the launch is switched off and "
                          "this case is not run")
