@@ -147,6 +147,26 @@ def test_random_forests_arena_even_variables(seed):
     _check(fs, sched, "staged")
 
 
+def test_forty_more_random_forests_both_layouts():
+    """Seeds 20..59 of the batched configuration (up to 500 blocks each), in
+    the arena layout with an even variable count and in the dense layout with
+    the drawn count: 80 fills, every value against the oracle."""
+    failed = []
+    for seed in range(20, 60):
+        f, ops, G, nvars, centering = W.random_forest(seed, max_blocks=500)
+        for layout in ("arena", "dense"):
+            nv = nvars[0] + (nvars[0] % 2 if layout == "arena" else 0)
+            print(f"[parity] seed={seed} {layout} {centering} G={G} nvars={nv} blocks={f.nleaves}", flush=True)
+            fs, sched = _prepare(f, ops, dict(nvars=nv, G=G, centering=centering), layout=layout, seed=seed)
+            expected = _oracle_result(fs, sched)
+            fill_ghosts(fs, sched)
+            torch.cuda.synchronize()
+            mism = int((_bits(fs.work) != expected.view(torch.int64)).sum())
+            if mism:
+                failed.append((seed, layout, mism))
+    assert not failed, failed
+
+
 @pytest.mark.parametrize("method", ["plain", "staged"])
 @pytest.mark.parametrize("layout", ["dense", "arena"])
 def test_config2_vertex_bit_exact(method, layout):
