@@ -52,6 +52,36 @@ def test_dot_product_identity(which, layout):
     assert abs(lhs - rhs) <= 1e-13 * max(abs(lhs), 1.0), (lhs, rhs)
 
 
+def test_dot_product_identity_random_forests():
+    """The identity on twelve forests of the batched configuration (seeds
+    20..31, cell and vertex, G 2 or 3, PointValue of order 2G), each in the
+    dense layout and in the arena layout with an even variable count."""
+    failed = []
+    for seed in range(20, 32):
+        f, ops, G, nvars, centering = W.random_forest(seed, max_blocks=400)
+        for layout in ("dense", "arena"):
+            nv = nvars[0] + (nvars[0] % 2 if layout == "arena" else 0)
+            print(f"[adjoint] seed={seed} {layout} {centering} G={G} nvars={nv} blocks={f.nleaves}", flush=True)
+            fs = FieldSet(f, nv, G=G, centering=centering, device="cuda", layout=layout, fill=0.0)
+            s = GhostSchedule(fs, ops)
+            g = torch.Generator(device="cuda")
+            g.manual_seed(seed)
+            own = AD.own_mask(fs)
+            x = torch.where(own, torch.rand(fs.work.shape, generator=g, device="cuda", dtype=torch.float64) * 2 - 1,
+                            torch.zeros_like(fs.work))
+            y = torch.rand(fs.work.shape, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+            fx = x.clone()
+            AD._fill_tensor(fs, s, fx)
+            aty = y.clone()
+            fill_ghosts_adjoint(fs, s, aty)
+            torch.cuda.synchronize()
+            lhs = float(torch.dot(fx, y))
+            rhs = float(torch.dot(x, aty))
+            if abs(lhs - rhs) > 1e-13 * max(abs(lhs), 1.0):
+                failed.append((seed, layout, lhs, rhs))
+    assert not failed, failed
+
+
 @pytest.mark.parametrize("which,layout,offset", [("config1", "dense", 0), ("reflecting", "dense", 0),
                                                  ("vertex", "dense", 0), ("config1", "arena", 0),
                                                  ("config1", "arena", 1), ("vertex", "arena", 0),
