@@ -101,6 +101,44 @@ def test_regrid_end_to_end_then_fill(centering):
     assert f.leaves != keys
 
 
+def test_transfers_on_random_forests():
+    """Twelve forests of the batched configuration (seeds 20..31): the leaves
+    of the finest level coarsened where whole families allow, a few coarse
+    leaves refined, balanced; the transfer in the dense layout and in the
+    arena layout with an even variable count, bit for bit against the
+    oracle's."""
+    import numpy as np
+    failed = []
+    for seed in range(20, 32):
+        f, ops, G, nvars, centering = W.random_forest(seed, max_blocks=300)
+        rng = np.random.default_rng(seed)
+        for layout in ("dense", "arena"):
+            g2, _, _, _, _ = W.random_forest(seed, max_blocks=300)   # a fresh copy of the forest
+            nv = nvars[0] + (nvars[0] % 2 if layout == "arena" else 0)
+            a = FieldSet(g2, nv, G=G, centering=centering, device="cuda", layout=layout, fill=0.0)
+            gen = torch.Generator(device="cuda")
+            gen.manual_seed(seed)
+            a.work.copy_(torch.rand(a.work.shape, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1)
+            lv = np.array([k.level for k in g2.leaves])
+            fine = [int(i) for i in np.nonzero(lv == lv.max())[0]] if lv.max() > 0 else []
+            coarse = [int(i) for i in np.nonzero(lv == lv.min())[0]]
+            refine = [int(i) for i in rng.choice(coarse, size=min(3, len(coarse)), replace=False)]
+            refine = [i for i in refine if i not in fine]
+            old = _move(g2, refine=refine, coarsen=fine)
+            print(f"[regrid] seed={seed} {layout} {centering} G={G} nvars={nv} {len(old[0])} -> {g2.nleaves} "
+                  f"blocks", flush=True)
+            rs = RegridSchedule(g2, old, ops, G=G, centering=centering, nvars=nv)
+            b = FieldSet(g2, nv, G=G, centering=centering, device="cuda", layout=layout, fill=0.0)
+            new, _ = regrid_transfer(b, a.layout, a.work, rs)
+            torch.cuda.synchronize()
+            host = np.zeros(b.nelements)
+            oracle.regrid(rs._descptr, b.layout, a.work.cpu().numpy().copy(), host)
+            mism = int((_bits(new) != torch.from_numpy(host).view(torch.int64)).sum())
+            if mism:
+                failed.append((seed, layout, mism))
+    assert not failed, failed
+
+
 def test_identity_transfer_at_size():
     """The reference's own regrid benchmark (bench/gpu.jl:159-174): old
     leaves = new leaves, the two-level mesh of 960 blocks of 32^3."""
