@@ -89,6 +89,58 @@ def test_unify_schedule_without_ghosts_face_centred():
     assert torch.equal(w, v)
 
 
+def test_unify_and_state_copies_on_random_forests():
+    """The vertex-centred forests among seeds 20..39 of the batched
+    configuration, dense and arena: unify! on the working array and on a
+    state vector, gather! and scatter!, each bit for bit against the
+    oracle."""
+    failed = []
+    done = 0
+    for seed in range(20, 40):
+        f, ops, G, nvars, centering = W.random_forest(seed, max_blocks=300)
+        if centering != "vertex":
+            continue
+        for layout in ("dense", "arena"):
+            nv = nvars[0]
+            print(f"[unify] seed={seed} {layout} G={G} nvars={nv} blocks={f.nleaves}", flush=True)
+            fs = FieldSet(f, nv, G=G, centering=centering, device="cuda", layout=layout, fill=0.0)
+            W.fill_uniform(fs, seed)
+            s = GhostSchedule(fs, ops)
+            # gather! against the oracle
+            v = torch.zeros(statelength(fs), device="cuda", dtype=torch.float64)
+            gather(v, fs)
+            want = np.zeros(statelength(fs))
+            oracle.gather(fs.layout, _own_first(fs), ownextents(fs), want, fs.work.cpu().numpy().copy())
+            torch.cuda.synchronize()
+            if not (_bits(v) == torch.from_numpy(want).view(torch.int64)).all():
+                failed.append((seed, layout, "gather"))
+            # unify! on the state vector
+            host_v = v.cpu().numpy().copy()
+            oracle.fill(unify_desc(s, state=True)._descptr, _state_layout(unify_desc(s, state=True), nv), host_v,
+                        fs.factors.cpu().numpy() if fs.factors is not None else None)
+            unify(v, fs, s)
+            torch.cuda.synchronize()
+            if not (_bits(v) == torch.from_numpy(host_v).view(torch.int64)).all():
+                failed.append((seed, layout, "unify state"))
+            # scatter! of the unified state, against the oracle
+            host = fs.work.cpu().numpy().copy()
+            oracle.scatter(fs.layout, _own_first(fs), ownextents(fs), host, v.cpu().numpy().copy())
+            scatter(fs, v)
+            torch.cuda.synchronize()
+            if not (_bits(fs.work) == torch.from_numpy(host).view(torch.int64)).all():
+                failed.append((seed, layout, "scatter"))
+            # unify! on the working array
+            host = fs.work.cpu().numpy().copy()
+            oracle.fill(unify_desc(s)._descptr, fs.layout, host,
+                        fs.factors.cpu().numpy() if fs.factors is not None else None)
+            unify(fs, s)
+            torch.cuda.synchronize()
+            if not (_bits(fs.work) == torch.from_numpy(host).view(torch.int64)).all():
+                failed.append((seed, layout, "unify array"))
+            done += 1
+    assert done >= 4 and not failed, (done, failed)
+
+
 def test_unify_refusals():
     f = Forest((2, 2, 2), N=8, periodic=(True,) * 3)
     f.refine([0])
