@@ -308,6 +308,7 @@ tah_status tah_graph_destroy(tah_graph* g);
  * in a fixed order (no float atomics), then the targets zeroed. */
 tah_status tah_fill_adjoint(tah_plan* p, double* adj, const double* factors, void* stream);
 
+
 /* A plan's launch options (no counterpart in the reference, whose launches
  * are fixed): fork runs a stage group's kernel classes on four forked
  * streams (1) or in order on the fill's stream (0, default); merge runs

@@ -57,3 +57,64 @@ def test_graph_replays_equal_single_fills():
         torch.cuda.synchronize()
         for (fs, _), r in zip(pairs, ref):
             assert torch.equal(fs.work.view(torch.int64), r.view(torch.int64)), f"replay {rep}"
+
+
+def test_batched_after_schedules_rebuilt():
+    """A batched fill, then its schedules dropped and others built on the same
+    field sets (other operators; new plans may take the old plans' addresses),
+    then a batched fill again: the cached batch of the old plans must not run.
+    Also a change of launch options between two batched fills."""
+    import gc
+
+    from treeamr_hip import Operators
+    pairs = _forests(range(2, 6), max_blocks=200)
+    init = [fs.work.clone() for fs, _ in pairs]  # NaN ghosts
+    fill_ghosts_batched(pairs)
+    torch.cuda.synchronize()
+    sets = [fs for fs, _ in pairs]
+    del pairs
+    gc.collect()
+    rebuilt = [(fs, GhostSchedule(fs, Operators(prolongation=2, restriction=2))) for fs in sets]
+    for fs, i0 in zip(sets, init):
+        fs.work.copy_(i0)
+    ref = _reference(rebuilt)
+    fill_ghosts_batched(rebuilt)
+    torch.cuda.synchronize()
+    for (fs, _), r in zip(rebuilt, ref):
+        assert torch.equal(fs.work.view(torch.int64), r.view(torch.int64)), "stale batch after a rebuild"
+    # options changed on one plan: the batch is built again from its new tiles
+    fs0, s0 = rebuilt[0]
+    s0.plan(fs0.layout).set_options(merge=0, copy_per_thread=4)
+    for fs, i0 in zip(sets, init):
+        fs.work.copy_(i0)
+    fill_ghosts_batched(rebuilt)
+    torch.cuda.synchronize()
+    for (fs, _), r in zip(rebuilt, ref):
+        assert torch.equal(fs.work.view(torch.int64), r.view(torch.int64)), "stale batch after set_options"
+
+
+def test_one_plan_two_streams():
+    """Two field sets over one forest and layout share one plan (and its
+    device Job). Filled back to back on two streams, each must see its own
+    array: the second call waits for the first before it rewrites the Job."""
+    from treeamr_hip import workloads as W2
+    f, ops, kw, _ = W2.config5(nvars=4, roots=4, N=16)
+    kw = dict(kw)
+    nv = kw.pop("nvars")
+    a = FieldSet(f, nv, device="cuda", layout="arena", fill=float("nan"), **kw)
+    b = FieldSet(f, nv, device="cuda", layout="arena", fill=float("nan"), **kw)
+    W2.fill_uniform(a, 1)
+    W2.fill_uniform(b, 2)
+    s = GhostSchedule(a, ops)
+    ra, rb = _reference([(a, s), (b, s)])
+    for rep in range(3):
+        sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+        sa.wait_stream(torch.cuda.current_stream())
+        sb.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(sa):
+            fill_ghosts(a, s)
+        with torch.cuda.stream(sb):
+            fill_ghosts(b, s)
+        torch.cuda.synchronize()
+        assert torch.equal(a.work.view(torch.int64), ra.view(torch.int64)), f"rep {rep}: first array"
+        assert torch.equal(b.work.view(torch.int64), rb.view(torch.int64)), f"rep {rep}: second array"

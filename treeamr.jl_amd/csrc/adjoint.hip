@@ -30,6 +30,8 @@
 #include <algorithm>
 #include <array>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "api_common.hpp"
@@ -505,7 +507,7 @@ void push_tiles(std::vector<AdjTile>& tiles, int32_t item, int64_t elems) {
   for (int64_t e = 0; e < elems; e += kAdjTile) tiles.push_back(AdjTile{item, int32_t(e)});
 }
 
-void build_adjoint(tah_plan* p) {
+void build_adjoint_tables(tah_plan* p) {
   auto* A = new AdjointTables();
   p->adj = A;
   std::vector<char> host;
@@ -729,7 +731,6 @@ void build_adjoint(tah_plan* p) {
   const int64_t off_zero = put(zeros.data(), int64_t(sizeof(Tile) * zeros.size()));
   const int64_t off_gt = put(nullptr, int64_t(sizeof(DevGroupT) * std::max<size_t>(1, gts.size())));
   HIPCHECK(hipMalloc(&A->dmem, host.size()));
-  if (scratch_max > 0) HIPCHECK(hipMalloc(&A->scratch, sizeof(double) * scratch_max));
   A->scratch_values = scratch_max;
   char* base = A->dmem;
   std::vector<DevGroupT> dg(gts.size());
@@ -754,17 +755,57 @@ void build_adjoint(tah_plan* p) {
   A->dzero = reinterpret_cast<Tile*>(base + off_zero);
 }
 
+// The inverted schedule, built once per plan; on failure nothing is kept.
+void build_adjoint(tah_plan* p) {
+  try {
+    build_adjoint_tables(p);
+  } catch (...) {
+    free_adjoint(p);
+    throw;
+  }
+}
+
+// staged: the passes' staging buffer, at least A.scratch_values doubles;
+// pass items address it from its start.
 template <int D>
-void launch_pass(tah_plan* p, const AdjStage& S, int q, hipStream_t s) {
+void launch_pass(tah_plan* p, const AdjStage& S, int q, hipStream_t s, double* staged) {
   const AdjointTables& A = *p->adj;
   const int64_t n = S.pass_tiles[q][1];
   if (n == 0) return;
-  // pass items address the scratch buffer from its start
-  const double* in = q == 0 ? nullptr : A.scratch;
-  double* out = A.scratch;
+  const double* in = q == 0 ? nullptr : staged;
   hipLaunchKernelGGL(adjoint_pass_kernel<D>, dim3(uint32_t(n)), dim3(kThreads), 0, s, p->djob, A.dgt, A.dpass,
-                     A.dptiles + S.pass_tiles[q][0], uint32_t(n), in, out);
+                     A.dptiles + S.pass_tiles[q][0], uint32_t(n), in, staged);
   HIPCHECK(hipGetLastError());
+}
+
+void check_adjoint_call(tah_plan* p, double* adj, const double* factors) {
+  if (!p || !adj) fail("null argument");
+  if (p->regrid) fail("the adjoint of a regrid transfer is not provided");
+  if (p->needs_factors && !factors)
+    fail("this schedule mirrors ghosts across a reflecting face, but no factor table was given");
+}
+
+// Every stage in reverse: passes, gather, zero. Each pass writes every
+// staged value the gather after it reads, so the buffer needs no clearing.
+void run_adjoint(tah_plan* p, double* adj, const double* factors, hipStream_t s, double* staged) {
+  plan_upload_job(p, plan_job(p, adj, factors), s);
+  const AdjointTables& A = *p->adj;
+  for (int32_t st = p->nstages - 1; st >= 0; --st) {
+    const AdjStage& S = A.stages[st];
+    launch_pass<2>(p, S, 0, s, staged);
+    launch_pass<1>(p, S, 1, s, staged);
+    launch_pass<0>(p, S, 2, s, staged);
+    if (S.gather[1] > 0) {
+      hipLaunchKernelGGL(adjoint_gather_kernel, dim3(uint32_t(S.gather[1])), dim3(kThreads), 0, s, p->djob,
+                         A.dcells, A.dtiles + S.gather[0], A.dentries, uint32_t(S.gather[1]), staged);
+      HIPCHECK(hipGetLastError());
+    }
+    if (S.zero[1] > 0) {
+      hipLaunchKernelGGL(zero_kernel, dim3(uint32_t(S.zero[1])), dim3(kThreads), 0, s, p->djob,
+                         A.dzero + S.zero[0]);
+      HIPCHECK(hipGetLastError());
+    }
+  }
 }
 
 }  // namespace
@@ -774,35 +815,14 @@ using namespace tah;
 
 extern "C" tah_status tah_fill_adjoint(tah_plan* p, double* adj, const double* factors, void* stream) {
   return guard([&] {
-    if (!p || !adj) fail("null argument");
-    if (p->regrid) fail("the adjoint of a regrid transfer is not provided");
-    if (p->needs_factors && !factors)
-      fail("this schedule mirrors ghosts across a reflecting face, but no factor table was given");
+    check_adjoint_call(p, adj, factors);
     HIPCHECK(hipSetDevice(p->ctx->device));
     if (!p->adj) build_adjoint(p);
+    AdjointTables& A = *p->adj;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    Job job = plan_job(p, adj, factors);
-    if (std::memcmp(&job, &p->last_job, sizeof(Job)) != 0) {
-      p->last_job = job;
-      HIPCHECK(hipMemcpyAsync(p->djob, &p->last_job, sizeof(Job), hipMemcpyHostToDevice, s));
-    }
-    const AdjointTables& A = *p->adj;
-    for (int32_t st = p->nstages - 1; st >= 0; --st) {
-      const AdjStage& S = A.stages[st];
-      launch_pass<2>(p, S, 0, s);
-      launch_pass<1>(p, S, 1, s);
-      launch_pass<0>(p, S, 2, s);
-      if (S.gather[1] > 0) {
-        hipLaunchKernelGGL(adjoint_gather_kernel, dim3(uint32_t(S.gather[1])), dim3(kThreads), 0, s, p->djob,
-                           A.dcells, A.dtiles + S.gather[0], A.dentries, uint32_t(S.gather[1]),
-                           A.scratch);
-        HIPCHECK(hipGetLastError());
-      }
-      if (S.zero[1] > 0) {
-        hipLaunchKernelGGL(zero_kernel, dim3(uint32_t(S.zero[1])), dim3(kThreads), 0, s, p->djob,
-                           A.dzero + S.zero[0]);
-        HIPCHECK(hipGetLastError());
-      }
-    }
+    plan_begin(p, s);
+    if (!A.scratch && A.scratch_values > 0) HIPCHECK(hipMalloc(&A.scratch, sizeof(double) * A.scratch_values));
+    run_adjoint(p, adj, factors, s, A.scratch);
+    plan_end(p, s);
   });
 }

@@ -10,6 +10,8 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "api_common.hpp"
@@ -19,8 +21,35 @@ using namespace tah;
 
 namespace tah {
 
+namespace {
+// Live plans and the serial of each one's current state. A serial is never
+// reused, so a plan freed and another created at the same address cannot be
+// taken for the old one.
+std::mutex registry_mutex;
+std::unordered_map<const tah_plan*, uint64_t> live_plans;
+uint64_t next_serial = 1;
+}  // namespace
+
+void plan_register(tah_plan* p) {
+  std::lock_guard<std::mutex> lock(registry_mutex);
+  p->serial = next_serial++;
+  live_plans[p] = p->serial;
+}
+
+void plan_unregister(tah_plan* p) {
+  std::lock_guard<std::mutex> lock(registry_mutex);
+  live_plans.erase(p);
+}
+
+bool plan_serial_live(const tah_plan* p, uint64_t serial) {
+  std::lock_guard<std::mutex> lock(registry_mutex);
+  auto it = live_plans.find(p);
+  return it != live_plans.end() && it->second == serial;
+}
+
 struct Batch {
   std::vector<tah_plan*> plans;
+  std::vector<uint64_t> serials;  // each plan's state when the batch was built
   std::vector<double*> works;
   std::vector<const double*> factors;
   std::vector<Job> jobs;
@@ -54,6 +83,7 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
     if (p->needs_factors && !f)
       fail("plan " + std::to_string(i) + " mirrors ghosts across a reflecting face, but no factor table was given");
     b->plans.push_back(p);
+    b->serials.push_back(p->serial);
     b->fork = b->fork || p->fork != 0;
     b->works.push_back(works[i]);
     b->factors.push_back(f);
@@ -104,13 +134,20 @@ void run_batch(tah_ctx* ctx, Batch& b, hipStream_t s) {
   for (const auto& ls : b.launches) run_launches(ctx, ls, s, b.djobs, b.dtiles, b.events.data(), b.fork);
 }
 
+// Every plan of the batch alive and in the state the batch was built from.
+bool batch_current(const Batch& b) {
+  for (size_t i = 0; i < b.plans.size(); ++i)
+    if (!plan_serial_live(b.plans[i], b.serials[i])) return false;
+  return true;
+}
+
 bool same_batch(const Batch& b, int32_t n, tah_plan* const* plans, double* const* works,
                 const double* const* factors) {
   if (int32_t(b.plans.size()) != n) return false;
   for (int32_t i = 0; i < n; ++i)
     if (b.plans[i] != plans[i] || b.works[i] != works[i] || b.factors[i] != (factors ? factors[i] : nullptr))
       return false;
-  return true;
+  return batch_current(b);
 }
 
 }  // namespace
@@ -128,9 +165,36 @@ struct tah_graph {
 };
 
 namespace {
-// The last batch of each context, reused while the same fills are asked for.
-std::map<tah_ctx*, std::unique_ptr<Batch>> last_batch;
+// The last batch of each context, reused while the same fills (the same
+// plans in the same states, arrays and factor tables) are asked for.
+std::mutex batch_mutex;
+std::map<const tah_ctx*, std::unique_ptr<Batch>> last_batch;
 }  // namespace
+
+namespace tah {
+
+void batch_forget_plan(const tah_plan* p) {
+  std::lock_guard<std::mutex> lock(batch_mutex);
+  for (auto it = last_batch.begin(); it != last_batch.end();) {
+    const auto& ps = it->second ? it->second->plans : std::vector<tah_plan*>{};
+    if (std::find(ps.begin(), ps.end(), p) != ps.end()) {
+      (void)hipDeviceSynchronize();  // its launches may still read the batch's tables
+      it = last_batch.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
+void batch_forget_ctx(const tah_ctx* c) {
+  std::lock_guard<std::mutex> lock(batch_mutex);
+  auto it = last_batch.find(c);
+  if (it == last_batch.end()) return;
+  (void)hipDeviceSynchronize();
+  last_batch.erase(it);
+}
+
+}  // namespace tah
 
 extern "C" {
 
@@ -139,6 +203,7 @@ tah_status tah_fill_batched(tah_ctx* ctx, int32_t n, tah_plan* const* plans, dou
   return guard([&] {
     if (!ctx) fail("null context");
     HIPCHECK(hipSetDevice(ctx->device));
+    std::lock_guard<std::mutex> lock(batch_mutex);
     auto& slot = last_batch[ctx];
     if (!slot || !same_batch(*slot, n, plans, works, factors)) {
       HIPCHECK(hipDeviceSynchronize());  // the previous batch's tables may still be in use
@@ -183,6 +248,9 @@ tah_status tah_graph_create(tah_ctx* ctx, int32_t n, tah_plan* const* plans, dou
 tah_status tah_graph_launch(tah_graph* g, void* stream) {
   return guard([&] {
     if (!g) fail("null graph");
+    if (!batch_current(*g->batch))
+      fail("a plan of this graph was destroyed, or its options changed, after the graph was created; "
+           "create the graph again");
     HIPCHECK(hipSetDevice(g->ctx->device));
     HIPCHECK(hipGraphLaunch(g->exec, reinterpret_cast<hipStream_t>(stream)));
   });

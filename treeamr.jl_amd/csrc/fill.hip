@@ -223,6 +223,7 @@ tah_status tah_ctx_create(int32_t device, tah_ctx** out) {
 }
 
 tah_status tah_ctx_destroy(tah_ctx* ctx) {
+  if (ctx) batch_forget_ctx(ctx);
   delete ctx;
   return TAH_OK;
 }
@@ -377,10 +378,8 @@ tah_status tah_regrid_transfer(tah_plan* p, const tah_layout* old_layout, const 
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     // Stage REGRID reads the old array; stage REGRID_KEPT the new one.
     const Job j0 = plan_job(p, new_work, nullptr, old_work), j1 = plan_job(p, new_work, nullptr);
-    if (std::memcmp(&j0, &p->last_job, sizeof(Job)) != 0) {
-      p->last_job = j0;
-      HIPCHECK(hipMemcpyAsync(p->djob, &p->last_job, sizeof(Job), hipMemcpyHostToDevice, s));
-    }
+    plan_begin(p, s);
+    plan_upload_job(p, j0, s);
     if (std::memcmp(&j1, &p->last_job2, sizeof(Job)) != 0) {
       p->last_job2 = j1;
       HIPCHECK(hipMemcpyAsync(p->djob2, &p->last_job2, sizeof(Job), hipMemcpyHostToDevice, s));
@@ -394,6 +393,7 @@ tah_status tah_regrid_transfer(tah_plan* p, const tah_layout* old_layout, const 
     for (size_t k = 0; k < p->launches.size(); ++k)
       run_launches(p->ctx, p->launches[k], s, k == 0 ? p->djob : p->djob2, p->dtiles, p->events.data(),
                    p->fork != 0);
+    plan_end(p, s);
   });
 }
 
@@ -405,12 +405,8 @@ tah_status tah_fill(tah_plan* p, double* work, const double* factors, void* stre
       fail("this schedule mirrors ghosts across a reflecting face, but no factor table was given");
     HIPCHECK(hipSetDevice(p->ctx->device));
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    Job job = plan_job(p, work, factors);
-    if (std::memcmp(&job, &p->last_job, sizeof(Job)) != 0) {
-      // From the plan's own copy: the host buffer outlives the copy.
-      p->last_job = job;
-      HIPCHECK(hipMemcpyAsync(p->djob, &p->last_job, sizeof(Job), hipMemcpyHostToDevice, s));
-    }
+    plan_begin(p, s);
+    plan_upload_job(p, plan_job(p, work, factors), s);
     if (p->events.empty())
       for (int i = 0; i < 8; ++i) {
         hipEvent_t e;
@@ -419,6 +415,7 @@ tah_status tah_fill(tah_plan* p, double* work, const double* factors, void* stre
       }
     for (const auto& sg : p->launches)
       run_launches(p->ctx, sg, s, p->djob, p->dtiles, p->events.data(), p->fork != 0);
+    plan_end(p, s);
   });
 }
 
@@ -440,11 +437,8 @@ tah_status tah_fill_timed(tah_plan* p, double* work, const double* factors, void
       fail("this schedule mirrors ghosts across a reflecting face, but no factor table was given");
     HIPCHECK(hipSetDevice(p->ctx->device));
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    Job job = plan_job(p, work, factors);
-    if (std::memcmp(&job, &p->last_job, sizeof(Job)) != 0) {
-      p->last_job = job;
-      HIPCHECK(hipMemcpyAsync(p->djob, &p->last_job, sizeof(Job), hipMemcpyHostToDevice, s));
-    }
+    plan_begin(p, s);
+    plan_upload_job(p, plan_job(p, work, factors), s);
     std::vector<hipEvent_t> ev(size_t(total) + 1);
     for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
     try {
@@ -458,6 +452,7 @@ tah_status tah_fill_timed(tah_plan* p, double* work, const double* factors, void
           values[k] = L.values;
           ++k;
         }
+      plan_end(p, s);
       HIPCHECK(hipEventSynchronize(ev[total]));
       for (int32_t i = 0; i < total; ++i) HIPCHECK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
     } catch (...) {

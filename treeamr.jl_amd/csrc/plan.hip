@@ -243,6 +243,7 @@ void build_plan_device(tah_plan* p) {
   HIPCHECK(hipMemcpy(p->dtables, blob.host.data(), blob.host.size(), hipMemcpyHostToDevice));
   HIPCHECK(hipMalloc(&p->djob, sizeof(Job)));
   std::memset(&p->last_job, 0, sizeof(Job));
+  HIPCHECK(hipEventCreateWithFlags(&p->done, hipEventDisableTiming));
   if (p->regrid) {
     HIPCHECK(hipMalloc(&p->djob2, sizeof(Job)));
     std::memset(&p->last_job2, 0, sizeof(Job));
@@ -449,6 +450,8 @@ void build_launches(tah_plan* p) {
     std::stable_sort(ls.begin(), ls.end(), [](const Launch& x, const Launch& y) { return x.count > y.count; });
     p->launches.push_back(std::move(ls));
   }
+  // A new state of the plan: batches built from an earlier one are stale.
+  plan_register(p);
   if (p->dtiles) {
     // a previous tile list may still be read by queued fills
     HIPCHECK(hipDeviceSynchronize());
@@ -480,6 +483,33 @@ Job plan_job(const tah_plan* p, double* work, const double* factors, const doubl
                   ? 1
                   : 0;
   return job;
+}
+
+namespace {
+bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  HIPCHECK(hipStreamIsCapturing(s, &st));
+  return st != hipStreamCaptureStatusNone;
+}
+}  // namespace
+
+void plan_begin(tah_plan* p, hipStream_t s) {
+  if (!p->pending || s == p->last_stream || capturing(s)) return;
+  HIPCHECK(hipStreamWaitEvent(s, p->done, 0));
+}
+
+void plan_end(tah_plan* p, hipStream_t s) {
+  if (capturing(s)) return;
+  HIPCHECK(hipEventRecord(p->done, s));
+  p->last_stream = s;
+  p->pending = true;
+}
+
+void plan_upload_job(tah_plan* p, const Job& job, hipStream_t s) {
+  if (std::memcmp(&job, &p->last_job, sizeof(Job)) == 0) return;
+  // From the plan's own copy: the host buffer outlives the copy.
+  p->last_job = job;
+  HIPCHECK(hipMemcpyAsync(p->djob, &p->last_job, sizeof(Job), hipMemcpyHostToDevice, s));
 }
 
 const std::vector<hipStream_t>& ctx_streams(tah_ctx* c) {
@@ -524,8 +554,13 @@ void free_adjoint(tah_plan* p);
 }
 
 tah_plan::~tah_plan() {
+  // queued work of this plan, and cached batches that use it, end first
+  (void)hipDeviceSynchronize();
+  tah::batch_forget_plan(this);
+  tah::plan_unregister(this);
   tah::free_adjoint(this);
   for (hipEvent_t e : events) (void)hipEventDestroy(e);
+  if (done) (void)hipEventDestroy(done);
   if (dtables) (void)hipFree(dtables);
   if (dtiles) (void)hipFree(dtiles);
   if (djob) (void)hipFree(djob);

@@ -130,6 +130,17 @@ struct tah_plan {
   std::vector<hipEvent_t> events;  // fork/join events, created on first use
   // The adjoint's inverted schedule, built on first use (adjoint.hip).
   tah::AdjointTables* adj = nullptr;
+  // A number no other plan, and no earlier state of this one, ever had:
+  // taken anew whenever the tiles and launches are (re)built, so that a
+  // batch built from this plan can tell that it is stale.
+  uint64_t serial = 0;
+  // Calls on one plan rewrite its device Job (and the adjoint its staging
+  // buffer) on the caller's stream. A call on another stream than the
+  // previous one first waits for `done`, recorded after the previous call's
+  // last launch on `last_stream`.
+  hipStream_t last_stream = nullptr;
+  bool pending = false;
+  hipEvent_t done = nullptr;
   ~tah_plan();
 };
 
@@ -155,4 +166,22 @@ void run_launches(tah_ctx* c, const std::vector<Launch>& ls, hipStream_t s, cons
                   const Tile* tiles, hipEvent_t* ev, bool fork);
 // Launch one class over ntiles tiles (fill.hip); grid > 0: a bounded grid.
 void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles, int32_t grid = 0);
+// Order a call on plan p and stream s after the plan's previous call (on
+// whatever stream it ran), before the call rewrites the plan's device Job
+// or staging buffer; plan_end marks the call's last launch. A stream under
+// graph capture is ordered by the graph's own launch instead: no event is
+// waited on or recorded there.
+void plan_begin(tah_plan* p, hipStream_t s);
+void plan_end(tah_plan* p, hipStream_t s);
+// Upload p's Job for this call where it differs from the last one uploaded.
+void plan_upload_job(tah_plan* p, const Job& job, hipStream_t s);
+// Whether the serial is that of a live plan in its current state, and the
+// registry behind it (batch.hip).
+bool plan_serial_live(const tah_plan* p, uint64_t serial);
+void plan_register(tah_plan* p);
+void plan_unregister(tah_plan* p);
+// Drop every cached batch that uses plan p (or that belongs to context c),
+// after the device has finished with it (batch.hip).
+void batch_forget_plan(const tah_plan* p);
+void batch_forget_ctx(const tah_ctx* c);
 }  // namespace tah
