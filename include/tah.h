@@ -308,6 +308,18 @@ tah_status tah_graph_destroy(tah_graph* g);
  * in a fixed order (no float atomics), then the targets zeroed. */
 tah_status tah_fill_adjoint(tah_plan* p, double* adj, const double* factors, void* stream);
 
+/* tah_fill_adjoint without any allocation. No counterpart in the reference:
+ * this is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The passes stage their values in a buffer compiled into the library, at
+ * most 8 Mi doubles (64 MiB) per GPU, held by each GPU the library runs on
+ * until the process ends; a plan whose adjoint stages more is refused
+ * (TAH_EUNSUPPORTED). The plan's adjoint tables must exist: the plan's first
+ * tah_fill_adjoint call builds them at set-up (TAH_EINVAL otherwise). Every
+ * call on one GPU shares that GPU's buffer: calls made one after another
+ * are ordered by the library, on any streams, but calls that overlap (from
+ * several host threads at once, or graph replays on several streams) race. */
+tah_status tah_fill_adjoint_noalloc(tah_plan* p, double* adj, const double* factors, void* stream);
 
 /* A plan's launch options (no counterpart in the reference, whose launches
  * are fixed): fork runs a stage group's kernel classes on four forked

@@ -811,6 +811,38 @@ void run_adjoint(tah_plan* p, double* adj, const double* factors, hipStream_t s,
 }  // namespace
 }  // namespace tah
 
+// The staging buffer of tah_fill_adjoint_noalloc: one array per GPU in the
+// code object, so that the call allocates nothing. Host code never uses this
+// name as an address (on the host it stands in for the symbol only): the
+// device address of each GPU's copy is taken once per device by
+// noalloc_scratch_init and used from there on.
+static __device__ __align__(256) double scratch[tah::kNoallocScratchValues];
+
+namespace tah {
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex noalloc_mutex;
+// Per device: the array's address, and the stream and completion event of
+// the last no-allocation adjoint on that device.
+double* noalloc_ptr[kMaxDevices] = {};
+hipEvent_t noalloc_done[kMaxDevices] = {};
+hipStream_t noalloc_stream[kMaxDevices] = {};
+bool noalloc_pending[kMaxDevices] = {};
+}  // namespace
+
+void noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) fail("device index out of range for the no-allocation adjoint");
+  std::lock_guard<std::mutex> lock(noalloc_mutex);
+  if (noalloc_ptr[device]) return;
+  void* ptr = nullptr;
+  HIPCHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) fail("no device address for the no-allocation adjoint's staging buffer", kHip);
+  HIPCHECK(hipEventCreateWithFlags(&noalloc_done[device], hipEventDisableTiming));
+  noalloc_ptr[device] = static_cast<double*>(ptr);
+}
+
+}  // namespace tah
+
 using namespace tah;
 
 extern "C" tah_status tah_fill_adjoint(tah_plan* p, double* adj, const double* factors, void* stream) {
@@ -823,6 +855,41 @@ extern "C" tah_status tah_fill_adjoint(tah_plan* p, double* adj, const double* f
     plan_begin(p, s);
     if (!A.scratch && A.scratch_values > 0) HIPCHECK(hipMalloc(&A.scratch, sizeof(double) * A.scratch_values));
     run_adjoint(p, adj, factors, s, A.scratch);
+    plan_end(p, s);
+  });
+}
+
+extern "C" tah_status tah_fill_adjoint_noalloc(tah_plan* p, double* adj, const double* factors, void* stream) {
+  return guard([&] {
+    check_adjoint_call(p, adj, factors);
+    if (!p->adj)
+      fail("the plan's adjoint tables are not built yet: they are built (and allocated) at set-up, by the "
+           "plan's first tah_fill_adjoint call");
+    const int dev = p->ctx->device;
+    if (p->adj->scratch_values > kNoallocScratchValues)
+      fail("this adjoint stages " + std::to_string(p->adj->scratch_values) +
+               " values; the no-allocation staging buffer holds at most " + std::to_string(kNoallocScratchValues) +
+               " (64 MiB): use tah_fill_adjoint",
+           kUnsupported);
+    std::lock_guard<std::mutex> lock(noalloc_mutex);
+    if (dev < 0 || dev >= kMaxDevices || !noalloc_ptr[dev])
+      fail("the no-allocation staging buffer of device " + std::to_string(dev) + " was not set up", kHip);
+    HIPCHECK(hipSetDevice(dev));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    plan_begin(p, s);
+    // The buffer is shared by every call on this GPU: a call on another
+    // stream than the last one waits for it.
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIPCHECK(hipStreamIsCapturing(s, &cap));
+    const bool captured = cap != hipStreamCaptureStatusNone;
+    if (!captured && noalloc_pending[dev] && noalloc_stream[dev] != s)
+      HIPCHECK(hipStreamWaitEvent(s, noalloc_done[dev], 0));
+    run_adjoint(p, adj, factors, s, noalloc_ptr[dev]);
+    if (!captured) {
+      HIPCHECK(hipEventRecord(noalloc_done[dev], s));
+      noalloc_stream[dev] = s;
+      noalloc_pending[dev] = true;
+    }
     plan_end(p, s);
   });
 }

@@ -218,6 +218,8 @@ tah_status tah_ctx_create(int32_t device, tah_ctx** out) {
     auto c = std::make_unique<tah_ctx>();
     c->device = device;
     HIPCHECK(hipGetDeviceProperties(&c->prop, device));
+    HIPCHECK(hipSetDevice(device));
+    noalloc_scratch_init(device);
     *out = c.release();
   });
 }

@@ -180,6 +180,12 @@ void plan_upload_job(tah_plan* p, const Job& job, hipStream_t s);
 bool plan_serial_live(const tah_plan* p, uint64_t serial);
 void plan_register(tah_plan* p);
 void plan_unregister(tah_plan* p);
+// The no-allocation adjoint's staging buffer: at most this many doubles
+// (64 MiB) per GPU, held by every GPU the library runs on until the process
+// ends. noalloc_scratch_init takes its device address, once per device, with
+// that device current (adjoint.hip; called when a context is created).
+constexpr int64_t kNoallocScratchValues = int64_t(8) << 20;
+void noalloc_scratch_init(int device);
 // Drop every cached batch that uses plan p (or that belongs to context c),
 // after the device has finished with it (batch.hip).
 void batch_forget_plan(const tah_plan* p);

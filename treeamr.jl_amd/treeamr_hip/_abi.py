@@ -90,6 +90,7 @@ SIGNATURES = {
     "tah_fill_timed": [P, P, P, P, I32P, I32P, C.POINTER(C.c_int64), C.POINTER(C.c_float)],
     "tah_kernel_class_name": [C.c_int32, C.c_char_p, C.c_int32],
     "tah_fill_adjoint": [P, P, P, P],
+    "tah_fill_adjoint_noalloc": [P, P, P, P],
     "tah_plan_get_options": [P, C.POINTER(tah_plan_options)],
     "tah_plan_set_options": [P, C.POINTER(tah_plan_options)],
     "tah_tune": [P, P, P, C.c_char_p, C.POINTER(C.c_double)],
