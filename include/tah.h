@@ -79,6 +79,17 @@ tah_status tah_forest_isbalanced(const tah_forest* f, int32_t* balanced);
  * becomes its parent, unless the parent would meet a leaf two levels finer
  * (2:1 balance holds after). coarsened (may be NULL): families merged. */
 tah_status tah_forest_coarsen(tah_forest* f, int32_t n, const int32_t* leaves, int32_t* coarsened);
+/* Positions of the stored points along dimension d (0-based) of every leaf:
+ * out[b*S + j] for stored index j = 0 .. S-1 of leaf b, ghosts included, of a
+ * set with ghost width G and centering c (0 cell-like, 1 vertex-like) along
+ * d. Formed as nodeposition does (src/storage.jl:1008-1041, with the anchor,
+ * integer base and spacing of src/geometry.jl:57,76-131): the integer k =
+ * B + j - G counts spacings h of the leaf's level from the anchor lo (zero
+ * where zero is a node of the level-0 grid, else the low corner), and the
+ * position is fma(k, h, lo) (vertex-like) or fma(2k+1, h/2, lo) (cell-like),
+ * rounded once; so every copy of a point is at one position bit for bit. */
+tah_status tah_forest_positions(const tah_forest* f, int32_t d, int32_t G, int32_t c, int32_t S,
+                                double* out);
 
 enum { TAH_POINTVALUE = 0, TAH_CONSERVATIVE = 1 };
 /* Operators(; prolongation, restriction, family, bounded,

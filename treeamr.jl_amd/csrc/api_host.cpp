@@ -1,5 +1,6 @@
 // C ABI, host side: forests, operators, schedules and their flattened
 // descriptors. Nothing here touches a GPU.
+#include <cmath>
 #include <cstring>
 #include <memory>
 
@@ -210,6 +211,64 @@ tah_status tah_forest_leaves(const tah_forest* f, int32_t* root, int32_t* level,
       if (level) level[i] = L[i].level;
       if (coords)
         for (int d = 0; d < kMaxDim; ++d) coords[3 * i + d] = L[i].c[d];
+    }
+  });
+}
+
+namespace {
+
+// A double as an exact integer times a power of two.
+void exact_parts(double x, __int128& m, int& e) {
+  int ex = 0;
+  const double fr = std::frexp(x, &ex);  // x = fr * 2^ex, 0.5 <= |fr| < 1
+  m = (__int128)std::ldexp(fr, 53);
+  e = ex - 53;
+}
+
+// The anchor of dimension d (src/geometry.jl:76-91): m = -lo*n/(hi-lo), n
+// = roots*N level-0 spacings, when that is an integer in 0..n (zero is a
+// node of the level-0 grid), else 0. Exact: lo and hi are brought to one
+// power of two; a difference of exponents beyond 60 bits leaves m tiny or
+// far outside 0..n (or lo zero: m = 0 either way).
+int64_t position_anchor(double lo, double hi, int64_t n) {
+  if (lo == 0.0 || !(hi > lo)) return 0;
+  __int128 L, H;
+  int el, eh;
+  exact_parts(lo, L, el);
+  exact_parts(hi, H, eh);
+  const int e = std::min(el, eh);
+  if (el - e > 60 || eh - e > 60) return 0;
+  L <<= (el - e);
+  H <<= (eh - e);
+  const __int128 num = -L * (__int128)n, den = H - L;  // den > 0
+  if (num % den != 0) return 0;
+  const __int128 m = num / den;
+  return (m >= 0 && m <= n) ? int64_t(m) : 0;
+}
+
+}  // namespace
+
+tah_status tah_forest_positions(const tah_forest* f, int32_t d, int32_t G, int32_t c, int32_t S, double* out) {
+  return guard([&] {
+    if (!f || !out) fail("null argument");
+    const Forest& F = *f->f;
+    if (d < 0 || d >= F.D) fail("dimension out of range");
+    if (G < 0 || S < 1 || (c != 0 && c != 1)) fail("bad ghost width, stored size or centering");
+    const int64_t n = int64_t(F.roots[d]) * F.N;
+    const int64_t m = position_anchor(F.extents[d][0], F.extents[d][1], n);
+    const double lo = m > 0 ? 0.0 : F.extents[d][0];
+    // root_spacing (src/geometry.jl:19-20): from dimension 1's extent and roots
+    const double h0 = (F.extents[0][1] - F.extents[0][0]) / F.roots[0] / F.N;
+    for (size_t b = 0; b < F.leaves.size(); ++b) {
+      const Key& k = F.leaves[b];
+      const int lvl = k.level;
+      const double h = std::ldexp(h0, -lvl);
+      const int64_t rootpos = F.root_position(k.root)[d];
+      const int64_t B = ((rootpos << lvl) + int64_t(k.c[d])) * F.N - (m << lvl);
+      for (int32_t j = 0; j < S; ++j) {
+        const int64_t kk = B + j - G;
+        out[b * S + j] = c == 1 ? std::fma(double(kk), h, lo) : std::fma(double(2 * kk + 1), h / 2, lo);
+      }
     }
   });
 }

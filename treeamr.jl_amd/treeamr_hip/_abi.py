@@ -96,6 +96,7 @@ SIGNATURES = {
     "tah_tune": [P, P, P, C.c_char_p, C.POINTER(C.c_double)],
     "tah_state_length": [C.POINTER(tah_layout), I32P, C.POINTER(C.c_int64)],
     "tah_forest_coarsen": [P, C.c_int32, I32P, I32P],
+    "tah_forest_positions": [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, DP],
     "tah_unify_schedule_create": [P, I32P, I32P, C.c_int32, C.POINTER(C.c_int8), C.POINTER(P)],
     "tah_unify_desc_get": [P, C.c_int32, C.POINTER(C.POINTER(tah_schedule_desc))],
     "tah_regrid_schedule_create": [P, C.c_int32, I32P, I32P, C.POINTER(C.c_uint32), C.POINTER(tah_operators),
@@ -116,6 +117,11 @@ def lib():
             raise RuntimeError(
                 f"libtah is not built ({LIBPATH} missing): run `make -C treeamr.jl_amd` "
                 "or __graft_entry__.build()")
+        # torch first: its HIP runtime (soname libamdhip64.so.7, which
+        # libtah needs) is then the one the process binds libtah to. Loaded
+        # the other way round, torch brings in a second runtime beside
+        # libtah's, and libtah's finds no device once torch's has the GPU.
+        import torch  # noqa: F401
         L = C.CDLL(LIBPATH)
         for name, args in SIGNATURES.items():
             fn = getattr(L, name)

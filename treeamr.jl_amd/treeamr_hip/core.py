@@ -136,6 +136,14 @@ class Forest:
         h = H[0] * scale / self.N
         return lows, h, level
 
+    def positions(self, d, G, c, S):
+        """tah_forest_positions: the positions along dimension d (0-based) of
+        stored indices 0..S-1 of every leaf, for ghost width G and centering c
+        (0 cell, 1 vertex), as nodeposition forms them, (nleaves, S)."""
+        out = np.zeros((self.nleaves, int(S)), np.float64)
+        check(lib().tah_forest_positions(self._h, int(d), int(G), int(c), int(S), out.ctypes.data_as(_abi.DP)))
+        return out
+
     def block_centers(self):
         lows, h, level = self.block_geometry()
         return lows + 0.5 * (h * self.N)[:, None]
@@ -438,9 +446,14 @@ class FieldSet:
                 sl.append(slice(0, 1))
         return tuple(sl)
 
+    def positions(self, d):
+        """Positions along dimension d of every stored index of every block,
+        (nblocks, S[d]), as nodeposition forms them (one rounding;
+        src/storage.jl:1008-1041)."""
+        return self.forest.positions(d, self.G[d], self.centering[d], self.stored[d])
+
     def coordinates(self):
         """Positions of every stored point: three arrays (nblocks, S2, S1, S0)."""
-        lows, h, _ = self.forest.block_geometry()
         D = self.forest.D
         out = []
         S = self.stored
@@ -448,9 +461,7 @@ class FieldSet:
             if d >= D:
                 out.append(np.zeros((self.nblocks, S[2], S[1], S[0])))
                 continue
-            I = np.arange(S[d]) - self.G[d]
-            off = I + (0.5 if self.centering[d] == 0 else 0.0)
-            x = lows[:, d][:, None] + off[None, :] * h[:, None]
+            x = self.positions(d)
             shape = [self.nblocks, 1, 1, 1]
             shape[3 - d] = S[d]
             out.append(np.broadcast_to(x.reshape(shape), (self.nblocks, S[2], S[1], S[0])))

@@ -184,16 +184,12 @@ def fill_gaussians(fs, seed, noise=1e-4):
     import torch
     rng = np.random.default_rng(seed)
     dev = fs.work.device
-    lows, h, _ = fs.forest.block_geometry()
-    lows_t = torch.tensor(lows, dtype=torch.float64, device=dev)
-    h_t = torch.tensor(h, dtype=torch.float64, device=dev)
-    N = fs.forest.N
     a = fs.array()
     own = fs.own_slices()
     g = torch.Generator(device=dev)
     g.manual_seed(seed)
-    idx = torch.arange(N, dtype=torch.float64, device=dev) + 0.5
-    x = [lows_t[:, d, None] + idx[None, :] * h_t[:, None] for d in range(3)]  # (nb, N)
+    # own points' positions per dimension, (nb, own extent), one rounding each
+    x = [torch.tensor(fs.positions(d)[:, own[2 - d]], dtype=torch.float64, device=dev) for d in range(3)]
     for v in range(fs.nvars):
         c = rng.uniform(0.2, 0.8, size=3)
         s = rng.uniform(0.05, 0.2)
