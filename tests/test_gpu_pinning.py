@@ -48,3 +48,38 @@ def test_reflecting_equals_doubled_gpu(side, centering, layout):
     assert not np.isnan(ah).any()
     worst = max(np.abs(ah[b] - af[fb]).max() for b, fb in pairs)
     assert worst < 1e-13, worst
+
+
+@pytest.mark.parametrize("which", ["config2", "config5_small", "random10"])
+@pytest.mark.parametrize("layout", ["dense", "arena"])
+def test_grouped_fill_in_any_order_within_its_groups_gpu(which, layout):
+    """Plans of the schedule with every stage group's stages reversed and
+    with every stage alone fill as the schedule's own plan does, from NaN
+    ghosts, bit for bit and with no NaN left: the levels a plan launches
+    together are independent."""
+    import ctypes as C
+
+    import torch
+
+    from test_pinning_host import grouped_case, restaged
+    from treeamr_hip._abi import check, lib
+    from treeamr_hip.core import _stream_ptr, default_context
+    print(f"[pinning] stage groups, {which} {layout}", flush=True)
+    fs, s = grouped_case(which, device="cuda", layout=layout)
+    init = fs.work.clone()
+    _fill(fs, s)
+    ref = fs.work.clone()
+    ctx = default_context()
+    fptr = C.c_void_p(fs.factors.data_ptr()) if fs.factors is not None else None
+    for mode in ("reversed", "alone"):
+        ptr, keep = restaged(s, mode)
+        plan = C.c_void_p()
+        check(lib().tah_plan_create(ctx._h, ptr, C.byref(fs.layout), C.byref(plan)))
+        try:
+            fs.work.copy_(init)
+            check(lib().tah_fill(plan, C.c_void_p(fs.work.data_ptr()), fptr, _stream_ptr(fs)))
+            torch.cuda.synchronize()
+        finally:
+            lib().tah_plan_destroy(plan)
+        assert not torch.isnan(fs.array()).any(), mode  # (the arena's padding stays NaN)
+        assert torch.equal(fs.work.view(torch.int64), ref.view(torch.int64)), mode

@@ -426,3 +426,87 @@ def test_reflecting_equals_doubled(side, centering):
     assert not np.isnan(ah).any()
     worst = max(np.abs(ah[b] - af[fb]).max() for b, fb in pairs)
     assert worst < 1e-13, worst
+
+
+# --- stage groups: any order within a group ---------------------------------
+
+def restaged(s, mode):
+    """The schedule's flattened descriptor with its stages rearranged: mode
+    "reversed" runs the stages of every stage group in reverse order
+    (fill_stagewise!(...; reverse=true), test/ghost_oracles.jl:314-329),
+    "alone" makes every stage a stage group of its own. Returns (pointer to
+    the new descriptor, objects it points into)."""
+    import ctypes as C
+
+    from treeamr_hip import _abi
+    d = s.desc
+    sgs = [(d.stagegroup_first[k], d.stagegroup_last[k]) for k in range(d.nstagegroups)]
+    newidx = list(range(d.nstages))
+    if mode == "reversed":
+        for a, b in sgs:
+            for st in range(a, b + 1):
+                newidx[st] = a + b - st
+    elif mode == "alone":
+        sgs = [(k, k) for k in range(d.nstages)]
+    tags = [0] * d.nstages
+    for st in range(d.nstages):
+        tags[newidx[st]] = d.stage_tags[st]
+    groups = [d.groups[gi] for gi in range(d.ngroups)]
+    order = sorted(range(len(groups)), key=lambda gi: newidx[groups[gi].stage])
+    garr = (_abi.tah_group_desc * max(1, len(groups)))()
+    for k, gi in enumerate(order):
+        C.memmove(C.byref(garr[k]), C.byref(groups[gi]), C.sizeof(_abi.tah_group_desc))
+        garr[k].stage = newidx[groups[gi].stage]
+    tag_arr = (C.c_int32 * d.nstages)(*tags)
+    first = (C.c_int32 * len(sgs))(*[a for a, _ in sgs])
+    last = (C.c_int32 * len(sgs))(*[b for _, b in sgs])
+    nd = _abi.tah_schedule_desc()
+    C.memmove(C.byref(nd), C.byref(d), C.sizeof(nd))
+    nd.stage_tags = C.cast(tag_arr, C.POINTER(C.c_int32))
+    nd.nstagegroups = len(sgs)
+    nd.stagegroup_first = C.cast(first, C.POINTER(C.c_int32))
+    nd.stagegroup_last = C.cast(last, C.POINTER(C.c_int32))
+    nd.groups = C.cast(garr, C.POINTER(_abi.tah_group_desc))
+    return C.pointer(nd), (nd, garr, tag_arr, first, last, s)
+
+
+def grouped_case(which, device="cpu", layout="dense"):
+    if which == "config2":
+        f, ops, kw, _ = W.config2()
+        kw = dict(kw)
+        fs = FieldSet(f, kw.pop("nvars"), device=device, layout=layout, fill=float("nan"), **kw)
+        W.fill_waves(fs, 1)
+    else:
+        if which == "config5_small":
+            f, ops, kw, _ = W.config5(nvars=2, roots=4, N=16)
+        else:
+            f, ops, G, nv, centering = W.random_forest(int(which[6:]), max_blocks=300)
+            kw = dict(nvars=nv[0], G=G, centering=centering)
+        kw = dict(kw)
+        fs = FieldSet(f, kw.pop("nvars"), device=device, layout=layout, fill=float("nan"), **kw)
+        W.fill_uniform(fs, 3)
+    s = GhostSchedule(fs, ops)
+    return fs, s
+
+
+@pytest.mark.parametrize("which", ["config2", "config5_small", "random10"])
+def test_grouped_fill_in_any_order_within_its_groups(which):
+    """From NaN in every ghost: the fill, the fill with every stage group's
+    stages in reverse order, and the fill one stage at a time agree in every
+    bit and leave no NaN (test/ghost_tests.jl:399-453): a level grouped with
+    one it depends on would read a NaN."""
+    fs, s = grouped_case(which)
+    assert any(b > a for a, b in s.stagegroups())
+    init = fs.work.numpy().copy()
+    fac = fs.factors.numpy().copy() if fs.factors is not None else None
+    out = []
+    for mode in ("grouped", "reversed", "alone"):
+        w = init.copy()
+        if mode == "grouped":
+            oracle.fill(s._descptr, fs.layout, w, fac, nthreads=8)
+        else:
+            ptr, _keep = restaged(s, mode)
+            oracle.fill(ptr, fs.layout, w, fac, nthreads=8)
+        assert not np.isnan(w).any(), mode
+        out.append(w.view(np.int64))
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])

@@ -349,14 +349,21 @@ tah_status tah_fill_plain(tah_plan* p, double* work, const double* factors, void
       for (size_t gi = 0; gi < p->groups.size(); ++gi) {
         const PlanGroup& h = p->groups[gi];
         if (h.stage != stage) continue;
-        uint64_t total = uint64_t(h.points) * L.nvars * h.tblocks.size();
-        if (total == 0) continue;
-        if (total >= (uint64_t(1) << 32)) fail("a group larger than 2^32 values", kUnsupported);
-        uint32_t nb = uint32_t((total + 255) / 256);
-        hipLaunchKernelGGL(plain_kernel, dim3(nb), dim3(256), 0, s, p->devgroups[gi], work, factors,
-                           L.stride[0], L.stride[1], L.stride[2], L.stride[3], L.stride[4], L.nvars,
-                           dv, uint32_t(total));
-        HIPCHECK(hipGetLastError());
+        const uint64_t per = uint64_t(h.points) * L.nvars;  // values per transfer (< 2^31)
+        const int64_t nt = int64_t(h.tblocks.size());
+        if (per == 0 || nt == 0) continue;
+        // The flat index is 32-bit: a group of 2^31 values or more runs as
+        // several launches over runs of its transfers, where the reference
+        // switches its inverses to Int64 (src/threading.jl:240-267).
+        const int64_t chunk = std::max<int64_t>(1, int64_t((uint64_t(1) << 31) / per));
+        for (int64_t t0 = 0; t0 < nt; t0 += chunk) {
+          const uint64_t total = per * uint64_t(std::min(chunk, nt - t0));
+          const uint32_t nb = uint32_t((total + 255) / 256);
+          hipLaunchKernelGGL(plain_kernel, dim3(nb), dim3(256), 0, s, p->devgroups[gi], work, factors,
+                             L.stride[0], L.stride[1], L.stride[2], L.stride[3], L.stride[4], L.nvars,
+                             dv, uint32_t(total), int32_t(t0));
+          HIPCHECK(hipGetLastError());
+        }
       }
     }
   });

@@ -94,7 +94,7 @@ __device__ __forceinline__ double stencil_value(const DevGroup& g, const double*
 __global__ void __launch_bounds__(256) plain_kernel(DevGroup g, double* __restrict__ work,
                                                     const double* __restrict__ factors, int64_t s0,
                                                     int64_t s1, int64_t s2, int64_t s3, int64_t s4,
-                                                    int32_t nvars, FastDiv dv, uint32_t total) {
+                                                    int32_t nvars, FastDiv dv, uint32_t total, int32_t t0) {
   uint32_t e = blockIdx.x * 256u + threadIdx.x;
   if (e >= total) return;
   uint32_t r = g.dn[0].div(e);
@@ -103,8 +103,9 @@ __global__ void __launch_bounds__(256) plain_kernel(DevGroup g, double* __restri
   int32_t j = int32_t(r - r2 * uint32_t(g.n[1]));
   uint32_t r3 = g.dn[2].div(r2);
   int32_t k = int32_t(r2 - r3 * uint32_t(g.n[2]));
-  uint32_t t = dv.div(r3);
-  int32_t v = int32_t(r3 - t * uint32_t(nvars));
+  const uint32_t tl = dv.div(r3);
+  int32_t v = int32_t(r3 - tl * uint32_t(nvars));
+  const int32_t t = t0 + int32_t(tl);  // transfers t0 .. of this launch
   const double* sb = work + s4 * g.sblocks[t] + s3 * v;
   double acc = stencil_value(g, sb, s0, s1, s2, i, j, k);
   acc = scaled(acc, factors, g.factorcol, nvars, v);
