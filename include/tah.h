@@ -354,7 +354,9 @@ typedef struct tah_plan_options {
   int32_t fuse;            /* must be 0: the fused copy-and-interpolation launch is switched off */
   int32_t copy_wgs;        /* copies from a bounded grid of this many workgroups (a multiple of 8), or 0 */
   int32_t interp_pairs;    /* merged interpolations over variable pairs where the layout allows (1, default) */
-  int32_t reserved_[1];
+  int32_t lds;             /* interpolation from LDS copies of its source boxes, chunk by chunk, where the
+                              layout holds variables innermost in even number: bit 0 restriction, bit 1
+                              prolongation's unit classes of 64 units or more (0, the default: unit kernels) */
 } tah_plan_options;
 tah_status tah_plan_get_options(const tah_plan* p, tah_plan_options* out);
 tah_status tah_plan_set_options(tah_plan* p, const tah_plan_options* o);

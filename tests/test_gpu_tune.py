@@ -33,17 +33,18 @@ def test_every_option_combination_fills_alike(which):
     torch.cuda.synchronize()
     ref = fs.work.clone()
     plan = s.plan(fs.layout)
-    for fork, merge, defer, cpt, wgs, ip in itertools.product((0, 1), (0, 1), (0, 1), (4, 8, 16), (0, 64, 512),
-                                                              (0, 1)):
+    for fork, merge, defer, cpt, wgs, ip, lds in itertools.product((0, 1), (0, 1), (0, 1), (4, 8, 16),
+                                                                   (0, 64, 512), (0, 1), (0, 1, 2, 3)):
         print(f"[tune] {which} fork={fork} merge={merge} defer={defer} copy_per_thread={cpt} copy_wgs={wgs} "
-              f"interp_pairs={ip}", flush=True)
+              f"interp_pairs={ip} lds={lds}", flush=True)
         plan.set_options(fork=fork, merge=merge, fuse=0, defer=defer, copy_per_thread=cpt, copy_wgs=wgs,
-                         interp_pairs=ip)
+                         interp_pairs=ip, lds=lds)
         fs.work.copy_(init)
         fill_ghosts(fs, s)
         torch.cuda.synchronize()
-        assert torch.equal(fs.work.view(torch.int64), ref.view(torch.int64)), (fork, merge, defer, cpt, wgs, ip)
-    plan.set_options(fork=0, merge=1, fuse=0, defer=1, copy_per_thread=8, copy_wgs=0, interp_pairs=1)
+        assert torch.equal(fs.work.view(torch.int64), ref.view(torch.int64)), (fork, merge, defer, cpt, wgs, ip,
+                                                                                lds)
+    plan.set_options(fork=0, merge=1, fuse=0, defer=1, copy_per_thread=8, copy_wgs=0, interp_pairs=1, lds=0)
 
 
 def test_tune_caches_by_signature(tmp_path):
@@ -52,7 +53,7 @@ def test_tune_caches_by_signature(tmp_path):
     opts, ms = tune(fs, s, path)
     assert ms is not None and ms > 0
     lines = path.read_text().splitlines()
-    assert len(lines) == 1 and lines[0].startswith("tah-tune-3 ")
+    assert len(lines) == 1 and lines[0].startswith("tah-tune-4 ")
     s.plan(fs.layout).set_options(fork=1 - opts["fork"])
     opts2, ms2 = tune(fs, s, path)
     assert ms2 is None and opts2 == opts

@@ -55,6 +55,7 @@ struct Batch {
   std::vector<Job> jobs;
   Job* djobs = nullptr;
   Tile* dtiles = nullptr;
+  LdsTile* dltiles = nullptr;
   std::vector<std::vector<Launch>> launches;  // per stage position
   std::vector<hipEvent_t> events;
   int device = 0;
@@ -62,6 +63,7 @@ struct Batch {
   ~Batch() {
     if (djobs) (void)hipFree(djobs);
     if (dtiles) (void)hipFree(dtiles);
+    if (dltiles) (void)hipFree(dltiles);
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
   }
 };
@@ -95,22 +97,40 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
     for (size_t k = i + 1; k < b->works.size(); ++k)
       if (b->works[i] == b->works[k]) fail("two fills of one batch share a working array");
   std::vector<Tile> tiles;
+  std::vector<LdsTile> ltiles;
   for (size_t pos = 0; pos < npos; ++pos) {
     std::map<int, std::vector<Tile>> bycls;
+    std::map<int, std::vector<LdsTile>> ldscls;  // LDS-staged classes
+    std::map<int, int32_t> ldsbytes;
     for (int32_t i = 0; i < n; ++i) {
       tah_plan* p = b->plans[i];
       if (pos >= p->launches.size()) continue;
-      for (const Launch& L : p->launches[pos])
+      for (const Launch& L : p->launches[pos]) {
+        if (L.lds > 0) {
+          // the chunk index stays the plan's: its job points at its chunks
+          for (int64_t t = L.first; t < L.first + L.count; ++t) {
+            LdsTile tl = p->lds_tiles[size_t(t)];
+            tl.job = i;
+            ldscls[L.cls].push_back(tl);
+          }
+          ldsbytes[L.cls] = std::max(ldsbytes[L.cls], L.lds);
+          continue;
+        }
         for (int64_t t = L.first; t < L.first + L.count; ++t) {
           Tile tl = p->tiles[t];
           tl.job = i;
           bycls[L.cls].push_back(tl);
         }
+      }
     }
     std::vector<Launch> ls;
     for (auto& kv : bycls) {
       ls.push_back(Launch{kv.first, int64_t(tiles.size()), int64_t(kv.second.size())});
       tiles.insert(tiles.end(), kv.second.begin(), kv.second.end());
+    }
+    for (auto& kv : ldscls) {
+      ls.push_back(Launch{kv.first, int64_t(ltiles.size()), int64_t(kv.second.size()), 0, 0, ldsbytes[kv.first]});
+      ltiles.insert(ltiles.end(), kv.second.begin(), kv.second.end());
     }
     std::stable_sort(ls.begin(), ls.end(), [](const Launch& x, const Launch& y) { return x.count > y.count; });
     b->launches.push_back(std::move(ls));
@@ -122,6 +142,10 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
     HIPCHECK(hipMalloc(&b->dtiles, sizeof(Tile) * tiles.size()));
     HIPCHECK(hipMemcpy(b->dtiles, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice));
   }
+  if (!ltiles.empty()) {
+    HIPCHECK(hipMalloc(&b->dltiles, sizeof(LdsTile) * ltiles.size()));
+    HIPCHECK(hipMemcpy(b->dltiles, ltiles.data(), sizeof(LdsTile) * ltiles.size(), hipMemcpyHostToDevice));
+  }
   for (int i = 0; i < 8; ++i) {
     hipEvent_t e;
     HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -131,7 +155,7 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
 }
 
 void run_batch(tah_ctx* ctx, Batch& b, hipStream_t s) {
-  for (const auto& ls : b.launches) run_launches(ctx, ls, s, b.djobs, b.dtiles, b.events.data(), b.fork);
+  for (const auto& ls : b.launches) run_launches(ctx, ls, s, b.djobs, b.dtiles, b.dltiles, b.events.data(), b.fork);
 }
 
 // Every plan of the batch alive and in the state the batch was built from.

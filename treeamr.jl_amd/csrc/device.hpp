@@ -64,11 +64,82 @@ struct alignas(16) DevWClass {
   int32_t pad;
 };
 
+// A chunk of an interpolating group's target box whose transfers a
+// workgroup fills from an LDS copy of the chunk's source box: target rows
+// tlo .. tlo+tn-1 per dimension (0-based within the group's box), read
+// from source rows slo .. slo+sext-1 of the source block (the union of the
+// rows' windows), one pair of variables at a time.
+// In LDS each dimension of the box is split into its even and its odd rows:
+// source row x along dimension d sits at X = (x & 1)*half[d] + (x >> 1), and
+// point (X0, X1, X2) is 16-byte pair X0 + X1*pitch[0] + X2*pitch[1], both
+// pitches odd. Every window of the chunk starts an even number of rows into
+// the box (the windows of neighbouring targets two rows apart), so
+// neighbouring targets' windows start in neighbouring pairs of LDS. A
+// thread's targets are numbered with dimension ord[0] fastest, then ord[1],
+// then ord[2] (ord[0] a dimension of at least 8 rows where there is one):
+// the lanes that LDS serves together then read different banks.
+struct alignas(16) LdsChunk {
+  int32_t group;
+  int32_t tlo[3], tn[3];
+  int32_t slo[3], sext[3];
+  int32_t points;    // tn0*tn1*tn2
+  FastDiv dt[2];     // division by tn[ord[0]], tn[ord[1]]
+  FastDiv dsx[2];    // division by sext0, sext1
+  int32_t staged;    // sext0*sext1*sext2 (points copied)
+  int32_t ord[3];
+  int32_t half[3];   // (sext[d] + 1) / 2
+  int32_t pitch[2];
+  int32_t slots;     // 16-byte slots of LDS the box takes
+};
+
+// Units of one class in a chunk of a prolongation's box, for one wave:
+// units[unit0 .. unit0+nu-1] of the plan's LDS unit list, each two int32:
+// the rows (ra0, ra1, rb0, rb1) and (rc0, rc1) of the unit's targets along
+// x, y, z as bytes (a second row 255 where the unit has one row there).
+struct alignas(16) LdsWave {
+  int32_t wclass;  // index into the job's DevWClass list
+  int32_t unit0;
+  int32_t nu;
+  int32_t pad;
+};
+
+// A chunk of a prolongation's box: target rows as in LdsChunk, its source
+// box slo .. slo+sext-1 per dimension (point (x, y, z) at 16-byte pair
+// x + y*pitch[0] + z*pitch[1] of LDS, odd pitches), and its class waves
+// waves[wave0 .. wave0+nwave-1].
+struct alignas(16) LdsUnitChunk {
+  int32_t group;
+  int32_t slo[3], sext[3];
+  int32_t pitch[2];
+  int32_t slots;
+  int32_t staged;
+  int32_t wave0, nwave;
+  FastDiv dsx[2];  // division by sext0, sext1
+  int32_t pad[3];
+};
+
+// The most LDS an LDS-staged workgroup takes (a chunk's source box, as
+// 16-byte pairs of variables).
+constexpr int kLdsMaxBytes = 64 * 1024;
+
+// One workgroup's share of an LDS-staged launch: chunk `chunk` (of the job's
+// chunk list) of transfer `transfer`, variables v0 and v0+1.
+struct alignas(16) LdsTile {
+  int32_t job;
+  int32_t chunk;
+  int32_t transfer;
+  int32_t v0;
+};
+
 // One fill of one working array. Element (i, j, k, v) of block b is at
 // stride[0]*i + stride[1]*j + stride[2]*k + stride[3]*v + stride[4]*b.
 struct alignas(16) Job {
   const DevGroup* groups;
   const DevWClass* wclasses;
+  const LdsChunk* chunks;  // the plan's LDS-staged chunks
+  const LdsUnitChunk* uchunks;  // ... of prolongation, their class waves and units
+  const LdsWave* waves;
+  const uint32_t* units;
   double* work;        // the targets' array
   const double* src;   // the sources' array: work, or a regrid's old array
   const double* factors;

@@ -108,6 +108,32 @@ void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, cons
   HIPCHECK(hipGetLastError());
 }
 
+void launch_lds(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const LdsTile* tiles, int32_t bytes) {
+  if (bytes <= 0 || bytes > kLdsMaxBytes) fail("bad LDS size for an LDS-staged launch");
+  const dim3 grid(ntiles), blk(kThreads);
+  const size_t sh = size_t(bytes);
+  if (cls >= kClsLdsProlong) {
+    const dim3 pblk(kLdsProlongThreads);
+    switch (cls - kClsLdsProlong) {
+      case 0: hipLaunchKernelGGL((interp_lds_prolong_kernel<2>), grid, pblk, sh, s, jobs, tiles); break;
+      case 1: hipLaunchKernelGGL((interp_lds_prolong_kernel<3>), grid, pblk, sh, s, jobs, tiles); break;
+      case 2: hipLaunchKernelGGL((interp_lds_prolong_kernel<4>), grid, pblk, sh, s, jobs, tiles); break;
+      case 3: hipLaunchKernelGGL((interp_lds_prolong_kernel<5>), grid, pblk, sh, s, jobs, tiles); break;
+      case 4: hipLaunchKernelGGL((interp_lds_prolong_kernel<6>), grid, pblk, sh, s, jobs, tiles); break;
+      default: fail("no LDS-staged prolongation kernel for class " + std::to_string(cls));
+    }
+    HIPCHECK(hipGetLastError());
+    return;
+  }
+  switch (cls - kClsLds) {
+    case 5: hipLaunchKernelGGL((interp_lds_kernel<2>), grid, blk, sh, s, jobs, tiles); break;
+    case 6: hipLaunchKernelGGL((interp_lds_kernel<4>), grid, blk, sh, s, jobs, tiles); break;
+    case 7: hipLaunchKernelGGL((interp_lds_kernel<6>), grid, blk, sh, s, jobs, tiles); break;
+    default: fail("no LDS-staged kernel for class " + std::to_string(cls));
+  }
+  HIPCHECK(hipGetLastError());
+}
+
 int class_tile_elems(int cls) {
   if (cls > kClsCopyLaunch && cls <= kClsCopyLaunch + 16) return kThreads * (cls - kClsCopyLaunch);
   return cls == kClsGeneric ? kGenericTile : kThreads;
@@ -400,7 +426,7 @@ tah_status tah_regrid_transfer(tah_plan* p, const tah_layout* old_layout, const 
         p->events.push_back(e);
       }
     for (size_t k = 0; k < p->launches.size(); ++k)
-      run_launches(p->ctx, p->launches[k], s, k == 0 ? p->djob : p->djob2, p->dtiles, p->events.data(),
+      run_launches(p->ctx, p->launches[k], s, k == 0 ? p->djob : p->djob2, p->dtiles, p->dlds_tiles, p->events.data(),
                    p->fork != 0);
     plan_end(p, s);
   });
@@ -423,7 +449,7 @@ tah_status tah_fill(tah_plan* p, double* work, const double* factors, void* stre
         p->events.push_back(e);
       }
     for (const auto& sg : p->launches)
-      run_launches(p->ctx, sg, s, p->djob, p->dtiles, p->events.data(), p->fork != 0);
+      run_launches(p->ctx, sg, s, p->djob, p->dtiles, p->dlds_tiles, p->events.data(), p->fork != 0);
     plan_end(p, s);
   });
 }
@@ -455,7 +481,9 @@ tah_status tah_fill_timed(tah_plan* p, double* work, const double* factors, void
       HIPCHECK(hipEventRecord(ev[0], s));
       for (const auto& sg : p->launches)
         for (const Launch& L : sg) {
-          launch_class(L.cls, uint32_t(L.count), s, p->djob, p->dtiles + L.first);
+          Launch serial = L;  // timed one by one: the copies' bounded grid is not taken
+          serial.grid = 0;
+          launch_one(serial, s, p->djob, p->dtiles, p->dlds_tiles);
           HIPCHECK(hipEventRecord(ev[k + 1], s));
           cls[k] = L.cls;
           values[k] = L.values;
@@ -484,6 +512,10 @@ tah_status tah_kernel_class_name(int32_t cls, char* buf, int32_t len) {
     } else if (cls >= kClsMergedPair && cls - kClsMergedPair < int(kInterpTable.size())) {
       const auto& e = kInterpTable[cls - kClsMergedPair];
       name = "interp_pair_kernel<" + std::to_string(e[0]) + ", " + std::to_string(e[1]) + ">";
+    } else if (cls >= kClsLdsProlong && cls - kClsLdsProlong < int(kInterpTable.size())) {
+      name = "interp_lds_prolong_kernel<" + std::to_string(kInterpTable[cls - kClsLdsProlong][0]) + ">";
+    } else if (cls >= kClsLds && cls - kClsLds < int(kInterpTable.size())) {
+      name = "interp_lds_kernel<" + std::to_string(kInterpTable[cls - kClsLds][0]) + ">";
     } else if (cls >= kClsMerged && cls - kClsMerged < int(kInterpTable.size())) {
       const auto& e = kInterpTable[cls - kClsMerged];
       name = "interp_all_kernel<" + std::to_string(e[0]) + ", " + std::to_string(e[1]) + ">";

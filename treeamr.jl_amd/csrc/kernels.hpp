@@ -497,4 +497,268 @@ interp_pair_kernel(const Job* __restrict__ jobs, const Tile* __restrict__ tiles)
 
 #endif  // TAH_DEFINE_KERNELS
 
+// ---------------------------------------------------------------------------
+// Interpolation from an LDS copy of the source box (restriction and
+// prolongation of order P in every dimension, variables innermost in even
+// number). A workgroup takes one chunk of one transfer for one pair of
+// variables: it copies the chunk's source box into LDS once, with 16-byte
+// loads in the array's own order, and then every thread computes targets of
+// the chunk for both variables, reading its taps from LDS. Each source value
+// thus crosses the vector memory path once per chunk instead of once per
+// tap, and neighbouring targets' overlapping windows are read from LDS.
+// A target's weight is formed as the reference forms it, ((w1*w2)*w3) from
+// the three one-dimensional stencils, and its taps are summed in the
+// reference's order (m3, m2, m1 ascending, m1 innermost) from acc = +0 with
+// acc += w*u, every operation rounded on its own: bit for bit the other
+// kernels' results.
+
+#ifdef TAH_DEFINE_KERNELS
+template <int P>
+__global__ void __launch_bounds__(kThreads) interp_lds_kernel(const Job* __restrict__ jobs,
+                                                              const LdsTile* __restrict__ tiles) {
+  typedef double pair_t __attribute__((ext_vector_type(2)));
+  extern __shared__ __align__(16) double lds[];
+  const LdsTile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  const Job& job = jobs[tile.job];
+  const LdsChunk& ch = job.chunks[tile.chunk];
+  const DevGroup& g = job.groups[ch.group];
+  const int32_t s0 = int32_t(job.stride[0]), s1 = int32_t(job.stride[1]), s2 = int32_t(job.stride[2]);
+  const int64_t s4 = job.stride[4];
+  const int32_t e0 = ch.sext[0], e1 = ch.sext[1];
+  const int32_t p1 = ch.pitch[0], p2 = ch.pitch[1];
+  const int32_t h0 = ch.half[0], h1 = ch.half[1], h2 = ch.half[2];
+  // Stage the source box: point (x, y, z) of the box at pair X + Y*p1 + Z*p2,
+  // X = (x & 1)*h0 + (x >> 1) and so on (even rows first, then odd ones).
+  {
+    const double* __restrict__ src = job.src + s4 * g.sblocks[tile.transfer] +
+                                     (tile.v0 + s0 * ch.slo[0] + s1 * ch.slo[1] + s2 * ch.slo[2]);
+    const uint32_t n = uint32_t(ch.staged);
+    for (uint32_t q = threadIdx.x; q < n; q += kThreads) {
+      const uint32_t r = ch.dsx[0].div(q);
+      const int32_t x = int32_t(q - r * uint32_t(e0));
+      const uint32_t z = ch.dsx[1].div(r);
+      const int32_t y = int32_t(r - z * uint32_t(e1));
+      const double* p = src + (s2 * int32_t(z) + s1 * y + s0 * x);
+      pair_t v;
+      if (job.pairs) {
+        v = *reinterpret_cast<const pair_t*>(p);
+      } else {
+        v.x = p[0];
+        v.y = p[1];
+      }
+      const int32_t zi = int32_t(z);
+      reinterpret_cast<pair_t*>(lds)[((x & 1) * h0 + (x >> 1)) + ((y & 1) * h1 + (y >> 1)) * p1 +
+                                     ((zi & 1) * h2 + (zi >> 1)) * p2] = v;
+    }
+  }
+  __syncthreads();
+  const pair_t* __restrict__ box = reinterpret_cast<const pair_t*>(lds);
+  double* __restrict__ dst = job.work + s4 * g.tblocks[tile.transfer] + tile.v0;
+  const int32_t nvars = job.nvars;
+  const int32_t d0 = ch.ord[0], d1 = ch.ord[1];
+  const int32_t n0 = ch.tn[d0], n1 = ch.tn[d1];
+  for (uint32_t q = threadIdx.x; q < uint32_t(ch.points); q += kThreads) {
+    // target rows (i, j, k) of the group's box: a, b, c along ord[0..2]
+    const uint32_t r = ch.dt[0].div(q);
+    const int32_t a = int32_t(q - r * uint32_t(n0));
+    const uint32_t c = ch.dt[1].div(r);
+    const int32_t b = int32_t(r - c * uint32_t(n1));
+    const int32_t i = ch.tlo[0] + (d0 == 0 ? a : d1 == 0 ? b : int32_t(c));
+    const int32_t j = ch.tlo[1] + (d0 == 1 ? a : d1 == 1 ? b : int32_t(c));
+    const int32_t k = ch.tlo[2] + (d0 == 2 ? a : d1 == 2 ? b : int32_t(c));
+    double w0[P], w1[P], w2[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      w0[m] = g.weights[0][P * i + m];
+      w1[m] = g.weights[1][P * j + m];
+      w2[m] = g.weights[2][P * k + m];
+    }
+    const int32_t o0 = g.srcstart[0][i] - ch.slo[0], o1 = g.srcstart[1][j] - ch.slo[1],
+                  o2 = g.srcstart[2][k] - ch.slo[2];
+    // every window starts an even number of rows into the box (the chunk
+    // builder checks), so tap m of a dimension is at half (m & 1), row
+    // o/2 + m/2 there
+    const pair_t* __restrict__ base = box + ((o2 >> 1) * p2 + (o1 >> 1) * p1 + (o0 >> 1));
+    double ax = 0.0, ay = 0.0;
+#pragma unroll
+    for (int m2 = 0; m2 < P; ++m2) {
+      const double c2 = w2[m2];
+#pragma unroll
+      for (int m1 = 0; m1 < P; ++m1) {
+        const double c1 = w1[m1];
+        const pair_t* __restrict__ row =
+            base + (((m2 & 1) * h2 + (m2 >> 1)) * p2 + ((m1 & 1) * h1 + (m1 >> 1)) * p1);
+        const pair_t* __restrict__ odd = row + h0;
+#pragma unroll
+        for (int m0 = 0; m0 < P; ++m0) {
+          const double w = __dmul_rn(__dmul_rn(w0[m0], c1), c2);
+          const pair_t u = (m0 & 1) ? odd[m0 >> 1] : row[m0 >> 1];
+          ax = __dadd_rn(ax, __dmul_rn(w, u.x));
+          ay = __dadd_rn(ay, __dmul_rn(w, u.y));
+        }
+        // Row by row: without this the compiler hoists every row's LDS
+        // reads to the top and keeps them all live, spilling them.
+        asm volatile("" : "+v"(ax), "+v"(ay));
+      }
+    }
+    double* __restrict__ t = dst + (s0 * (g.tfirst[0] + i) + s1 * (g.tfirst[1] + j) + s2 * (g.tfirst[2] + k));
+    pair_t out;
+    out.x = scaled(ax, job.factors, g.factorcol, nvars, tile.v0);
+    out.y = scaled(ay, job.factors, g.factorcol, nvars, tile.v0 + 1);
+    if (job.pairs) {
+      *reinterpret_cast<pair_t*>(t) = out;
+    } else {
+      t[0] = out.x;
+      t[1] = out.y;
+    }
+  }
+}
+#endif  // TAH_DEFINE_KERNELS
+
+// Prolongation from an LDS copy of a chunk's coarse source box, by units. A
+// lane computes one unit (R0 x R1 x R2 fine targets sharing a coarse
+// window) for a pair of variables, exactly as interp_body does -- the same
+// taps in the same order, the weight products of its class's table W read
+// by scalar loads (each wave holds units of one class) -- but reads the
+// window's rows from LDS.
+template <int P, int R0, int R1, int R2>
+__device__ __forceinline__ void lds_unit_body(const Job& job, const DevGroup& g, const DevWClass& wc,
+                                              const LdsUnitChunk& ch, const double* __restrict__ lds, uint32_t pk0,
+                                              uint32_t pk1, int32_t tr, int32_t v0) {
+  typedef double pair_t __attribute__((ext_vector_type(2)));
+  const int32_t ra[2] = {int32_t(pk0 & 255u), int32_t((pk0 >> 8) & 255u)};
+  const int32_t rb[2] = {int32_t((pk0 >> 16) & 255u), int32_t(pk0 >> 24)};
+  const int32_t rc[2] = {int32_t(pk1 & 255u), int32_t((pk1 >> 8) & 255u)};
+  const int32_t p1 = ch.pitch[0], p2 = ch.pitch[1];
+  const pair_t* __restrict__ src = reinterpret_cast<const pair_t*>(lds) +
+                                   ((g.srcstart[2][rc[0]] - ch.slo[2]) * p2 + (g.srcstart[1][rb[0]] - ch.slo[1]) * p1 +
+                                    (g.srcstart[0][ra[0]] - ch.slo[0]));
+  uint64_t wbase = reinterpret_cast<uint64_t>(uniform(wc.W));
+  double acc[2][R0][R1][R2];
+#pragma unroll
+  for (int n = 0; n < 2; ++n)
+#pragma unroll
+    for (int a = 0; a < R0; ++a)
+#pragma unroll
+      for (int b = 0; b < R1; ++b)
+#pragma unroll
+        for (int c = 0; c < R2; ++c) acc[n][a][b][c] = 0.0;
+  int idx = 0;
+#pragma unroll
+  for (int m2 = 0; m2 < P; ++m2) {
+#pragma unroll
+    for (int m1 = 0; m1 < P; ++m1) {
+      asm volatile("" : "+s"(wbase));
+      const __attribute__((address_space(4))) double* __restrict__ W =
+          (const __attribute__((address_space(4))) double*)(wbase) + idx;
+      const pair_t* __restrict__ rowp = src + (m2 * p2 + m1 * p1);
+      double row[2][P];
+#pragma unroll
+      for (int m0 = 0; m0 < P; ++m0) {
+        const pair_t x = rowp[m0];
+        row[0][m0] = x.x;
+        row[1][m0] = x.y;
+      }
+      // the order of interp_body with SHIFT 0: targets (c, b, a), then m0
+      int widx = 0;
+#pragma unroll
+      for (int c = 0; c < R2; ++c)
+#pragma unroll
+        for (int b = 0; b < R1; ++b)
+#pragma unroll
+          for (int a = 0; a < R0; ++a)
+#pragma unroll
+            for (int m0 = 0; m0 < P; ++m0) {
+              const double w = W[widx];
+#pragma unroll
+              for (int n = 0; n < 2; ++n) acc[n][a][b][c] = __dadd_rn(acc[n][a][b][c], __dmul_rn(w, row[n][m0]));
+              ++widx;
+            }
+      idx += widx;
+#pragma unroll
+      for (int n = 0; n < 2; ++n)
+#pragma unroll
+        for (int a = 0; a < R0; ++a)
+#pragma unroll
+          for (int b = 0; b < R1; ++b)
+#pragma unroll
+            for (int c = 0; c < R2; ++c) asm volatile("" : "+v"(acc[n][a][b][c]));
+    }
+  }
+  const int32_t s0 = int32_t(job.stride[0]), s1 = int32_t(job.stride[1]), s2 = int32_t(job.stride[2]);
+  double* __restrict__ dst = job.work + job.stride[4] * g.tblocks[tr] + v0;
+  const int32_t nvars = job.nvars;
+#pragma unroll
+  for (int c = 0; c < R2; ++c)
+#pragma unroll
+    for (int b = 0; b < R1; ++b)
+#pragma unroll
+      for (int a = 0; a < R0; ++a) {
+        double* q = dst + (s0 * (g.tfirst[0] + ra[a]) + s1 * (g.tfirst[1] + rb[b]) + s2 * (g.tfirst[2] + rc[c]));
+        pair_t out;
+        out.x = scaled(acc[0][a][b][c], job.factors, g.factorcol, nvars, v0);
+        out.y = scaled(acc[1][a][b][c], job.factors, g.factorcol, nvars, v0 + 1);
+        if (job.pairs) {
+          *reinterpret_cast<pair_t*>(q) = out;
+        } else {
+          q[0] = out.x;
+          q[1] = out.y;
+        }
+      }
+}
+
+constexpr int kLdsProlongThreads = 128;  // two waves: a chunk's class waves in turn
+
+#ifdef TAH_DEFINE_KERNELS
+template <int P>
+__global__ void __launch_bounds__(kLdsProlongThreads) __attribute__((amdgpu_waves_per_eu(kPairWaves, 8)))
+interp_lds_prolong_kernel(const Job* __restrict__ jobs, const LdsTile* __restrict__ tiles) {
+  typedef double pair_t __attribute__((ext_vector_type(2)));
+  extern __shared__ __align__(16) double lds[];
+  const LdsTile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  const Job& job = jobs[tile.job];
+  const LdsUnitChunk& ch = job.uchunks[tile.chunk];
+  const DevGroup& g = job.groups[ch.group];
+  {
+    const int32_t s0 = int32_t(job.stride[0]), s1 = int32_t(job.stride[1]), s2 = int32_t(job.stride[2]);
+    const double* __restrict__ src = job.src + job.stride[4] * g.sblocks[tile.transfer] +
+                                     (tile.v0 + s0 * ch.slo[0] + s1 * ch.slo[1] + s2 * ch.slo[2]);
+    const int32_t e0 = ch.sext[0], e1 = ch.sext[1], p1 = ch.pitch[0], p2 = ch.pitch[1];
+    for (uint32_t q = threadIdx.x; q < uint32_t(ch.staged); q += kLdsProlongThreads) {
+      const uint32_t r = ch.dsx[0].div(q);
+      const int32_t x = int32_t(q - r * uint32_t(e0));
+      const uint32_t z = ch.dsx[1].div(r);
+      const int32_t y = int32_t(r - z * uint32_t(e1));
+      const double* p = src + (s2 * int32_t(z) + s1 * y + s0 * x);
+      pair_t v;
+      if (job.pairs) {
+        v = *reinterpret_cast<const pair_t*>(p);
+      } else {
+        v.x = p[0];
+        v.y = p[1];
+      }
+      reinterpret_cast<pair_t*>(lds)[x + y * p1 + int32_t(z) * p2] = v;
+    }
+  }
+  __syncthreads();
+  const int32_t hw = int32_t(threadIdx.x) / 64, lane = int32_t(threadIdx.x) % 64;
+  for (int32_t w = hw; w < ch.nwave; w += kLdsProlongThreads / 64) {
+    const LdsWave& wv = job.waves[ch.wave0 + w];
+    if (lane >= wv.nu) continue;
+    const uint32_t pk0 = job.units[2 * (wv.unit0 + lane)], pk1 = job.units[2 * (wv.unit0 + lane) + 1];
+    const DevWClass& wc = job.wclasses[wv.wclass];
+    switch (uniform(wc.rbits)) {
+      case 0: lds_unit_body<P, 1, 1, 1>(job, g, wc, ch, lds, pk0, pk1, tile.transfer, tile.v0); break;
+      case 1: lds_unit_body<P, 2, 1, 1>(job, g, wc, ch, lds, pk0, pk1, tile.transfer, tile.v0); break;
+      case 2: lds_unit_body<P, 1, 2, 1>(job, g, wc, ch, lds, pk0, pk1, tile.transfer, tile.v0); break;
+      case 3: lds_unit_body<P, 2, 2, 1>(job, g, wc, ch, lds, pk0, pk1, tile.transfer, tile.v0); break;
+      case 4: lds_unit_body<P, 1, 1, 2>(job, g, wc, ch, lds, pk0, pk1, tile.transfer, tile.v0); break;
+      case 5: lds_unit_body<P, 2, 1, 2>(job, g, wc, ch, lds, pk0, pk1, tile.transfer, tile.v0); break;
+      case 6: lds_unit_body<P, 1, 2, 2>(job, g, wc, ch, lds, pk0, pk1, tile.transfer, tile.v0); break;
+      default: lds_unit_body<P, 2, 2, 2>(job, g, wc, ch, lds, pk0, pk1, tile.transfer, tile.v0); break;
+    }
+  }
+}
+#endif  // TAH_DEFINE_KERNELS
+
 }  // namespace tah

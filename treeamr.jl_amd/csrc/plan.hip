@@ -56,6 +56,139 @@ void pair_rows(const std::vector<int32_t>& ss, int shift, std::vector<std::array
   }
 }
 
+// The chunks of group g (index gi) for LDS-staged launches: each dimension
+// of more than 4 target rows cut into runs of at most c rows (8 where two
+// dimensions are that long, so that a face's chunk holds 3*8*8 targets; 16
+// where one is), the others whole. A chunk's source box is the union of its
+// rows' windows. None where a chunk would hold fewer than 64 targets (a
+// corner: the unit kernels fill it better) or its box would not fit in LDS
+// as pairs of variables.
+std::vector<LdsChunk> lds_chunks_of(const PlanGroup& g, int32_t gi) {
+  std::vector<LdsChunk> out;
+  int nlong = 0;
+  for (int d = 0; d < 3; ++d) nlong += g.n[d] > 4 ? 1 : 0;
+  const int c = nlong >= 2 ? 8 : 16;
+  int C[3];
+  int64_t per = 1;
+  for (int d = 0; d < 3; ++d) {
+    C[d] = g.n[d] > 4 ? c : std::max(1, g.n[d]);
+    per *= std::min(C[d], std::max(1, g.n[d]));
+  }
+  if (per < 64) return out;
+  for (int z0 = 0; z0 < g.n[2]; z0 += C[2])
+    for (int y0 = 0; y0 < g.n[1]; y0 += C[1])
+      for (int x0 = 0; x0 < g.n[0]; x0 += C[0]) {
+        LdsChunk ch{};
+        ch.group = gi;
+        const int lo[3] = {x0, y0, z0};
+        int64_t staged = 1;
+        for (int d = 0; d < 3; ++d) {
+          ch.tlo[d] = lo[d];
+          ch.tn[d] = std::min(C[d], g.n[d] - lo[d]);
+          int32_t smin = INT32_MAX, smax = INT32_MIN;
+          for (int r = lo[d]; r < lo[d] + ch.tn[d]; ++r) {
+            smin = std::min(smin, g.srcstart[d][size_t(r)]);
+            smax = std::max(smax, g.srcstart[d][size_t(r)]);
+          }
+          ch.slo[d] = smin;
+          ch.sext[d] = smax + g.order[d] - smin;
+          staged *= ch.sext[d];
+        }
+        // every window an even number of rows into the box, for the LDS
+        // layout of even rows, then odd ones
+        for (int d = 0; d < 3; ++d)
+          for (int r = ch.tlo[d]; r < ch.tlo[d] + ch.tn[d]; ++r)
+            if ((g.srcstart[d][size_t(r)] - ch.slo[d]) % 2 != 0) return {};
+        for (int d = 0; d < 3; ++d) ch.half[d] = (ch.sext[d] + 1) / 2;
+        // odd pitches, so that rows and planes apart start in other banks
+        ch.pitch[0] = (2 * ch.half[0]) | 1;
+        ch.pitch[1] = (2 * ch.half[1] * ch.pitch[0]) | 1;
+        ch.slots = 2 * ch.half[2] * ch.pitch[1];
+        if (16 * int64_t(ch.slots) > kLdsMaxBytes) return {};
+        ch.points = ch.tn[0] * ch.tn[1] * ch.tn[2];
+        ch.staged = int32_t(staged);
+        // threads' targets fastest along the first dimension of 8 rows or more
+        int fast = 0;
+        while (fast < 2 && ch.tn[fast] < 8) ++fast;
+        if (ch.tn[fast] < 8) fast = 0;
+        ch.ord[0] = fast;
+        ch.ord[1] = fast == 0 ? 1 : 0;
+        ch.ord[2] = 3 - ch.ord[0] - ch.ord[1];
+        ch.dt[0] = FastDiv(uint32_t(ch.tn[ch.ord[0]]));
+        ch.dt[1] = FastDiv(uint32_t(ch.tn[ch.ord[1]]));
+        ch.dsx[0] = FastDiv(uint32_t(ch.sext[0]));
+        ch.dsx[1] = FastDiv(uint32_t(ch.sext[1]));
+        out.push_back(ch);
+      }
+  return out;
+}
+
+// The prolongation chunks of group g (index gi) over its unit classes cls:
+// their units binned by 16 target rows per dimension (by each unit's first
+// rows); a bin is a chunk whose source box is the union of its units'
+// windows, its units cut per class into waves of at most 64. Appends to
+// chunks, waves and units; false, with nothing appended, where a box would
+// not fit in LDS as pairs of variables.
+bool lds_unit_chunks_of(const tah_plan* p, const PlanGroup& g, int32_t gi, const std::vector<int32_t>& cls,
+                        std::vector<LdsUnitChunk>& chunks, std::vector<LdsWave>& waves,
+                        std::vector<uint32_t>& units) {
+  std::map<std::array<int32_t, 3>, std::vector<std::pair<int32_t, int32_t>>> bins;  // (class, unit)
+  for (int32_t ci : cls) {
+    const PlanWClass& c = p->wclasses[size_t(ci)];
+    for (int32_t k = 0; k < c.nunits(); ++k) {
+      const int32_t* u = &c.units[8 * size_t(k)];
+      bins[{u[0] / 16, u[2] / 16, u[4] / 16}].push_back({ci, k});
+    }
+  }
+  const size_t c0 = chunks.size(), w0 = waves.size(), u0 = units.size();
+  for (const auto& bin : bins) {
+    LdsUnitChunk ch{};
+    ch.group = gi;
+    int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+    for (const auto& cu : bin.second) {
+      const int32_t* u = &p->wclasses[size_t(cu.first)].units[8 * size_t(cu.second)];
+      for (int d = 0; d < 3; ++d) {
+        const int32_t s = g.srcstart[d][size_t(u[2 * d])];
+        lo[d] = std::min(lo[d], s);
+        hi[d] = std::max(hi[d], s);
+      }
+    }
+    for (int d = 0; d < 3; ++d) {
+      ch.slo[d] = lo[d];
+      ch.sext[d] = hi[d] + g.order[d] - lo[d];
+    }
+    ch.pitch[0] = ch.sext[0] | 1;
+    ch.pitch[1] = (ch.sext[1] * ch.pitch[0]) | 1;
+    ch.slots = ch.sext[2] * ch.pitch[1];
+    if (16 * int64_t(ch.slots) > kLdsMaxBytes) {
+      chunks.resize(c0);
+      waves.resize(w0);
+      units.resize(u0);
+      return false;
+    }
+    ch.staged = ch.sext[0] * ch.sext[1] * ch.sext[2];
+    ch.dsx[0] = FastDiv(uint32_t(ch.sext[0]));
+    ch.dsx[1] = FastDiv(uint32_t(ch.sext[1]));
+    ch.wave0 = int32_t(waves.size());
+    std::map<int32_t, std::vector<int32_t>> per;  // class -> its units in the bin
+    for (const auto& cu : bin.second) per[cu.first].push_back(cu.second);
+    for (const auto& cl : per)
+      for (size_t i = 0; i < cl.second.size(); i += 64) {
+        const int32_t nu = int32_t(std::min<size_t>(64, cl.second.size() - i));
+        waves.push_back(LdsWave{cl.first, int32_t(units.size() / 2), nu, 0});
+        for (int32_t j = 0; j < nu; ++j) {
+          const int32_t* u = &p->wclasses[size_t(cl.first)].units[8 * size_t(cl.second[i + size_t(j)])];
+          units.push_back(uint32_t(u[0] & 255) | uint32_t(u[1] & 255) << 8 | uint32_t(u[2] & 255) << 16 |
+                          uint32_t(u[3] & 255) << 24);
+          units.push_back(uint32_t(u[4] & 255) | uint32_t(u[5] & 255) << 8);
+        }
+      }
+    ch.nwave = int32_t(waves.size()) - ch.wave0;
+    chunks.push_back(ch);
+  }
+  return true;
+}
+
 // Bitwise key of a vector of doubles.
 std::vector<uint64_t> bits_of(const double* w, size_t n) {
   std::vector<uint64_t> k(n);
@@ -331,6 +464,7 @@ void build_launches(tah_plan* p) {
   const int32_t nvars = p->layout.nvars;
   const bool vfast = p->layout.stride[3] < p->layout.stride[0];
   p->tiles.clear();
+  p->lds_tiles.clear();
   p->launches.clear();
   // The launch an interpolation class runs in: its own, or with every
   // class of its order and shift (tah_plan::merge).
@@ -348,14 +482,57 @@ void build_launches(tah_plan* p) {
   };
   // The launch a copy or generic group runs in.
   auto group_launch = [&](int cls) { return cls == kClsCopy ? copy_launch(p->copy_per_thread) : cls; };
+  // LDS-staged restriction (tah_plan::lds): the groups whose boxes cut into
+  // chunks of workgroup size, in a layout that holds variables innermost in
+  // even number; their chunks, and the launch class of each such group.
+  const bool pair_capable = st[3] == 1 && nvars % 2 == 0 && st[0] % 2 == 0 && st[1] % 2 == 0 &&
+                            st[2] % 2 == 0 && st[4] % 2 == 0;
+  p->lds_chunks.clear();
+  std::vector<std::array<int32_t, 2>> lds_of(p->groups.size(), {0, 0});  // first chunk, count
+  if ((p->lds & 1) && pair_capable)
+    for (size_t gi = 0; gi < p->groups.size(); ++gi) {
+      const PlanGroup& g = p->groups[gi];
+      if (g.cls == kClsInterp && g.kind == 2) {
+        const auto cs = lds_chunks_of(g, int32_t(gi));
+        lds_of[gi] = {int32_t(p->lds_chunks.size()), int32_t(cs.size())};
+        p->lds_chunks.insert(p->lds_chunks.end(), cs.begin(), cs.end());
+      }
+    }
+  auto lds_launch = [&](const PlanGroup& g) { return kClsLds + interp_index(g.order[0], g.kind == 2 ? 2 : 0); };
+  // LDS-staged prolongation by units: the unit classes of at least 64 units
+  // (a face's inner units); a group's other classes stay with the unit
+  // kernels.
+  p->lds_uchunks.clear();
+  p->lds_waves.clear();
+  p->lds_units.clear();
+  std::vector<char> lds_cls(p->wclasses.size(), 0);
+  std::vector<std::array<int32_t, 2>> ulds_of(p->groups.size(), {0, 0});  // first chunk, count
+  if ((p->lds & 2) && pair_capable)
+    for (size_t gi = 0; gi < p->groups.size(); ++gi) {
+      const PlanGroup& g = p->groups[gi];
+      if (g.cls != kClsInterp || g.kind != 3) continue;
+      std::vector<int32_t> cls;
+      for (int32_t ci : g.wclasses)
+        if (p->wclasses[size_t(ci)].nunits() >= 64) cls.push_back(ci);
+      if (cls.empty()) continue;
+      const size_t c0 = p->lds_uchunks.size();
+      if (!lds_unit_chunks_of(p, g, int32_t(gi), cls, p->lds_uchunks, p->lds_waves, p->lds_units)) continue;
+      ulds_of[gi] = {int32_t(c0), int32_t(p->lds_uchunks.size() - c0)};
+      for (int32_t ci : cls) lds_cls[size_t(ci)] = 1;
+    }
+  auto ulds_launch = [&](const PlanGroup& g) { return kClsLdsProlong + interp_index(g.order[0], 0); };
   for (int k = 0; k < nsg; ++k) {
     std::vector<Launch> ls;
     std::vector<int> classes;
     for (size_t gi = 0; gi < p->groups.size(); ++gi) {
       const PlanGroup& g = p->groups[gi];
       if (g.points == 0 || std::find(run_in[gi].begin(), run_in[gi].end(), k) == run_in[gi].end()) continue;
-      if (g.cls == kClsInterp) {
-        for (int32_t ci : g.wclasses) classes.push_back(launch_of(p->wclasses[ci].cls));
+      if (lds_of[gi][1] > 0) {
+        classes.push_back(lds_launch(g));
+      } else if (g.cls == kClsInterp) {
+        if (ulds_of[gi][1] > 0) classes.push_back(ulds_launch(g));
+        for (int32_t ci : g.wclasses)
+          if (!lds_cls[size_t(ci)]) classes.push_back(launch_of(p->wclasses[ci].cls));
       } else {
         classes.push_back(group_launch(g.cls));
       }
@@ -366,15 +543,76 @@ void build_launches(tah_plan* p) {
       int cls;
       std::vector<std::pair<int32_t, Tile>> tiles;  // (target block, tile)
       int64_t values;
+      std::vector<std::pair<int32_t, LdsTile>> ltiles;  // LDS-staged: (source block, tile)
+      int32_t lds_bytes = 0;
     };
     std::vector<Built> built;
     for (int cls : classes) {
+      if (cls >= kClsLdsProlong) {
+        // Every chunk of a transfer for every pair of variables in turn,
+        // ordered by target block as the unit tiles of prolongation are.
+        Built B{cls, {}, 0, {}, 0};
+        for (size_t gi = 0; gi < p->groups.size(); ++gi) {
+          const PlanGroup& g = p->groups[gi];
+          if (g.points == 0 || ulds_of[gi][1] == 0 || ulds_launch(g) != cls) continue;
+          for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t) {
+            if (run_in[gi][t] != k) continue;
+            for (int32_t c = ulds_of[gi][0]; c < ulds_of[gi][0] + ulds_of[gi][1]; ++c) {
+              const LdsUnitChunk& ch = p->lds_uchunks[size_t(c)];
+              B.lds_bytes = std::max(B.lds_bytes, int32_t(16 * ch.slots));
+              int64_t targets = 0;
+              for (int32_t w = ch.wave0; w < ch.wave0 + ch.nwave; ++w) {
+                const LdsWave& wv = p->lds_waves[size_t(w)];
+                const int rb = (p->wclasses[size_t(wv.wclass)].cls - kClsInterp) % 8;
+                targets += int64_t(wv.nu) * (1 + (rb & 1)) * (1 + ((rb >> 1) & 1)) * (1 + ((rb >> 2) & 1));
+              }
+              for (int32_t v0 = 0; v0 < nvars; v0 += 2) {
+                B.ltiles.push_back({g.tblocks[t], LdsTile{0, c, t, v0}});
+                B.values += 2 * targets;
+              }
+            }
+          }
+        }
+        std::stable_sort(B.ltiles.begin(), B.ltiles.end(),
+                         [](const std::pair<int32_t, LdsTile>& x, const std::pair<int32_t, LdsTile>& y) {
+                           return x.first < y.first;
+                         });
+        if (!B.ltiles.empty()) built.push_back(std::move(B));
+        continue;
+      }
+      if (cls >= kClsLds) {
+        // A transfer's chunks one after another, each for every pair of
+        // variables in turn (the pairs of one point share its cache lines),
+        // ordered by source block as the unit tiles of restriction are.
+        Built B{cls, {}, 0, {}, 0};
+        for (size_t gi = 0; gi < p->groups.size(); ++gi) {
+          const PlanGroup& g = p->groups[gi];
+          if (g.points == 0 || lds_of[gi][1] == 0 || lds_launch(g) != cls) continue;
+          for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t) {
+            if (run_in[gi][t] != k) continue;
+            for (int32_t c = lds_of[gi][0]; c < lds_of[gi][0] + lds_of[gi][1]; ++c) {
+              const LdsChunk& ch = p->lds_chunks[size_t(c)];
+              B.lds_bytes = std::max(B.lds_bytes, int32_t(16 * ch.slots));
+              for (int32_t v0 = 0; v0 < nvars; v0 += 2) {
+                B.ltiles.push_back({g.sblocks[t], LdsTile{0, c, t, v0}});
+                B.values += 2 * int64_t(ch.points);
+              }
+            }
+          }
+        }
+        std::stable_sort(B.ltiles.begin(), B.ltiles.end(),
+                         [](const std::pair<int32_t, LdsTile>& x, const std::pair<int32_t, LdsTile>& y) {
+                           return x.first < y.first;
+                         });
+        if (!B.ltiles.empty()) built.push_back(std::move(B));
+        continue;
+      }
       std::vector<std::pair<int32_t, Tile>> tl;
       int64_t per = class_tile_elems(cls);
       int64_t values = 0;
       for (size_t gi = 0; gi < p->groups.size(); ++gi) {
         const PlanGroup& g = p->groups[gi];
-        if (g.points == 0) continue;
+        if (g.points == 0 || lds_of[gi][1] > 0) continue;
         if (g.cls == kClsInterp) {
           // A transfer's classes one after another, so that a merged
           // launch runs the tiles that share its windows together.
@@ -383,7 +621,7 @@ void build_launches(tah_plan* p) {
           // that tiles stay full across transfers with few units.
           for (int32_t ci : g.wclasses) {
             const PlanWClass& c = p->wclasses[ci];
-            if (launch_of(c.cls) != cls) continue;
+            if (launch_of(c.cls) != cls || lds_cls[size_t(ci)]) continue;
             // targets per unit: 2 along every dimension with R = 2
             const int rb = (c.cls - kClsInterp) % 8;
             const int64_t per_unit = (1 + (rb & 1)) * (1 + ((rb >> 1) & 1)) * (1 + ((rb >> 2) & 1));
@@ -437,9 +675,15 @@ void build_launches(tah_plan* p) {
                        [](const std::pair<int32_t, Tile>& x, const std::pair<int32_t, Tile>& y) {
                          return x.first < y.first;
                        });
-      if (!tl.empty()) built.push_back(Built{cls, std::move(tl), values});
+      if (!tl.empty()) built.push_back(Built{cls, std::move(tl), values, {}, 0});
     }
     for (Built& B : built) {
+      if (!B.ltiles.empty()) {
+        Launch L{B.cls, int64_t(p->lds_tiles.size()), int64_t(B.ltiles.size()), B.values, 0, B.lds_bytes};
+        for (const auto& e : B.ltiles) p->lds_tiles.push_back(e.second);
+        ls.push_back(L);
+        continue;
+      }
       Launch L{B.cls, int64_t(p->tiles.size()), int64_t(B.tiles.size()), B.values, 0};
       if (p->copy_wgs > 0 && B.cls == copy_launch(p->copy_per_thread) && int64_t(B.tiles.size()) >= 8)
         L.grid = int32_t(std::min<int64_t>(p->copy_wgs, int64_t(B.tiles.size())) / 8 * 8);
@@ -452,16 +696,31 @@ void build_launches(tah_plan* p) {
   }
   // A new state of the plan: batches built from an earlier one are stale.
   plan_register(p);
-  if (p->dtiles) {
+  void* old[] = {p->dtiles, p->dchunks, p->dlds_tiles, p->duchunks, p->dwaves, p->dunits};
+  if (std::any_of(std::begin(old), std::end(old), [](void* q) { return q != nullptr; })) {
     // a previous tile list may still be read by queued fills
     HIPCHECK(hipDeviceSynchronize());
-    HIPCHECK(hipFree(p->dtiles));
+    for (void* q : old)
+      if (q) HIPCHECK(hipFree(q));
     p->dtiles = nullptr;
+    p->dchunks = nullptr;
+    p->dlds_tiles = nullptr;
+    p->duchunks = nullptr;
+    p->dwaves = nullptr;
+    p->dunits = nullptr;
   }
-  if (!p->tiles.empty()) {
-    HIPCHECK(hipMalloc(&p->dtiles, sizeof(Tile) * p->tiles.size()));
-    HIPCHECK(hipMemcpy(p->dtiles, p->tiles.data(), sizeof(Tile) * p->tiles.size(), hipMemcpyHostToDevice));
-  }
+  auto upload = [](auto*& dev, const auto& host) {
+    if (host.empty()) return;
+    const size_t bytes = sizeof(host[0]) * host.size();
+    HIPCHECK(hipMalloc(&dev, bytes));
+    HIPCHECK(hipMemcpy(dev, host.data(), bytes, hipMemcpyHostToDevice));
+  };
+  upload(p->dtiles, p->tiles);
+  upload(p->dchunks, p->lds_chunks);
+  upload(p->dlds_tiles, p->lds_tiles);
+  upload(p->duchunks, p->lds_uchunks);
+  upload(p->dwaves, p->lds_waves);
+  upload(p->dunits, p->lds_units);
 }
 
 Job plan_job(const tah_plan* p, double* work, const double* factors, const double* src) {
@@ -469,6 +728,10 @@ Job plan_job(const tah_plan* p, double* work, const double* factors, const doubl
   std::memset(&job, 0, sizeof(Job));
   job.groups = p->dgroups;
   job.wclasses = p->dwclasses;
+  job.chunks = p->dchunks;
+  job.uchunks = p->duchunks;
+  job.waves = p->dwaves;
+  job.units = p->dunits;
   job.work = work;
   job.src = src ? src : work;
   job.factors = factors;
@@ -524,10 +787,15 @@ const std::vector<hipStream_t>& ctx_streams(tah_ctx* c) {
   return c->streams;
 }
 
+void launch_one(const Launch& L, hipStream_t s, const Job* jobs, const Tile* tiles, const LdsTile* ltiles) {
+  if (L.lds > 0) launch_lds(L.cls, uint32_t(L.count), s, jobs, ltiles + L.first, L.lds);
+  else launch_class(L.cls, uint32_t(L.count), s, jobs, tiles + L.first, L.grid);
+}
+
 void run_launches(tah_ctx* c, const std::vector<Launch>& ls, hipStream_t s, const Job* jobs,
-                  const Tile* tiles, hipEvent_t* ev, bool fork) {
+                  const Tile* tiles, const LdsTile* ltiles, hipEvent_t* ev, bool fork) {
   if (!fork || ls.size() <= 1) {
-    for (const Launch& L : ls) launch_class(L.cls, uint32_t(L.count), s, jobs, tiles + L.first, L.grid);
+    for (const Launch& L : ls) launch_one(L, s, jobs, tiles, ltiles);
     return;
   }
   const auto& streams = ctx_streams(c);
@@ -535,7 +803,7 @@ void run_launches(tah_ctx* c, const std::vector<Launch>& ls, hipStream_t s, cons
   for (size_t i = 0; i < ls.size(); ++i) {
     hipStream_t si = streams[i % streams.size()];
     if (i < streams.size()) HIPCHECK(hipStreamWaitEvent(si, ev[0], 0));
-    launch_class(ls[i].cls, uint32_t(ls[i].count), si, jobs, tiles + ls[i].first, ls[i].grid);
+    launch_one(ls[i], si, jobs, tiles, ltiles);
   }
   for (size_t i = 0; i < std::min(ls.size(), streams.size()); ++i) {
     HIPCHECK(hipEventRecord(ev[1 + i], streams[i]));
@@ -563,6 +831,11 @@ tah_plan::~tah_plan() {
   if (done) (void)hipEventDestroy(done);
   if (dtables) (void)hipFree(dtables);
   if (dtiles) (void)hipFree(dtiles);
+  if (dchunks) (void)hipFree(dchunks);
+  if (dlds_tiles) (void)hipFree(dlds_tiles);
+  if (duchunks) (void)hipFree(duchunks);
+  if (dwaves) (void)hipFree(dwaves);
+  if (dunits) (void)hipFree(dunits);
   if (djob) (void)hipFree(djob);
   if (djob2) (void)hipFree(djob2);
 }

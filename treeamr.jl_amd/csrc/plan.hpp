@@ -40,6 +40,8 @@ inline int copy_launch(int cpt) { return kClsCopyLaunch + cpt; }
 constexpr int kClsInterp = 2;  // + 8*(index into kInterpTable) + rbits
 constexpr int kClsMerged = 1000;  // + index into kInterpTable: every rbits in one launch
 constexpr int kClsMergedPair = 1100;  // + index: the same over tiles of variable pairs
+constexpr int kClsLds = 1200;  // + index: LDS-staged chunks (interp_lds_kernel)
+constexpr int kClsLdsProlong = 1300;  // + index: LDS-staged prolongation by units
 
 // (P, SHIFT) of each specialised interpolation kernel family.
 extern const std::array<std::array<int, 2>, 8> kInterpTable;
@@ -72,6 +74,7 @@ struct Launch {
   int64_t first, count;
   int64_t values = 0;  // values the launch writes
   int32_t grid = 0;    // copies: workgroups of a bounded grid (0: one per tile)
+  int32_t lds = 0;     // LDS-staged: bytes of LDS per workgroup; first/count index LdsTiles
 };
 
 struct AdjointTables;
@@ -97,6 +100,19 @@ struct tah_plan {
   tah::DevWClass* dwclasses = nullptr;
   std::vector<tah::Tile> tiles;          // host copy, job field 0
   tah::Tile* dtiles = nullptr;
+  // LDS-staged launches: the chunks of the groups they cover and their
+  // tiles (host copies and device arrays, rebuilt with the launches).
+  std::vector<tah::LdsChunk> lds_chunks;
+  std::vector<tah::LdsTile> lds_tiles;
+  tah::LdsChunk* dchunks = nullptr;
+  tah::LdsTile* dlds_tiles = nullptr;
+  // ... of prolongation: chunks by units, their class waves and packed units
+  std::vector<tah::LdsUnitChunk> lds_uchunks;
+  std::vector<tah::LdsWave> lds_waves;
+  std::vector<uint32_t> lds_units;
+  tah::LdsUnitChunk* duchunks = nullptr;
+  tah::LdsWave* dwaves = nullptr;
+  uint32_t* dunits = nullptr;
   tah::Job* djob = nullptr;
   tah::Job last_job{};
   // A regrid transfer (stage tags TAH_STAGE_REGRID, then
@@ -127,6 +143,12 @@ struct tah_plan {
   // per thread where the layout has variables innermost and an even count
   // (1, the default), or of one variable per thread (0).
   int32_t interp_pairs = 1;
+  // Interpolation from LDS copies of its source boxes, chunk by chunk, where
+  // the layout holds variables innermost in even number: bit 0 restriction,
+  // bit 1 the prolongation of the units classes of at least 64 units (faces);
+  // the rest through the unit kernels. Off by default: on the large forest
+  // neither beat the unit kernels (the tuner tries them per mesh).
+  int32_t lds = 0;
   std::vector<hipEvent_t> events;  // fork/join events, created on first use
   // The adjoint's inverted schedule, built on first use (adjoint.hip).
   tah::AdjointTables* adj = nullptr;
@@ -163,7 +185,11 @@ const std::vector<hipStream_t>& ctx_streams(tah_ctx* c);
 // Run a stage group's launches forked over the context's streams and
 // joined back into `s`. ev must hold launches.size() + 1 events.
 void run_launches(tah_ctx* c, const std::vector<Launch>& ls, hipStream_t s, const Job* jobs,
-                  const Tile* tiles, hipEvent_t* ev, bool fork);
+                  const Tile* tiles, const LdsTile* ltiles, hipEvent_t* ev, bool fork);
+// An LDS-staged class over ntiles tiles with `bytes` of LDS per workgroup (fill.hip).
+void launch_lds(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const LdsTile* tiles, int32_t bytes);
+// Launch one Launch (copy, interpolation or LDS-staged) on stream s.
+void launch_one(const Launch& L, hipStream_t s, const Job* jobs, const Tile* tiles, const LdsTile* ltiles);
 // Launch one class over ntiles tiles (fill.hip); grid > 0: a bounded grid.
 void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles, int32_t grid = 0);
 // Order a call on plan p and stream s after the plan's previous call (on

@@ -82,14 +82,16 @@ void set_options(tah_plan* p, const tah_plan_options& o) {
   const int32_t wgs = o.copy_wgs;
   if (wgs < 0 || wgs % 8 != 0) fail("copy_wgs is 0 or a positive multiple of 8");
   if (o.interp_pairs < 0 || o.interp_pairs > 1) fail("plan option interp_pairs is 0 or 1");
+  if (o.lds < 0 || o.lds > 3) fail("plan option lds is 0 to 3 (bit 0 restriction, bit 1 prolongation)");
   const bool rebuild = o.merge != p->merge || o.defer != p->defer || o.copy_per_thread != p->copy_per_thread ||
-                       wgs != p->copy_wgs || o.interp_pairs != p->interp_pairs;
+                       wgs != p->copy_wgs || o.interp_pairs != p->interp_pairs || o.lds != p->lds;
   p->fork = o.fork;
   p->merge = o.merge;
   p->defer = o.defer;
   p->copy_per_thread = o.copy_per_thread;
   p->copy_wgs = wgs;
   p->interp_pairs = o.interp_pairs;
+  p->lds = o.lds;
   if (rebuild) build_launches(p);
 }
 
@@ -101,9 +103,9 @@ bool lookup(const std::string& path, const std::string& key, tah_plan_options& o
   while (std::getline(in, line)) {
     std::istringstream ss(line);
     std::string tag, k;
-    int f, m, d, c, w, ip;
-    if (!(ss >> tag >> k >> f >> m >> d >> c >> w >> ip) || tag != "tah-tune-3" || k != key) continue;
-    o = tah_plan_options{f, m, d, c, 0, w, ip, {0}};
+    int f, m, d, c, w, ip, l;
+    if (!(ss >> tag >> k >> f >> m >> d >> c >> w >> ip >> l) || tag != "tah-tune-4" || k != key) continue;
+    o = tah_plan_options{f, m, d, c, 0, w, ip, l};
     found = true;  // the last entry wins
   }
   return found;
@@ -120,10 +122,11 @@ std::vector<tah_plan_options> candidates() {
       for (int ip = m; ip >= 0; --ip)
         for (int d = 0; d <= 1; ++d)
           for (int c : {4, 8, 16})
-            for (int w : {0, 512, 1024, 2048}) {
-              if (w && !f) continue;
-              out.push_back(tah_plan_options{f, m, d, c, 0, w, m ? ip : 1, {0}});
-            }
+            for (int w : {0, 512, 1024, 2048})
+              for (int l = 3; l >= 0; --l) {
+                if (w && !f) continue;
+                out.push_back(tah_plan_options{f, m, d, c, 0, w, m ? ip : 1, l});
+              }
   return out;
 }
 
@@ -141,7 +144,7 @@ tah_status tah_plan_get_options(const tah_plan* p, tah_plan_options* out) {
     out->fuse = 0;
     out->copy_wgs = p->copy_wgs;
     out->interp_pairs = p->interp_pairs;
-    out->reserved_[0] = 0;
+    out->lds = p->lds;
   });
 }
 
@@ -173,7 +176,7 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
-    tah_plan_options best{p->fork, p->merge, p->defer, p->copy_per_thread, 0, p->copy_wgs, p->interp_pairs, {0}};
+    tah_plan_options best{p->fork, p->merge, p->defer, p->copy_per_thread, 0, p->copy_wgs, p->interp_pairs, p->lds};
     float best_t = 1e30f;
     auto run = [&]() {
       tah_status st = tah_fill(p, work, factors, s);
@@ -214,8 +217,9 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
     if (cache_path) {
       std::ofstream out(cache_path, std::ios::app);
       if (!out) fail(std::string("tah_tune: cannot write the cache file ") + cache_path);
-      out << "tah-tune-3 " << key << " " << best.fork << " " << best.merge << " " << best.defer << " "
-          << best.copy_per_thread << " " << best.copy_wgs << " " << best.interp_pairs << " " << best_t << "\n";
+      out << "tah-tune-4 " << key << " " << best.fork << " " << best.merge << " " << best.defer << " "
+          << best.copy_per_thread << " " << best.copy_wgs << " " << best.interp_pairs << " " << best.lds << " "
+          << best_t << "\n";
     }
   });
 }

@@ -339,21 +339,22 @@ class Plan:
 
     @property
     def options(self):
-        """The launch options: dict(fork=, merge=, defer=, fuse=, interp_pairs=)
-        each 0 or 1 (fuse always 0), copy_per_thread 4, 8 or 16, copy_wgs 0
-        or a multiple of 8."""
+        """The launch options: dict(fork=, merge=, defer=, fuse=, interp_pairs=,
+        lds=) each 0 or 1 (fuse always 0), copy_per_thread 4, 8 or 16,
+        copy_wgs 0 or a multiple of 8."""
         o = _abi.tah_plan_options()
         check(lib().tah_plan_get_options(self._h, C.byref(o)))
         return dict(fork=o.fork, merge=o.merge, defer=o.defer, copy_per_thread=o.copy_per_thread,
-                    fuse=o.fuse, copy_wgs=o.copy_wgs, interp_pairs=o.interp_pairs)
+                    fuse=o.fuse, copy_wgs=o.copy_wgs, interp_pairs=o.interp_pairs, lds=o.lds)
 
     def set_options(self, fork=None, merge=None, defer=None, copy_per_thread=None, fuse=None,
-                    copy_wgs=None, interp_pairs=None):
+                    copy_wgs=None, interp_pairs=None, lds=None):
         cur = self.options
         pick = lambda name, v: cur[name] if v is None else int(v)
         o = _abi.tah_plan_options(pick("fork", fork), pick("merge", merge), pick("defer", defer),
                                   pick("copy_per_thread", copy_per_thread), pick("fuse", fuse),
-                                  pick("copy_wgs", copy_wgs), pick("interp_pairs", interp_pairs))
+                                  pick("copy_wgs", copy_wgs), pick("interp_pairs", interp_pairs),
+                                  pick("lds", lds))
         check(lib().tah_plan_set_options(self._h, C.byref(o)))
 
 
