@@ -357,6 +357,10 @@ typedef struct tah_plan_options {
   int32_t lds;             /* interpolation from LDS copies of its source boxes, chunk by chunk, where the
                               layout holds variables innermost in even number: bit 0 restriction, bit 1
                               prolongation's unit classes of 64 units or more (0, the default: unit kernels) */
+  int32_t queue;           /* in a batched fill of plans that all set it: one persistent launch per stage
+                              position over a work queue of every plan's tiles (1), or a launch per kernel
+                              class (0, default); reserved_ must be zero */
+  int32_t reserved_[3];
 } tah_plan_options;
 tah_status tah_plan_get_options(const tah_plan* p, tah_plan_options* out);
 tah_status tah_plan_set_options(tah_plan* p, const tah_plan_options* o);

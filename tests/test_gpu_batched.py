@@ -44,6 +44,34 @@ def test_batched_equals_single_fills():
         assert torch.equal(fs.work.view(torch.int64), r.view(torch.int64))
 
 
+@pytest.mark.parametrize("queue", ["all", "none", "mixed"])
+def test_batched_queue_modes(queue):
+    """The work-queue launch (every plan's option queue set), the per-class
+    launches (none set) and a batch with one plan opted out (no queue for the
+    batch): each bit for bit the single fills, two calls in a row (the queue's
+    counters start again from zero), then again after the per-plan copy tile
+    changes (a batch built anew)."""
+    pairs = _forests(range(20, 26), max_blocks=250)
+    ref = _reference(pairs)
+    init = [fs.work.clone() for fs, _ in pairs]
+    for i, (fs, s) in enumerate(pairs):
+        on = queue == "all" or (queue == "mixed" and i != 2)
+        s.plan(fs.layout).set_options(queue=int(on))
+    print(f"[batched queue={queue}] {len(pairs)} field sets", flush=True)
+    for rep in range(3):
+        if rep == 2:
+            for fs, s in pairs:
+                s.plan(fs.layout).set_options(copy_per_thread=16 if s.plan(fs.layout).options["merge"] else 4)
+        for (fs, _), i0 in zip(pairs, init):
+            fs.work.copy_(i0)
+        fill_ghosts_batched(pairs)
+        torch.cuda.synchronize()
+        for k, ((fs, _), r) in enumerate(zip(pairs, ref)):
+            assert torch.equal(fs.work.view(torch.int64), r.view(torch.int64)), f"call {rep}, field set {k}"
+    for fs, s in pairs:
+        s.plan(fs.layout).set_options(queue=0, copy_per_thread=8)
+
+
 def test_graph_replays_equal_single_fills():
     pairs = _forests(range(10, 16))
     ref = _reference(pairs)

@@ -87,7 +87,7 @@ def test_option_values_refused():
     plan = s.plan(fs.layout)
     before = plan.options
     for bad in (dict(interp_pairs=2), dict(interp_pairs=-1), dict(copy_per_thread=5), dict(copy_wgs=12),
-                dict(fork=2)):
+                dict(fork=2), dict(lds=4), dict(queue=2), dict(queue=-1)):
         with pytest.raises(ValueError):
             plan.set_options(**bad)
         assert plan.options == before, bad

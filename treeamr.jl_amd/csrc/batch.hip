@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "api_common.hpp"
+#include "kernels.hpp"
 #include "plan.hpp"
 
 using namespace tah;
@@ -60,10 +61,23 @@ struct Batch {
   std::vector<hipEvent_t> events;
   int device = 0;
   bool fork = false;  // forked when any of its plans is
+  // The work queue (every plan's option `queue` set): per stage position
+  // the count of its queued tiles (dqtiles, Tile.pad the kernel code), nine
+  // share bounds (dshares) and eight counters (dcounters); the classes the
+  // queue kernel does not take stay in `launches`.
+  bool queue = false;
+  uint32_t queue_grid = 0;
+  std::vector<int64_t> qcount;
+  Tile* dqtiles = nullptr;
+  uint32_t* dshares = nullptr;
+  uint32_t* dcounters = nullptr;
   ~Batch() {
     if (djobs) (void)hipFree(djobs);
     if (dtiles) (void)hipFree(dtiles);
     if (dltiles) (void)hipFree(dltiles);
+    if (dqtiles) (void)hipFree(dqtiles);
+    if (dshares) (void)hipFree(dshares);
+    if (dcounters) (void)hipFree(dcounters);
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
   }
 };
@@ -87,6 +101,7 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
     b->plans.push_back(p);
     b->serials.push_back(p->serial);
     b->fork = b->fork || p->fork != 0;
+    b->queue = i == 0 ? p->queue != 0 : b->queue && p->queue != 0;
     b->works.push_back(works[i]);
     b->factors.push_back(f);
     Job j = plan_job(p, works[i], f);
@@ -98,6 +113,16 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
       if (b->works[i] == b->works[k]) fail("two fills of one batch share a working array");
   std::vector<Tile> tiles;
   std::vector<LdsTile> ltiles;
+  std::vector<Tile> qtiles;
+  std::vector<uint32_t> shares;
+  // the block a tile works on: its target block (its source block for
+  // restriction, whose tiles are ordered by source block)
+  auto block_of = [&](const Tile& t, int kind) {
+    const tah_plan* p = b->plans[size_t(t.job)];
+    const bool unit = kind >= kQueueMerged;
+    const PlanGroup& g = p->groups[size_t(unit ? p->wclasses[size_t(t.item)].group : t.item)];
+    return (g.kind == 2 ? g.sblocks : g.tblocks)[size_t(t.transfer)];
+  };
   for (size_t pos = 0; pos < npos; ++pos) {
     std::map<int, std::vector<Tile>> bycls;
     std::map<int, std::vector<LdsTile>> ldscls;  // LDS-staged classes
@@ -123,6 +148,32 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
         }
       }
     }
+    if (b->queue) {
+      // the classes the queue kernel takes, in one list ordered by (plan,
+      // block) so that every XCD's share holds a mix of them, each tile
+      // where the tiles that read or write its blocks are
+      std::vector<std::pair<uint64_t, Tile>> q;
+      for (auto it = bycls.begin(); it != bycls.end();) {
+        const int kind = queue_kind(it->first);
+        if (kind < 0) {
+          ++it;
+          continue;
+        }
+        for (Tile t : it->second) {
+          t.pad = kind;
+          q.push_back({(uint64_t(uint32_t(t.job)) << 32) | uint32_t(block_of(t, kind)), t});
+        }
+        it = bycls.erase(it);
+      }
+      std::stable_sort(q.begin(), q.end(), [](const std::pair<uint64_t, Tile>& x, const std::pair<uint64_t, Tile>& y) {
+        return x.first < y.first;
+      });
+      const uint32_t first = uint32_t(qtiles.size());
+      for (const auto& e : q) qtiles.push_back(e.second);
+      for (uint32_t x = 0; x <= 8; ++x) shares.push_back(first + uint32_t(uint64_t(q.size()) * x / 8));
+      b->qcount.push_back(int64_t(q.size()));
+      if (qtiles.size() > size_t(UINT32_MAX / 2)) fail("too many tiles for the batch's work queue", kUnsupported);
+    }
     std::vector<Launch> ls;
     for (auto& kv : bycls) {
       ls.push_back(Launch{kv.first, int64_t(tiles.size()), int64_t(kv.second.size())});
@@ -146,6 +197,20 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
     HIPCHECK(hipMalloc(&b->dltiles, sizeof(LdsTile) * ltiles.size()));
     HIPCHECK(hipMemcpy(b->dltiles, ltiles.data(), sizeof(LdsTile) * ltiles.size(), hipMemcpyHostToDevice));
   }
+  if (b->queue) {
+    // a persistent grid: as many workgroups as the CUs hold at the queue
+    // kernel's occupancy, a multiple of 8 (one share per XCD)
+    const int32_t wgs = ctx->prop.multiProcessorCount * (queue_waves_per_cu() / (kThreads / 64));
+    b->queue_grid = uint32_t(std::max(8, wgs / 8 * 8));
+    if (!qtiles.empty()) {
+      HIPCHECK(hipMalloc(&b->dqtiles, sizeof(Tile) * qtiles.size()));
+      HIPCHECK(hipMemcpy(b->dqtiles, qtiles.data(), sizeof(Tile) * qtiles.size(), hipMemcpyHostToDevice));
+    }
+    HIPCHECK(hipMalloc(&b->dshares, sizeof(uint32_t) * std::max<size_t>(1, shares.size())));
+    if (!shares.empty())
+      HIPCHECK(hipMemcpy(b->dshares, shares.data(), sizeof(uint32_t) * shares.size(), hipMemcpyHostToDevice));
+    HIPCHECK(hipMalloc(&b->dcounters, sizeof(uint32_t) * 8 * std::max<size_t>(1, npos)));
+  }
   for (int i = 0; i < 8; ++i) {
     hipEvent_t e;
     HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -155,7 +220,12 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
 }
 
 void run_batch(tah_ctx* ctx, Batch& b, hipStream_t s) {
-  for (const auto& ls : b.launches) run_launches(ctx, ls, s, b.djobs, b.dtiles, b.dltiles, b.events.data(), b.fork);
+  if (b.queue) launch_queue_reset(s, b.dcounters, uint32_t(8 * b.launches.size()));
+  for (size_t pos = 0; pos < b.launches.size(); ++pos) {
+    if (b.queue && b.qcount[pos] > 0)
+      launch_queue(b.queue_grid, s, b.djobs, b.dqtiles, b.dshares + 9 * pos, b.dcounters + 8 * pos);
+    run_launches(ctx, b.launches[pos], s, b.djobs, b.dtiles, b.dltiles, b.events.data(), b.fork);
+  }
 }
 
 // Every plan of the batch alive and in the state the batch was built from.

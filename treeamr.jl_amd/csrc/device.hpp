@@ -45,6 +45,7 @@ struct alignas(16) DevGroup {
   int32_t ca[3];      // a unit copy's source row of target row 0 ...
   int32_t cs[3];      // ... and its step per row (1, -1 or 0)
   FastDiv dn[3];      // division by n0, n1, n2
+  FastDiv dper;       // division by points*nvars, the values of one transfer
   const int32_t* tblocks;
   const int32_t* sblocks;
   const int32_t* srcstart[3];

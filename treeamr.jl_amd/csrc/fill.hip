@@ -134,6 +134,37 @@ void launch_lds(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const 
   HIPCHECK(hipGetLastError());
 }
 
+int queue_kind(int cls) {
+  if (cls == copy_launch(8)) return kQueueCopy;
+  if (cls == copy_launch(4)) return kQueueCopy4;
+  if (cls == copy_launch(16)) return kQueueCopy16;
+  if (cls == kClsGeneric) return kQueueGeneric;
+  if (cls >= kClsMerged && cls < kClsMerged + int(kInterpTable.size())) {
+    const int i = cls - kClsMerged;
+    return (i == 0 || i == 2 || i == 4 || i == 5 || i == 6 || i == 7) ? kQueueMerged + i : -1;
+  }
+  if (cls >= kClsMergedPair && cls < kClsMergedPair + int(kInterpTable.size())) {
+    const int i = cls - kClsMergedPair;
+    return (i == 0 || i == 2 || i == 4) ? kQueuePair + i : -1;
+  }
+  return -1;
+}
+
+void launch_queue(uint32_t grid, hipStream_t s, const Job* jobs, const Tile* tiles, const uint32_t* share,
+                  uint32_t* counters) {
+  if (grid == 0 || grid % 8 != 0) fail("the queue's grid must be a positive multiple of 8");
+  hipLaunchKernelGGL(queue_kernel, dim3(grid), dim3(kThreads), 0, s, jobs, tiles, share, counters);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_queue_reset(hipStream_t s, uint32_t* counters, uint32_t n) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(queue_reset_kernel, dim3((n + 255) / 256), dim3(256), 0, s, counters, n);
+  HIPCHECK(hipGetLastError());
+}
+
+int queue_waves_per_cu() { return 4 * kQueueWaves; }
+
 int class_tile_elems(int cls) {
   if (cls > kClsCopyLaunch && cls <= kClsCopyLaunch + 16) return kThreads * (cls - kClsCopyLaunch);
   return cls == kClsGeneric ? kGenericTile : kThreads;

@@ -176,9 +176,68 @@ __global__ void __launch_bounds__(kThreads) generic_kernel(const Job* __restrict
 constexpr int kCopyPerThread = 8;
 constexpr int kCopyTile = kThreads * kCopyPerThread;
 
+// A tile of tile.nu > 1 small transfers of one group, packed: its values
+// are the nu transfers' values one after another (transfers tile.transfer
+// ..), so that a tile of small boxes (corners, edges of small blocks) keeps
+// its threads busy. Each value finds its transfer by one division.
+template <int CPT>
+__device__ __forceinline__ void copy_packed_body(const Job& job, const Tile& tile) {
+  typedef double pair_t __attribute__((ext_vector_type(2)));
+  const DevGroup& g = job.groups[tile.item];
+  const int32_t s0 = int32_t(job.stride[0]), s1 = int32_t(job.stride[1]),
+                s2 = int32_t(job.stride[2]), s3 = int32_t(job.stride[3]);
+  const int64_t s4 = job.stride[4];
+  const int32_t nvars = job.nvars;
+  const uint32_t per = uint32_t(g.points) * uint32_t(nvars);
+  const uint32_t total = per * uint32_t(tile.nu);
+  const int32_t tb0 = s0 * g.tfirst[0] + s1 * g.tfirst[1] + s2 * g.tfirst[2];
+  const int32_t sb0 = s0 * g.ca[0] + s1 * g.ca[1] + s2 * g.ca[2];
+  const int32_t ds0 = s0 * g.cs[0], ds1 = s1 * g.cs[1], ds2 = s2 * g.cs[2];
+  constexpr int STEP = 2;  // pairs where the array allows, else single values twice
+#pragma unroll
+  for (int r = 0; r < CPT / STEP; ++r) {
+    const uint32_t q = STEP * (uint32_t(r * kThreads) + threadIdx.x);
+    if (q >= total) break;
+    if (job.pairs) {
+      // values q, q+1: one point's variables v, v+1 of one transfer (the
+      // count of a transfer's values is even)
+      const uint32_t tl = g.dper.div(q);
+      const int32_t tr = tile.transfer + int32_t(tl);
+      const double* __restrict__ sb = job.src + s4 * g.sblocks[tr];
+      double* __restrict__ tb = job.work + s4 * g.tblocks[tr];
+      int32_t i, j, k, v;
+      split_element(g, job, q - tl * per, i, j, k, v);
+      const pair_t x = *reinterpret_cast<const pair_t*>(sb + (v + sb0 + ds0 * i + ds1 * j + ds2 * k));
+      pair_t out;
+      out.x = scaled(__dadd_rn(0.0, x.x), job.factors, g.factorcol, nvars, v);
+      out.y = scaled(__dadd_rn(0.0, x.y), job.factors, g.factorcol, nvars, v + 1);
+      __builtin_nontemporal_store(out, reinterpret_cast<pair_t*>(tb + (v + tb0 + s0 * i + s1 * j + s2 * k)));
+    } else {
+#pragma unroll
+      for (int h = 0; h < STEP; ++h) {
+        const uint32_t qh = q + uint32_t(h);
+        if (qh >= total) break;
+        const uint32_t tlh = g.dper.div(qh);
+        const int32_t trh = tile.transfer + int32_t(tlh);
+        int32_t ih, jh, kh, vh;
+        split_element(g, job, qh - tlh * per, ih, jh, kh, vh);
+        const double* __restrict__ sbh = job.src + s4 * g.sblocks[trh];
+        double* __restrict__ tbh = job.work + s4 * g.tblocks[trh];
+        const double x = sbh[s3 * vh + sb0 + ds0 * ih + ds1 * jh + ds2 * kh];
+        tbh[s3 * vh + tb0 + s0 * ih + s1 * jh + s2 * kh] =
+            scaled(__dadd_rn(0.0, x), job.factors, g.factorcol, nvars, vh);
+      }
+    }
+  }
+}
+
 // CPT values per thread (the tile shape a plan option picks: 4, 8 or 16).
 template <int CPT>
 __device__ __forceinline__ void copy_body(const Job& job, const Tile& tile) {
+  if (tile.nu > 1) {
+    copy_packed_body<CPT>(job, tile);
+    return;
+  }
   const DevGroup& g = job.groups[tile.item];
   // Offsets within one block fit 32 bits (plans refuse larger blocks).
   const int32_t s0 = int32_t(job.stride[0]), s1 = int32_t(job.stride[1]),
@@ -495,6 +554,101 @@ interp_pair_kernel(const Job* __restrict__ jobs, const Tile* __restrict__ tiles)
   else interp_pair_dispatch<1, P, SHIFT>(job, tile, wc);
 }
 
+#endif  // TAH_DEFINE_KERNELS
+
+// ---------------------------------------------------------------------------
+// The work queue of a batched fill: every tile of one stage position of every
+// plan of the batch -- copies, interpolations of the orders below, generic
+// transfers -- in one launch of a persistent grid. The tiles are cut into
+// eight contiguous shares, one per XCD (a workgroup's XCD is its index mod 8,
+// as the dispatcher deals them); the workgroups of an XCD take the tiles of
+// its share in order from an atomic counter of its own until the share is
+// used up, so that small classes fill the gaps of large ones instead of each
+// class's launch ending on a tail. Tile.pad names the tile's kernel: a
+// kQueue* code.
+constexpr int kQueueCopy = 0;     // copy_body<8>
+constexpr int kQueueGeneric = 1;  // the generic tile
+constexpr int kQueueCopy4 = 2;    // copy_body<4>
+constexpr int kQueueCopy16 = 3;   // copy_body<16>
+constexpr int kQueueMerged = 16;  // + index into kInterpTable: one variable per thread
+constexpr int kQueuePair = 32;    // + index: two variables per thread (prolongation)
+constexpr int kQueueWaves = 4;    // waves per SIMD the queue kernel is compiled for
+
+#ifdef TAH_DEFINE_KERNELS
+__device__ __forceinline__ void generic_body(const Job& job, const Tile& tile) {
+  const DevGroup& g = job.groups[tile.item];
+  const int64_t s0 = job.stride[0], s1 = job.stride[1], s2 = job.stride[2], s3 = job.stride[3],
+                s4 = job.stride[4];
+  const int32_t nvars = job.nvars;
+  const uint32_t total = uint32_t(g.points) * uint32_t(nvars);
+  const double* sbase = job.src + s4 * g.sblocks[tile.transfer];
+  double* tbase = job.work + s4 * g.tblocks[tile.transfer];
+  for (int r = 0; r < kGenericPerThread; ++r) {
+    uint32_t e = uint32_t(tile.e0) + uint32_t(r * kThreads) + threadIdx.x;
+    if (e >= total) break;
+    int32_t i, j, k, v;
+    split_element(g, job, e, i, j, k, v);
+    double acc = stencil_value(g, sbase + s3 * v, s0, s1, s2, i, j, k);
+    acc = scaled(acc, job.factors, g.factorcol, nvars, v);
+    tbase[s3 * v + s0 * (g.tfirst[0] + i) + s1 * (g.tfirst[1] + j) + s2 * (g.tfirst[2] + k)] = acc;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kQueueWaves, 8)))
+queue_kernel(const Job* __restrict__ jobs, const Tile* __restrict__ tiles, const uint32_t* __restrict__ share,
+             uint32_t* __restrict__ counters) {
+  __shared__ uint32_t slot;
+  const uint32_t x = blockIdx.x % 8;
+  const uint32_t lo = share[x], hi = share[x + 1];
+  for (;;) {
+    if (threadIdx.x == 0) slot = lo + atomicAdd(counters + x, 1u);
+    __syncthreads();
+    const uint32_t it = slot;
+    __syncthreads();  // every thread has read slot before thread 0 writes it again
+    if (it >= hi) break;  // the same for the whole workgroup
+    const Tile tile = tiles[it];
+    const Job& job = jobs[tile.job];
+    const int32_t kind = uniform(tile.pad);
+    if (kind == kQueueCopy) {
+      copy_body<8>(job, tile);
+    } else if (kind == kQueueCopy4) {
+      copy_body<4>(job, tile);
+    } else if (kind == kQueueCopy16) {
+      copy_body<16>(job, tile);
+    } else if (kind == kQueueGeneric) {
+      generic_body(job, tile);
+    } else {
+      const DevWClass& wc = job.wclasses[tile.item];
+      switch (kind) {
+        case kQueueMerged + 0: interp_pair_dispatch<0, 2, 0>(job, tile, wc); break;
+        case kQueueMerged + 2: interp_pair_dispatch<0, 4, 0>(job, tile, wc); break;
+        case kQueueMerged + 4: interp_pair_dispatch<0, 6, 0>(job, tile, wc); break;
+        case kQueueMerged + 5: interp_pair_dispatch<0, 2, 2>(job, tile, wc); break;
+        case kQueueMerged + 6: interp_pair_dispatch<0, 4, 2>(job, tile, wc); break;
+        case kQueueMerged + 7: interp_pair_dispatch<0, 6, 2>(job, tile, wc); break;
+        case kQueuePair + 0:
+          if (job.pairs) interp_pair_dispatch<2, 2, 0>(job, tile, wc);
+          else interp_pair_dispatch<1, 2, 0>(job, tile, wc);
+          break;
+        case kQueuePair + 2:
+          if (job.pairs) interp_pair_dispatch<2, 4, 0>(job, tile, wc);
+          else interp_pair_dispatch<1, 4, 0>(job, tile, wc);
+          break;
+        default:
+          if (job.pairs) interp_pair_dispatch<2, 6, 0>(job, tile, wc);
+          else interp_pair_dispatch<1, 6, 0>(job, tile, wc);
+          break;
+      }
+    }
+  }
+}
+
+// The queue's counters to zero (a kernel, not a memset, so that the batch
+// can be captured in a graph and replayed).
+__global__ void queue_reset_kernel(uint32_t* __restrict__ counters, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) counters[i] = 0u;
+}
 #endif  // TAH_DEFINE_KERNELS
 
 // ---------------------------------------------------------------------------

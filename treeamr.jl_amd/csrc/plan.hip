@@ -355,6 +355,7 @@ void build_plan_device(tah_plan* p) {
       g.srcstart[d] = reinterpret_cast<const int32_t*>(base + o.ss[d]);
       g.weights[d] = reinterpret_cast<const double*>(base + o.w[d]);
     }
+    g.dper = FastDiv(uint32_t(std::max<int64_t>(1, int64_t(h.points) * p->layout.nvars)));
     p->devgroups.push_back(g);
   }
   if (!p->devgroups.empty())
@@ -662,12 +663,30 @@ void build_launches(tah_plan* p) {
             }
           }
         } else if (group_launch(g.cls) == cls) {
-          int64_t elems = int64_t(g.points) * nvars;
-          for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t) {
-            if (run_in[gi][t] != k) continue;
-            for (int64_t e = 0; e < elems; e += per)
-              tl.push_back({g.tblocks[t], Tile{0, int32_t(gi), t, int32_t(e), 0, 0, 0, 0}});
-            values += elems;
+          const int64_t elems = int64_t(g.points) * nvars;
+          const int32_t nt = int32_t(g.tblocks.size());
+          // A copy group's transfers of at most half a tile, several to a
+          // tile: consecutive transfers of this stage group (tile.nu of them).
+          const int64_t pack = g.cls == kClsCopy ? per / elems : 1;
+          if (pack >= 2) {
+            for (int32_t t = 0; t < nt;) {
+              if (run_in[gi][t] != k) {
+                ++t;
+                continue;
+              }
+              int32_t t1 = t;
+              while (t1 < nt && run_in[gi][t1] == k && t1 - t < pack) ++t1;
+              tl.push_back({g.tblocks[t], Tile{0, int32_t(gi), t, 0, t1 - t, 0, 0, 0}});
+              values += elems * (t1 - t);
+              t = t1;
+            }
+          } else {
+            for (int32_t t = 0; t < nt; ++t) {
+              if (run_in[gi][t] != k) continue;
+              for (int64_t e = 0; e < elems; e += per)
+                tl.push_back({g.tblocks[t], Tile{0, int32_t(gi), t, int32_t(e), 0, 0, 0, 0}});
+              values += elems;
+            }
           }
         }
       }

@@ -149,6 +149,12 @@ struct tah_plan {
   // the rest through the unit kernels. Off by default: on the large forest
   // neither beat the unit kernels (the tuner tries them per mesh).
   int32_t lds = 0;
+  // In a batched fill of plans that all have it set: one persistent launch
+  // per stage position over a work queue of every plan's tiles (1), or a
+  // launch per kernel class (0, the default: on the 64-forest batch the
+  // queue took 26.4 ms against 21.5 ms, its one register budget holding the
+  // copies to four waves per SIMD).
+  int32_t queue = 0;
   std::vector<hipEvent_t> events;  // fork/join events, created on first use
   // The adjoint's inverted schedule, built on first use (adjoint.hip).
   tah::AdjointTables* adj = nullptr;
@@ -186,6 +192,14 @@ const std::vector<hipStream_t>& ctx_streams(tah_ctx* c);
 // joined back into `s`. ev must hold launches.size() + 1 events.
 void run_launches(tah_ctx* c, const std::vector<Launch>& ls, hipStream_t s, const Job* jobs,
                   const Tile* tiles, const LdsTile* ltiles, hipEvent_t* ev, bool fork);
+// The batched fill's work queue (fill.hip): the queue_kernel code of a
+// launch class (-1: not in the queue), its launch over shares of tiles, the
+// reset of its counters, and the waves a CU holds of it.
+int queue_kind(int cls);
+void launch_queue(uint32_t grid, hipStream_t s, const Job* jobs, const Tile* tiles, const uint32_t* share,
+                  uint32_t* counters);
+void launch_queue_reset(hipStream_t s, uint32_t* counters, uint32_t n);
+int queue_waves_per_cu();
 // An LDS-staged class over ntiles tiles with `bytes` of LDS per workgroup (fill.hip).
 void launch_lds(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const LdsTile* tiles, int32_t bytes);
 // Launch one Launch (copy, interpolation or LDS-staged) on stream s.

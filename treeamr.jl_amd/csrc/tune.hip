@@ -83,6 +83,8 @@ void set_options(tah_plan* p, const tah_plan_options& o) {
   if (wgs < 0 || wgs % 8 != 0) fail("copy_wgs is 0 or a positive multiple of 8");
   if (o.interp_pairs < 0 || o.interp_pairs > 1) fail("plan option interp_pairs is 0 or 1");
   if (o.lds < 0 || o.lds > 3) fail("plan option lds is 0 to 3 (bit 0 restriction, bit 1 prolongation)");
+  if (o.queue < 0 || o.queue > 1) fail("plan option queue is 0 or 1");
+  if (o.reserved_[0] || o.reserved_[1] || o.reserved_[2]) fail("tah_plan_options.reserved_ must be zero");
   const bool rebuild = o.merge != p->merge || o.defer != p->defer || o.copy_per_thread != p->copy_per_thread ||
                        wgs != p->copy_wgs || o.interp_pairs != p->interp_pairs || o.lds != p->lds;
   p->fork = o.fork;
@@ -92,6 +94,11 @@ void set_options(tah_plan* p, const tah_plan_options& o) {
   p->copy_wgs = wgs;
   p->interp_pairs = o.interp_pairs;
   p->lds = o.lds;
+  if (o.queue != p->queue) {
+    // batches built from this plan are taken again with the new mode
+    p->queue = o.queue;
+    plan_register(p);
+  }
   if (rebuild) build_launches(p);
 }
 
@@ -145,6 +152,8 @@ tah_status tah_plan_get_options(const tah_plan* p, tah_plan_options* out) {
     out->copy_wgs = p->copy_wgs;
     out->interp_pairs = p->interp_pairs;
     out->lds = p->lds;
+    out->queue = p->queue;
+    out->reserved_[0] = out->reserved_[1] = out->reserved_[2] = 0;
   });
 }
 
@@ -167,6 +176,7 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
     const std::string key = signature(p);
     tah_plan_options o{};
     if (cache_path && lookup(cache_path, key, o)) {
+      o.queue = p->queue;  // a batch setting: the tuner times single fills
       set_options(p, o);
       if (best_ms) *best_ms = -1.0;
       return;
@@ -176,14 +186,16 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
-    tah_plan_options best{p->fork, p->merge, p->defer, p->copy_per_thread, 0, p->copy_wgs, p->interp_pairs, p->lds};
+    tah_plan_options best{p->fork, p->merge, p->defer, p->copy_per_thread, 0, p->copy_wgs, p->interp_pairs, p->lds,
+                          p->queue, {0, 0, 0}};
     float best_t = 1e30f;
     auto run = [&]() {
       tah_status st = tah_fill(p, work, factors, s);
       if (st != TAH_OK) fail(std::string("tah_tune: a trial fill failed: ") + tah_last_error(), st);
     };
     try {
-      for (const tah_plan_options& cand : candidates()) {
+      for (tah_plan_options cand : candidates()) {
+        cand.queue = p->queue;
         set_options(p, cand);
         for (int w = 0; w < 2; ++w) run();
         std::vector<float> ts;

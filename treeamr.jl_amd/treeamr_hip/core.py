@@ -345,16 +345,17 @@ class Plan:
         o = _abi.tah_plan_options()
         check(lib().tah_plan_get_options(self._h, C.byref(o)))
         return dict(fork=o.fork, merge=o.merge, defer=o.defer, copy_per_thread=o.copy_per_thread,
-                    fuse=o.fuse, copy_wgs=o.copy_wgs, interp_pairs=o.interp_pairs, lds=o.lds)
+                    fuse=o.fuse, copy_wgs=o.copy_wgs, interp_pairs=o.interp_pairs, lds=o.lds,
+                    queue=o.queue)
 
     def set_options(self, fork=None, merge=None, defer=None, copy_per_thread=None, fuse=None,
-                    copy_wgs=None, interp_pairs=None, lds=None):
+                    copy_wgs=None, interp_pairs=None, lds=None, queue=None):
         cur = self.options
         pick = lambda name, v: cur[name] if v is None else int(v)
         o = _abi.tah_plan_options(pick("fork", fork), pick("merge", merge), pick("defer", defer),
                                   pick("copy_per_thread", copy_per_thread), pick("fuse", fuse),
                                   pick("copy_wgs", copy_wgs), pick("interp_pairs", interp_pairs),
-                                  pick("lds", lds))
+                                  pick("lds", lds), pick("queue", queue))
         check(lib().tah_plan_set_options(self._h, C.byref(o)))
 
 
