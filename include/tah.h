@@ -161,6 +161,26 @@ typedef struct tah_group_desc {
   tah_stencil1d stencils[3];
 } tah_group_desc;
 
+/* One outer-face hook region (BoundaryRegion, src/schedule.jl:200-229;
+ * boundary_region, :2188-2217): block, direction, and the 0-based half-open
+ * box lo..hi of stored indices the hook writes. For the relative hook:
+ * clamps[d], the stored index of the nearest evolved point along a
+ * component of the direction that leaves the domain through an outer face
+ * (-1 along every other, where the point keeps the ghost's own index), and
+ * slot, the tag of the stage after which the ghosts that point reads are
+ * filled (TAH_STAGE_PHASE1, or the prolongations onto the block's level).
+ * first: the index of the region's first point among every region's
+ * points, a region's points numbered with i fastest, then j, then k. */
+typedef struct tah_boundary_region {
+  int32_t block;
+  int32_t direction[3];
+  int32_t lo[3];
+  int32_t hi[3];
+  int32_t clamps[3];
+  int32_t slot;
+  int64_t first;
+} tah_boundary_region;
+
 typedef struct tah_schedule_desc {
   int32_t dim;
   int32_t nblocks;
@@ -177,6 +197,12 @@ typedef struct tah_schedule_desc {
   int64_t generation;           /* forest generation it was built for */
   int32_t nsources;             /* regrid: the old leaves the sources index; 0: nblocks */
   int32_t pad_;
+  /* The outer-face hook regions, in tah_schedule_boundaries' order, and
+   * their points in all (none for a regrid or unify descriptor). */
+  int32_t nboundaries;
+  int32_t pad2_;
+  const tah_boundary_region* boundaries;
+  int64_t boundary_points;
 } tah_schedule_desc;
 
 /* The regrid transfer of one field-set layout onto the forest's current
@@ -289,6 +315,27 @@ tah_status tah_regrid_transfer(tah_plan* p, const tah_layout* old_layout, const 
 /* The staged fill: one launch per stage group over a tile list of every
  * group of its stages, class-specialised kernels. Same results, bit for bit. */
 tah_status tah_fill(tah_plan* p, double* work, const double* factors, void* stream);
+
+/* fill_ghosts!(fs, schedule; boundary = CellBoundary(...)) (exchange_ghosts!,
+ * src/ghosts.jl:1099-1160; the hook kernels :1177-1318): the staged fill
+ * with the outer-face hook run between its stage groups, the hook's values
+ * computed by the caller at the positions nodeposition gives (the callback
+ * itself cannot cross a C boundary). values and fr are device arrays of
+ * desc.boundary_points x nvars doubles, the variables of one point
+ * contiguous, points in the regions' numbering (tah_boundary_region.first).
+ *   TAH_HOOK_VALUES (CellBoundary(g), its AllVariables and WithPatch forms):
+ *     after phase 1's stage group, each hook ghost of variable v becomes
+ *     values[p*nvars + v]; fr is not read.
+ *   TAH_HOOK_RELATIVE (CellBoundary(Relative(f))): values = f(x_g), fr =
+ *     f(x_r), x_r the nearest evolved point; after the stage group that
+ *     holds the region's slot stage, each ghost becomes
+ *     values + (work[r] - fr), in that order.
+ * A schedule without hook regions runs as tah_fill. The prescribed wall
+ * (CellBoundary(g; wall = :prescribed), stage W) is not provided: the wall
+ * plane stays the application's, the reference's default. */
+enum { TAH_HOOK_VALUES = 0, TAH_HOOK_RELATIVE = 1 };
+tah_status tah_fill_boundary(tah_plan* p, double* work, const double* factors, int32_t kind,
+                             const double* values, const double* fr, void* stream);
 
 /* The staged fill with its launches run one after another on `stream`, each
  * timed by HIP events (for measurement: the same kernels and results as

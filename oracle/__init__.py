@@ -37,6 +37,9 @@ def lib():
         L.oracle_fill.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
         L.oracle_fill_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_write_counts.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_fill_boundary.restype = C.c_int
+        L.oracle_fill_boundary.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                           C.c_void_p, C.c_int]
         L.oracle_max_threads.restype = C.c_int
         L.oracle_regrid.restype = C.c_int
         L.oracle_regrid.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
@@ -61,6 +64,16 @@ def fill(desc_ptr, layout, work, factors=None, nthreads=0, skip=True):
                            _ptr(work), _ptr(factors), int(nthreads), 1 if skip else 0)
     if rc:
         raise RuntimeError(f"oracle_fill failed ({rc})")
+
+
+def fill_boundary(desc_ptr, layout, work, factors, kind, values, fr=None, nthreads=0):
+    """The fill with the outer-face hook between its stage groups (kind 0:
+    values after phase 1; kind 1: relative, values = f(x_g), fr = f(x_r)),
+    the values as contiguous (points, nvars) arrays."""
+    rc = lib().oracle_fill_boundary(C.cast(desc_ptr, C.c_void_p), C.cast(C.pointer(layout), C.c_void_p),
+                                    _ptr(work), _ptr(factors), int(kind), _ptr(values), _ptr(fr), int(nthreads))
+    if rc:
+        raise RuntimeError(f"oracle_fill_boundary failed ({rc})")
 
 
 def fill_adjoint(desc_ptr, layout, adj, factors=None):

@@ -18,7 +18,9 @@
  *     src/schedule.jl:86-89); then scaled (:369-370): acc*f + 0 when the group
  *     has a factor column;
  *   - exchange_ghosts! (src/ghosts.jl:1099-1160): stages in order (P, 0,
- *     phase 1, prolongation levels coarsest first).
+ *     phase 1, prolongation levels coarsest first), and with the outer-face
+ *     hook (oracle_fill_boundary) the hook between the stage groups, its
+ *     values given by the caller.
  * Built with -ffp-contract=off so that no product and sum fuse.
  *
  * Also: oracle_fill_adjoint, the transpose of the same linear map assembled
@@ -127,6 +129,29 @@ static void run_transfers(const tah_group_desc* g, const tah_layout* L, double* 
   }
 }
 
+/* One stage of a fill as one parallel loop over T threads. */
+static void run_stage(const tah_schedule_desc* d, const tah_layout* L, double* work, const double* factors,
+                      int32_t stage, int T, int skip) {
+#pragma omp parallel num_threads(T)
+  {
+    int t = omp_get_thread_num();
+    int nt = omp_get_num_threads();
+    span_t own = equalsplit(d->nblocks, nt, t);
+    for (int32_t gi = 0; gi < d->ngroups; ++gi) {
+      const tah_group_desc* g = &d->groups[gi];
+      if (g->stage != stage) continue;
+      span_t tr = owned_transfers(g, own.lo, own.hi);
+      if (tr.lo < tr.hi) run_transfers(g, L, work, work, factors, tr.lo, tr.hi, skip);
+    }
+  }
+}
+
+static int fill_threads(const tah_schedule_desc* d, int nthreads) {
+  int T = nthreads > 0 ? nthreads : omp_get_max_threads();
+  if (T > d->nblocks) T = d->nblocks > 0 ? d->nblocks : 1;
+  return T;
+}
+
 /* One full fill. nthreads <= 0: all OpenMP threads. skip: the CPU's one-tap
  * skip (1) or the device's full sum (0). */
 int oracle_fill(const tah_schedule_desc* d, const tah_layout* L, double* work,
@@ -134,21 +159,73 @@ int oracle_fill(const tah_schedule_desc* d, const tah_layout* L, double* work,
   if (!d || !L || !work) return 1;
   for (int32_t gi = 0; gi < d->ngroups; ++gi)
     if (d->groups[gi].factorcol && !factors) return 2;
-  int T = nthreads > 0 ? nthreads : omp_get_max_threads();
-  if (T > d->nblocks) T = d->nblocks > 0 ? d->nblocks : 1;
-  for (int32_t stage = 0; stage < d->nstages; ++stage) {
-#pragma omp parallel num_threads(T)
-    {
-      int t = omp_get_thread_num();
-      int nt = omp_get_num_threads();
-      span_t own = equalsplit(d->nblocks, nt, t);
-      for (int32_t gi = 0; gi < d->ngroups; ++gi) {
-        const tah_group_desc* g = &d->groups[gi];
-        if (g->stage != stage) continue;
-        span_t tr = owned_transfers(g, own.lo, own.hi);
-        if (tr.lo < tr.hi) run_transfers(g, L, work, work, factors, tr.lo, tr.hi, skip);
-      }
+  const int T = fill_threads(d, nthreads);
+  for (int32_t stage = 0; stage < d->nstages; ++stage) run_stage(d, L, work, factors, stage, T, skip);
+  return 0;
+}
+
+/* The hook over the regions of one kind: every region (relative = 0, the
+ * values form: boundary_kernel!, src/ghosts.jl:1177-1200), or the regions
+ * whose slot is the tag of a stage of stage group k (relative = 1:
+ * relative_kernel!, :1240-1260, which sets each ghost to f(x_g) + (u(r) -
+ * f(x_r)), r the ghost's point with its clamped components replaced, as
+ * relative_point does, :1239). Point p of a region is (i, j, k) of its box
+ * with i fastest, numbered from the region's `first`; its values sit at
+ * p*nvars + v. */
+static void apply_hook(const tah_schedule_desc* d, const tah_layout* L, double* work, int relative,
+                       const double* values, const double* fr, int32_t k) {
+  const int32_t nv = L->nvars;
+  for (int32_t ri = 0; ri < d->nboundaries; ++ri) {
+    const tah_boundary_region* R = &d->boundaries[ri];
+    if (relative) {
+      int in = 0;
+      for (int32_t st = d->stagegroup_first[k]; st <= d->stagegroup_last[k]; ++st)
+        in = in || d->stage_tags[st] == R->slot;
+      if (!in) continue;
     }
+    const int32_t n0 = R->hi[0] - R->lo[0], n1 = R->hi[1] - R->lo[1];
+    double* b = work + L->stride[4] * R->block;
+    for (int32_t z = R->lo[2]; z < R->hi[2]; ++z)
+      for (int32_t y = R->lo[1]; y < R->hi[1]; ++y)
+        for (int32_t x = R->lo[0]; x < R->hi[0]; ++x) {
+          const int64_t p =
+              R->first + ((int64_t)(z - R->lo[2]) * n1 + (y - R->lo[1])) * n0 + (x - R->lo[0]);
+          const int32_t rx = R->clamps[0] >= 0 ? R->clamps[0] : x;
+          const int32_t ry = R->clamps[1] >= 0 ? R->clamps[1] : y;
+          const int32_t rz = R->clamps[2] >= 0 ? R->clamps[2] : z;
+          for (int32_t v = 0; v < nv; ++v) {
+            const int64_t at = p * nv + v;
+            double u = values[at];
+            if (relative)
+              u = u + (b[L->stride[0] * rx + L->stride[1] * ry + L->stride[2] * rz + L->stride[3] * v] - fr[at]);
+            b[L->stride[0] * x + L->stride[1] * y + L->stride[2] * z + L->stride[3] * v] = u;
+          }
+        }
+  }
+}
+
+/* fill_ghosts!(fs, schedule; boundary) (exchange_ghosts!,
+ * src/ghosts.jl:1099-1160): the stage groups in order; after phase 1's
+ * group the values hook over every region (kind 0; apply_boundary!,
+ * :1134-1136), or after each group the relative hook over the regions whose
+ * slot stage it holds (kind 1; phase 1's through apply_boundary!, the
+ * others through boundary_after_stage!, :1151-1156). */
+int oracle_fill_boundary(const tah_schedule_desc* d, const tah_layout* L, double* work, const double* factors,
+                         int kind, const double* values, const double* fr, int nthreads) {
+  if (!d || !L || !work) return 1;
+  for (int32_t gi = 0; gi < d->ngroups; ++gi)
+    if (d->groups[gi].factorcol && !factors) return 2;
+  if (kind != 0 && kind != 1) return 3;
+  if (d->boundary_points > 0 && (!values || (kind == 1 && !fr))) return 4;
+  const int T = fill_threads(d, nthreads);
+  for (int32_t k = 0; k < d->nstagegroups; ++k) {
+    int phase1 = 0;
+    for (int32_t st = d->stagegroup_first[k]; st <= d->stagegroup_last[k]; ++st) {
+      run_stage(d, L, work, factors, st, T, 1);
+      phase1 = phase1 || d->stage_tags[st] == TAH_STAGE_PHASE1;
+    }
+    if (kind == 0 && phase1) apply_hook(d, L, work, 0, values, NULL, k);
+    if (kind == 1) apply_hook(d, L, work, 1, values, fr, k);
   }
   return 0;
 }

@@ -594,3 +594,26 @@ def test_arena_layout_gives_the_same_values_as_the_dense_one():
     oracle.fill(s._descptr, dense.layout, dense.work.numpy(), dense.factors.numpy())
     oracle.fill(s._descptr, arena.layout, arena.work.numpy(), arena.factors.numpy())
     assert np.array_equal(dense.array().numpy().view(np.int64), arena.array().numpy().view(np.int64))
+
+
+@pytest.mark.parametrize("D", [1, 2, 3])
+@pytest.mark.parametrize("nvars", [1, 2])
+def test_layouts_of_every_dimension_validate(D, nvars):
+    """Dense and arena layouts of 1, 2 and 3 dimensions and of one variable
+    (axes of extent one, which add no offset) pass the stride check; a layout
+    whose axes overlap is refused (tah_layout_size)."""
+    import ctypes as C
+    f = Forest((2,) * D, N=8)
+    for layout in ("dense", "arena"):
+        fs = FieldSet(f, nvars, G=2, device="cpu", layout=layout)
+        assert fs.nelements >= fs.nblocks * nvars * int(np.prod(fs.stored))
+    bad = _abi.tah_layout()
+    for d in range(3):
+        bad.stored[d] = 4
+    bad.nvars, bad.nblocks = 2, 3
+    for d, s in enumerate((1, 4, 16, 32, 128)):
+        bad.stride[d] = s
+    bad.stride[3] = 8  # variables over the z rows
+    n = C.c_int64()
+    with pytest.raises(_abi.LayoutError, match="overlap"):
+        _abi.check(_abi.lib().tah_layout_size(C.byref(bad), C.byref(n)))

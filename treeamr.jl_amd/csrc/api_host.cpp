@@ -34,6 +34,7 @@ struct tah_schedule {
   std::vector<std::array<std::vector<int32_t>, kMaxDim>> srcstart0;
   std::vector<tah_group_desc> gdesc;
   std::vector<int32_t> tags, sgfirst, sglast;
+  std::vector<tah_boundary_region> bregions;
   tah_schedule_desc desc{};
   // Stages P and 0 alone, in the working array's indices (0) or a state
   // vector's (1), built on first request (tah_unify_desc_get).
@@ -89,6 +90,26 @@ void flatten(tah_schedule* S, int nblocks) {
   d.groups = S->gdesc.data();
   d.generation = gs.generation;
   d.nsources = gs.nsources;
+  int64_t first = 0;
+  for (const BoundaryRegion& r : gs.boundaries) {
+    tah_boundary_region e{};
+    e.block = r.block;
+    int64_t n = 1;
+    for (int k = 0; k < kMaxDim; ++k) {
+      e.direction[k] = r.direction[k];
+      e.lo[k] = r.lo[k] - 1;
+      e.hi[k] = r.hi[k];
+      e.clamps[k] = r.clamps[k] - 1;
+      n *= std::max(0, e.hi[k] - e.lo[k]);
+    }
+    e.slot = r.slot;
+    e.first = first;
+    first += n;
+    S->bregions.push_back(e);
+  }
+  d.nboundaries = int32_t(S->bregions.size());
+  d.boundaries = S->bregions.empty() ? nullptr : S->bregions.data();
+  d.boundary_points = first;
 }
 
 // Stages P and 0 of a schedule (unify_arrays!, src/ghosts.jl:1874-1906): on
@@ -158,6 +179,9 @@ std::unique_ptr<FlatDesc> unify_desc(const tah_schedule* S, bool state) {
   d.stagegroup_last = U->sglast.data();
   d.ngroups = int32_t(U->gdesc.size());
   d.groups = U->gdesc.data();
+  d.nboundaries = 0;
+  d.boundaries = nullptr;
+  d.boundary_points = 0;
   return U;
 }
 

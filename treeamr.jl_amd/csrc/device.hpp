@@ -154,6 +154,27 @@ struct alignas(16) Job {
   int32_t pairs;
 };
 
+// An outer-face hook region in HBM: its block, box (first stored index lo,
+// extents n), the nearest evolved point's index along each clamped
+// dimension (-1: the ghost's own index) and its first point in the caller's
+// value arrays.
+struct alignas(16) HookRegion {
+  int32_t block;
+  int32_t lo[3];
+  int32_t n[3];
+  int32_t clamp[3];
+  int64_t first;
+};
+
+// A hook launch's share of one region: values e0 .. e0 + kHookTile - 1 of
+// its point-major, variable-innermost numbering (256 threads, four each).
+constexpr int kHookPerThread = 4;
+constexpr int kHookTile = 256 * kHookPerThread;
+struct alignas(8) HookTile {
+  int32_t region;
+  int32_t e0;
+};
+
 // A slice of work. Copy and generic tiles: group `item`, transfer, flat
 // elements from e0. Interpolation tiles: class `item`, nu units from unit e0
 // of transfer `transfer` on through the next transfers (unit e0+i is unit

@@ -201,7 +201,8 @@ void validate_strides(const tah_layout* l) {
   for (int k = 0; k < 3; ++k)
     if (l->stored[k] < 1) fail("the layout's stored sizes must be positive", kLayout);
   // Strides positive and the five axes nested without overlap: sorted by
-  // stride, each at least the span of the ones inside it.
+  // stride, each at least the span of the ones inside it. An axis of extent
+  // one adds no offset and takes no part.
   int64_t ext[5] = {l->stored[0], l->stored[1], l->stored[2], l->nvars, l->nblocks};
   int order[5] = {0, 1, 2, 3, 4};
   for (int k = 0; k < 5; ++k)
@@ -209,6 +210,7 @@ void validate_strides(const tah_layout* l) {
   std::sort(order, order + 5, [&](int a, int b) { return l->stride[a] < l->stride[b]; });
   int64_t span = 1;
   for (int k = 0; k < 5; ++k) {
+    if (ext[order[k]] == 1) continue;
     if (l->stride[order[k]] < span) fail("the layout's strides overlap", kLayout);
     span = l->stride[order[k]] * ext[order[k]];
   }
@@ -262,6 +264,7 @@ tah_status tah_layout_arena(const tah_schedule_desc* d, int32_t nvars, tah_layou
 tah_status tah_layout_size(const tah_layout* l, int64_t* elements) {
   return guard([&] {
     if (!l || !elements) fail("null argument");
+    validate_strides(l);
     *elements = l->stride[4] * l->nblocks;
   });
 }
@@ -375,6 +378,35 @@ tah_status tah_plan_create(tah_ctx* ctx, const tah_schedule_desc* d, const tah_l
       p->values += pts * g.ntransfers * l->nvars;
       p->groups.push_back(std::move(h));
     }
+    // --- the outer-face hook regions -------------------------------------
+    if (d->nboundaries < 0 || (d->nboundaries > 0 && !d->boundaries)) fail("bad hook region list");
+    if (p->regrid && d->nboundaries > 0) fail("a regrid schedule has no hook regions");
+    int64_t first = 0;
+    for (int32_t r = 0; r < d->nboundaries; ++r) {
+      const tah_boundary_region& B = d->boundaries[r];
+      const std::string where = "hook region " + std::to_string(r) + ": ";
+      if (B.block < 0 || B.block >= d->nblocks) fail(where + "block index out of range");
+      HookRegion h{};
+      h.block = B.block;
+      int64_t n = 1;
+      for (int k = 0; k < 3; ++k) {
+        if (B.lo[k] < 0 || B.hi[k] < B.lo[k] || B.hi[k] > l->stored[k]) fail(where + "box outside the stored extent");
+        if (B.clamps[k] < -1 || B.clamps[k] >= l->stored[k])
+          fail(where + "clamped index outside the stored extent");
+        h.lo[k] = B.lo[k];
+        h.n[k] = B.hi[k] - B.lo[k];
+        h.clamp[k] = B.clamps[k];
+        n *= h.n[k];
+      }
+      if (B.first != first) fail(where + "its first point does not follow the previous region's");
+      if (n * l->nvars >= (int64_t(1) << 31)) fail(where + "too large", kUnsupported);
+      h.first = first;
+      first += n;
+      p->hooks.push_back(h);
+      p->hook_slots.push_back(B.slot);
+    }
+    if (first != d->boundary_points) fail("the hook regions' points do not add up to boundary_points");
+    p->hook_points = first;
     build_plan_device(p.get());
     *out = p.release();
   });
@@ -481,6 +513,45 @@ tah_status tah_fill(tah_plan* p, double* work, const double* factors, void* stre
       }
     for (const auto& sg : p->launches)
       run_launches(p->ctx, sg, s, p->djob, p->dtiles, p->dlds_tiles, p->events.data(), p->fork != 0);
+    plan_end(p, s);
+  });
+}
+
+tah_status tah_fill_boundary(tah_plan* p, double* work, const double* factors, int32_t kind, const double* values,
+                             const double* fr, void* stream) {
+  return guard([&] {
+    if (!p || !work) fail("null argument");
+    if (p->regrid) fail("this plan is a regrid transfer's: run it with tah_regrid_transfer");
+    if (kind != TAH_HOOK_VALUES && kind != TAH_HOOK_RELATIVE)
+      fail("the hook kind is TAH_HOOK_VALUES or TAH_HOOK_RELATIVE");
+    if (p->needs_factors && !factors)
+      fail("this schedule mirrors ghosts across a reflecting face, but no factor table was given");
+    if (p->hook_points > 0 && (!values || (kind == TAH_HOOK_RELATIVE && !fr)))
+      fail(kind == TAH_HOOK_RELATIVE ? "the relative hook needs f at the ghosts (values) and at their nearest "
+                                       "evolved points (fr)"
+                                     : "the hook's values are missing");
+    if (p->launches.size() != p->stagegroups.size()) fail("plan launches do not match its stage groups", kHip);
+    HIPCHECK(hipSetDevice(p->ctx->device));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    plan_begin(p, s);
+    plan_upload_job(p, plan_job(p, work, factors), s);
+    if (p->events.empty())
+      for (int i = 0; i < 8; ++i) {
+        hipEvent_t e;
+        HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        p->events.push_back(e);
+      }
+    auto hook = [&](int64_t first, int64_t count, int32_t relative) {
+      if (count == 0) return;
+      hipLaunchKernelGGL(hook_kernel, dim3(uint32_t(count)), dim3(256), 0, s, p->djob, p->dhooks,
+                         p->dhook_tiles + first, values, fr, relative);
+      HIPCHECK(hipGetLastError());
+    };
+    for (size_t k = 0; k < p->launches.size(); ++k) {
+      run_launches(p->ctx, p->launches[k], s, p->djob, p->dtiles, p->dlds_tiles, p->events.data(), p->fork != 0);
+      if (kind == TAH_HOOK_VALUES && int(k) == p->phase1_sg) hook(p->hook_all[0], p->hook_all[1], 0);
+      if (kind == TAH_HOOK_RELATIVE) hook(p->hook_after[k][0], p->hook_after[k][1], 1);
+    }
     plan_end(p, s);
   });
 }

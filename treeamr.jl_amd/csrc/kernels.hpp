@@ -115,6 +115,50 @@ __global__ void __launch_bounds__(256) plain_kernel(DevGroup g, double* __restri
 
 #endif  // TAH_DEFINE_KERNELS
 
+// The outer-face hook (the reference's boundary_kernel! and relative
+// kernels, src/ghosts.jl:1177-1260) with its values computed by the caller:
+// value e of a tile's region is variable e mod nvars of point e div nvars,
+// the point (i, j, k) of the region's box with i fastest, the caller's
+// value at index (first + point)*nvars + variable. Relative: the ghost
+// becomes values + (work[r] - fr), r the point with the clamped indices
+// replaced; r never lies in a region of the same launch (an evolved point,
+// or a ghost a transfer filled).
+
+#ifdef TAH_DEFINE_KERNELS
+__global__ void __launch_bounds__(256) hook_kernel(const Job* __restrict__ jobs,
+                                                   const HookRegion* __restrict__ regions,
+                                                   const HookTile* __restrict__ tiles,
+                                                   const double* __restrict__ values,
+                                                   const double* __restrict__ fr, int32_t relative) {
+  const Job& job = jobs[0];
+  const HookTile t = tiles[blockIdx.x];
+  const HookRegion R = regions[t.region];
+  const int32_t nv = job.nvars;
+  const uint32_t total = uint32_t(R.n[0]) * uint32_t(R.n[1]) * uint32_t(R.n[2]) * uint32_t(nv);
+  const int64_t s0 = job.stride[0], s1 = job.stride[1], s2 = job.stride[2], s3 = job.stride[3];
+  double* b = job.work + job.stride[4] * R.block;
+  for (int r = 0; r < kHookPerThread; ++r) {
+    const uint32_t e = uint32_t(t.e0) + uint32_t(r * 256) + threadIdx.x;
+    if (e >= total) break;
+    const uint32_t p = e / uint32_t(nv);
+    const int32_t v = int32_t(e - p * uint32_t(nv));
+    const uint32_t q = p / uint32_t(R.n[0]);
+    const int32_t i = R.lo[0] + int32_t(p - q * uint32_t(R.n[0]));
+    const int32_t j = R.lo[1] + int32_t(q % uint32_t(R.n[1]));
+    const int32_t k = R.lo[2] + int32_t(q / uint32_t(R.n[1]));
+    const int64_t at = (R.first + int64_t(p)) * nv + v;
+    double x = values[at];
+    if (relative) {
+      const int32_t ri = R.clamp[0] >= 0 ? R.clamp[0] : i;
+      const int32_t rj = R.clamp[1] >= 0 ? R.clamp[1] : j;
+      const int32_t rk = R.clamp[2] >= 0 ? R.clamp[2] : k;
+      x = __dadd_rn(x, __dsub_rn(b[s0 * ri + s1 * rj + s2 * rk + s3 * v], fr[at]));
+    }
+    b[s0 * i + s1 * j + s2 * k + s3 * v] = x;
+  }
+}
+#endif  // TAH_DEFINE_KERNELS
+
 // Split a flat element of a (box x nvars) space: variables fastest when
 // they are innermost in memory, else x fastest.
 __device__ __forceinline__ void split_element(const DevGroup& g, const Job& job, uint32_t e,

@@ -305,6 +305,60 @@ void classify(tah_plan* p) {
   }
 }
 
+// The hook's tiles and the boxes the relative form reads (tah_plan). A
+// region whose slot names no stage of the plan is never run by the
+// relative form, as in the reference, whose relative batches run only
+// after a stage of their slot (src/ghosts.jl:1527-1531, 1546-1547).
+void build_hooks(tah_plan* p) {
+  const int nsg = int(p->stagegroups.size());
+  p->phase1_sg = -1;
+  for (int k = 0; k < nsg; ++k)
+    for (int st = p->stagegroups[k][0]; st <= p->stagegroups[k][1]; ++st)
+      if (p->tags[st] == TAH_STAGE_PHASE1) p->phase1_sg = k;
+  auto sg_of_slot = [&](int32_t slot) {
+    for (int k = 0; k < nsg; ++k)
+      for (int st = p->stagegroups[k][0]; st <= p->stagegroups[k][1]; ++st)
+        if (p->tags[st] == slot) return k;
+    return -1;
+  };
+  const int32_t nvars = p->layout.nvars;
+  std::vector<HookTile> tiles;
+  auto push = [&](int32_t r) {
+    const HookRegion& R = p->hooks[size_t(r)];
+    const int64_t total = int64_t(R.n[0]) * R.n[1] * R.n[2] * nvars;
+    for (int64_t e = 0; e < total; e += kHookTile) tiles.push_back(HookTile{r, int32_t(e)});
+  };
+  for (size_t r = 0; r < p->hooks.size(); ++r) push(int32_t(r));
+  p->hook_all[0] = 0;
+  p->hook_all[1] = int64_t(tiles.size());
+  p->hook_after.assign(size_t(nsg), {0, 0});
+  p->hook_reads.assign(size_t(nsg), {});
+  for (int k = 0; k < nsg; ++k) {
+    p->hook_after[size_t(k)][0] = int64_t(tiles.size());
+    for (size_t r = 0; r < p->hooks.size(); ++r) {
+      if (sg_of_slot(p->hook_slots[r]) != k) continue;
+      push(int32_t(r));
+      const HookRegion& R = p->hooks[r];
+      std::array<int32_t, 7> box{R.block, 0, 0, 0, 0, 0, 0};
+      for (int d = 0; d < 3; ++d) {
+        box[1 + d] = R.clamp[d] >= 0 ? R.clamp[d] : R.lo[d];
+        box[4 + d] = R.clamp[d] >= 0 ? R.clamp[d] : R.lo[d] + R.n[d] - 1;
+      }
+      p->hook_reads[size_t(k)].push_back(box);
+    }
+    p->hook_after[size_t(k)][1] = int64_t(tiles.size()) - p->hook_after[size_t(k)][0];
+  }
+  if (tiles.size() > size_t(INT32_MAX / 2)) fail("too many hook tiles", kUnsupported);
+  if (!p->hooks.empty()) {
+    HIPCHECK(hipMalloc(&p->dhooks, sizeof(HookRegion) * p->hooks.size()));
+    HIPCHECK(hipMemcpy(p->dhooks, p->hooks.data(), sizeof(HookRegion) * p->hooks.size(), hipMemcpyHostToDevice));
+  }
+  if (!tiles.empty()) {
+    HIPCHECK(hipMalloc(&p->dhook_tiles, sizeof(HookTile) * tiles.size()));
+    HIPCHECK(hipMemcpy(p->dhook_tiles, tiles.data(), sizeof(HookTile) * tiles.size(), hipMemcpyHostToDevice));
+  }
+}
+
 void build_plan_device(tah_plan* p) {
   classify(p);
   Blob blob;
@@ -378,6 +432,7 @@ void build_plan_device(tah_plan* p) {
   HIPCHECK(hipMalloc(&p->djob, sizeof(Job)));
   std::memset(&p->last_job, 0, sizeof(Job));
   HIPCHECK(hipEventCreateWithFlags(&p->done, hipEventDisableTiming));
+  build_hooks(p);
   if (p->regrid) {
     HIPCHECK(hipMalloc(&p->djob2, sizeof(Job)));
     std::memset(&p->last_job2, 0, sizeof(Job));
@@ -425,6 +480,11 @@ void build_launches(tah_plan* p) {
   {
     std::map<int32_t, std::vector<Box3>> later_reads, later_writes;
     for (int k = nsg - 1; k >= 0; --k) {
+      // the relative hook after this group reads what this group and the
+      // ones before it wrote
+      if (size_t(k) < p->hook_reads.size())
+        for (const auto& h : p->hook_reads[size_t(k)])
+          later_reads[h[0]].push_back(Box3{{h[1], h[2], h[3]}, {h[4], h[5], h[6]}});
       if (k < nsg - 1) {
         for (size_t gi = 0; gi < p->groups.size(); ++gi) {
           const PlanGroup& g = p->groups[gi];
@@ -857,4 +917,6 @@ tah_plan::~tah_plan() {
   if (dunits) (void)hipFree(dunits);
   if (djob) (void)hipFree(djob);
   if (djob2) (void)hipFree(djob2);
+  if (dhooks) (void)hipFree(dhooks);
+  if (dhook_tiles) (void)hipFree(dhook_tiles);
 }

@@ -155,6 +155,22 @@ struct tah_plan {
   // queue took 26.4 ms against 21.5 ms, its one register budget holding the
   // copies to four waves per SIMD).
   int32_t queue = 0;
+  // The outer-face hook (tah_fill_boundary): the regions in HBM and their
+  // tiles -- every region's first (the values form, after phase 1's stage
+  // group), then per stage group those of the regions whose slot stage it
+  // holds (the relative form, after that group) -- and, per stage group,
+  // the boxes the relative form reads after it (block, lo[3], hi[3]
+  // inclusive), which no transfer of that group or before may be deferred
+  // past.
+  std::vector<tah::HookRegion> hooks;
+  std::vector<int32_t> hook_slots;
+  int64_t hook_points = 0;
+  tah::HookRegion* dhooks = nullptr;
+  tah::HookTile* dhook_tiles = nullptr;
+  int64_t hook_all[2] = {0, 0};
+  std::vector<std::array<int64_t, 2>> hook_after;
+  std::vector<std::vector<std::array<int32_t, 7>>> hook_reads;
+  int32_t phase1_sg = -1;
   std::vector<hipEvent_t> events;  // fork/join events, created on first use
   // The adjoint's inverted schedule, built on first use (adjoint.hip).
   tah::AdjointTables* adj = nullptr;

@@ -829,7 +829,16 @@ struct Builder {
     }
   }
 
+  // A hook region with its relative-hook fields (src/schedule.jl:2188-2217).
+  // A component of delta leaves through an outer face when the single step
+  // along it finds no block and no reflecting or outflow face. The rest of
+  // the direction, mirrored at walls, names the block's own ghosts the
+  // nearest evolved point lies in: filled at phase 1, or with the
+  // prolongations onto the block's level where they come from a coarser
+  // block; where it finds no block at all the point is clamped along every
+  // component of delta instead.
   void boundary(int b, const Dir& delta) {
+    const Key& k = F.leaves[b];
     BoundaryRegion r;
     r.block = b;
     r.direction = delta;
@@ -838,6 +847,36 @@ struct Builder {
       r.lo[d] = rg.lo;
       r.hi[d] = rg.hi;
     }
+    std::array<bool, kMaxDim> exits{false, false, false};
+    std::array<int8_t, kMaxDim> walls;
+    for (int d = 0; d < D; ++d) {
+      if (delta[d] == 0) continue;
+      Dir step{0, 0, 0};
+      step[d] = delta[d];
+      F.wall_direction(k, step, walls);
+      exits[d] = walls[d] == OuterFace && F.neighbors(k, step).empty();
+    }
+    Dir rest{0, 0, 0};
+    bool anyrest = false;
+    for (int d = 0; d < D; ++d) {
+      rest[d] = exits[d] ? 0 : delta[d];
+      anyrest = anyrest || rest[d] != 0;
+    }
+    if (anyrest) {
+      const Dir mirrored = F.wall_direction(k, rest, walls);
+      bool anym = false;
+      for (int d = 0; d < D; ++d) anym = anym || mirrored[d] != 0;
+      if (anym) {
+        const std::vector<int> nbrs = F.neighbors(k, mirrored);
+        if (nbrs.empty()) {
+          for (int d = 0; d < D; ++d) exits[d] = delta[d] != 0;
+        } else if (F.leaves[nbrs[0]].level < k.level) {
+          r.slot = prolongation_tag(k.level);
+        }
+      }
+    }
+    for (int d = 0; d < D; ++d)
+      r.clamps[d] = !exits[d] ? 0 : delta[d] > 0 ? G[d] + N + c[d] : G[d] + 1;
     boundaries.push_back(r);
   }
 
@@ -1177,6 +1216,22 @@ GhostSchedule::GhostSchedule(const Forest& f, const Operators& ops_, const int* 
           access[si].writes.push_back({g.targetblocks[t], wb});
         }
       }
+    // A hook region whose relative form runs after a prolongation stage
+    // writes its box with that stage and reads its nearest evolved points
+    // (src/schedule.jl:2907-2911, hook_read_box :2795-2797).
+    for (const BoundaryRegion& r : boundaries) {
+      int si = -1;
+      for (int i = 0; i < (int)tags.size(); ++i)
+        if (tags[i] == r.slot) si = i;
+      if (si <= p1) continue;
+      Box wb, rb;
+      for (int d = 0; d < kMaxDim; ++d) {
+        wb[d] = Range{r.lo[d], r.hi[d]};
+        rb[d] = r.clamps[d] == 0 ? wb[d] : Range{r.clamps[d], r.clamps[d]};
+      }
+      access[si].writes.push_back({r.block, wb});
+      access[si].reads.push_back({r.block, rb});
+    }
     std::unordered_map<int, std::vector<Box>> written, read;
     auto meets = [&](std::unordered_map<int, std::vector<Box>>& boxes,
                      const std::vector<std::pair<int, Box>>& list) {

@@ -88,12 +88,19 @@ struct TransferGroup {
   }
 };
 
-// A region of an outer face, filled by the application's hook in the
-// reference (src/schedule.jl:200-260); the fill never writes it.
+// A region of an outer face, filled by the boundary hook, never by a
+// transfer (BoundaryRegion, src/schedule.jl:200-229). For the relative hook:
+// clamps, per dimension, the 1-based stored index of the nearest evolved
+// point along a component of the direction that leaves the domain through
+// an outer face (0 along every other, where the point keeps the ghost's own
+// index), and slot, the tag of the stage after which the ghosts it reads
+// are filled.
 struct BoundaryRegion {
   int32_t block;
   Dir direction;
   std::array<int, kMaxDim> lo, hi;  // 1-based inclusive stored ranges
+  std::array<int, kMaxDim> clamps{0, 0, 0};
+  int slot = kPhase1Tag;
 };
 
 struct Stage {

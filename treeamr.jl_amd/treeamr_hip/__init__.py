@@ -19,5 +19,7 @@ from .core import (  # noqa: F401
     Context, default_context, Plan, fill_ghosts_batched, FillGraph, fill_ghosts_timed, kernel_class_name, tune,
     ownextents, statelength, statevector, staterange, stateview, statearray, scatter, gather,
     RegridSchedule, regrid, regrid_transfer, UnifySchedule, unify, unify_desc,
+    AllVariables, WithPatch, Relative, CellBoundary, boundary_by_coordinates, boundary_values,
+    fill_by_coordinates,
 )
 from ._abi import TahError, StaleScheduleError, LayoutError, UnsupportedError  # noqa: F401

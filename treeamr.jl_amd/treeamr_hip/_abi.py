@@ -10,6 +10,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(os.path.dirname(_HERE), "lib", "libtah.so")
 
 TAH_OK, TAH_EINVAL, TAH_ESTALE, TAH_ELAYOUT, TAH_EHIP, TAH_ENOMEM, TAH_EUNSUPPORTED = range(7)
+TAH_HOOK_VALUES, TAH_HOOK_RELATIVE = 0, 1
 
 
 class tah_operators(C.Structure):
@@ -29,13 +30,20 @@ class tah_group_desc(C.Structure):
                 ("sourceblocks", C.POINTER(C.c_int32)), ("stencils", tah_stencil1d * 3)]
 
 
+class tah_boundary_region(C.Structure):
+    _fields_ = [("block", C.c_int32), ("direction", C.c_int32 * 3), ("lo", C.c_int32 * 3),
+                ("hi", C.c_int32 * 3), ("clamps", C.c_int32 * 3), ("slot", C.c_int32), ("first", C.c_int64)]
+
+
 class tah_schedule_desc(C.Structure):
     _fields_ = [("dim", C.c_int32), ("nblocks", C.c_int32), ("stored", C.c_int32 * 3),
                 ("nstages", C.c_int32), ("stage_tags", C.POINTER(C.c_int32)),
                 ("nstagegroups", C.c_int32), ("stagegroup_first", C.POINTER(C.c_int32)),
                 ("stagegroup_last", C.POINTER(C.c_int32)), ("ngroups", C.c_int32),
                 ("groups", C.POINTER(tah_group_desc)), ("generation", C.c_int64),
-                ("nsources", C.c_int32), ("pad_", C.c_int32)]
+                ("nsources", C.c_int32), ("pad_", C.c_int32), ("nboundaries", C.c_int32),
+                ("pad2_", C.c_int32), ("boundaries", C.POINTER(tah_boundary_region)),
+                ("boundary_points", C.c_int64)]
 
 
 class tah_layout(C.Structure):
@@ -84,6 +92,7 @@ SIGNATURES = {
     "tah_plan_values": [P, C.POINTER(C.c_int64)],
     "tah_fill_plain": [P, P, P, P],
     "tah_fill": [P, P, P, P],
+    "tah_fill_boundary": [P, P, P, C.c_int32, P, P, P],
     "tah_fill_batched": [P, C.c_int32, C.POINTER(P), C.POINTER(P), C.POINTER(P), P],
     "tah_graph_create": [P, C.c_int32, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.POINTER(P)],
     "tah_graph_launch": [P, P],

@@ -497,21 +497,247 @@ def _stream_ptr(fs):
     return C.c_void_p(torch.cuda.current_stream(fs.work.device).cuda_stream)
 
 
+# --- callbacks of position (src/storage.jl, src/ghosts.jl) ------------------
+#
+# The reference hands its callbacks one point at a time inside a kernel. Here
+# a callback is called once per use over every point at once: a position x
+# is a tuple of D float64 tensors (one entry per point, on the field set's
+# device), a direction a tuple of D integer tensors, a patch an integer
+# tensor (1 on every block of a brick forest), and a variable index v is
+# 1-based, as in the reference. A callback returns a tensor over the points
+# (or a number), or, wrapped in AllVariables, nvars of them (a sequence, or
+# a tensor whose last axis is the variable). Written with torch operations
+# it runs on the GPU.
+
+class AllVariables:
+    """AllVariables(f): the callback returns every variable's value at a point
+    at once, instead of taking the variable index."""
+
+    def __init__(self, f):
+        self.f = f
+
+
+class WithPatch:
+    """WithPatch(f): the callback is handed the point's patch after the
+    position."""
+
+    def __init__(self, f):
+        self.f = f
+
+
+class Relative:
+    """Relative(f) (src/ghosts.jl:1419-1465): a ghost beyond an outer face
+    becomes f(x_g) + (u(x_r) - f(x_r)), x_r its nearest evolved point; f(x)
+    gives one variable's value (a field set of one variable) or, wrapped in
+    AllVariables, all of them."""
+
+    def __init__(self, f):
+        self.f = f
+
+
+class CellBoundary:
+    """CellBoundary(g; wall) (src/ghosts.jl:1321-1413): the outer-face hook
+    per ghost point, g(x, v, delta) -> value. Forms: CellBoundary(g),
+    CellBoundary(AllVariables(g)) with g(x, delta) -> values,
+    CellBoundary(WithPatch(g)) with g(x, p, v, delta),
+    CellBoundary(WithPatch(AllVariables(g))) with g(x, p, delta),
+    CellBoundary(Relative(f)) and CellBoundary(Relative(AllVariables(f))).
+    wall="evolved" (the default) leaves a vertex-like dimension's wall plane
+    to the application; wall="prescribed" (stage W) is not provided."""
+
+    def __init__(self, g, *, wall="evolved"):
+        if wall not in ("evolved", "prescribed"):
+            raise ValueError("wall must be 'evolved' (the wall plane of a vertex-like dimension is an own "
+                             "point of the state, evolved by the application, and the hook fills the ghosts "
+                             "beyond it) or 'prescribed' (the hook gives the wall plane its value too, a "
+                             f"Dirichlet condition), got {wall!r}")
+        if wall == "prescribed":
+            raise _abi.UnsupportedError(_abi.TAH_EUNSUPPORTED, "CellBoundary(g, wall='prescribed') (stage "
+                                        "W) is not provided: the wall plane stays the application's")
+        if isinstance(g, Relative) and isinstance(g.f, WithPatch):
+            raise ValueError("CellBoundary(Relative(WithPatch(f))) is not a form the package has: the relative "
+                             "hook calls f at the ghost and at the nearest evolved point, by position alone, "
+                             "and is not handed the patch")
+        self.g = g
+        self.wall = wall
+
+
+def boundary_by_coordinates(f, *, wall="evolved"):
+    """boundary_by_coordinates(f) (src/ghosts.jl:2189-2219): CellBoundary of
+    a function of position and variable, f(x, v), or of its AllVariables and
+    WithPatch forms, the direction dropped."""
+    if isinstance(f, WithPatch):
+        if isinstance(f.f, AllVariables):
+            h = f.f.f
+            return CellBoundary(WithPatch(AllVariables(lambda x, p, d: h(x, p))), wall=wall)
+        h = f.f
+        return CellBoundary(WithPatch(lambda x, p, v, d: h(x, p, v)), wall=wall)
+    if isinstance(f, AllVariables):
+        h = f.f
+        return CellBoundary(AllVariables(lambda x, d: h(x)), wall=wall)
+    return CellBoundary(lambda x, v, d: f(x, v), wall=wall)
+
+
+def _values_of(out, npts, nvars, what, device):
+    """A callback's result as a contiguous (npts, nvars) float64 tensor."""
+    import torch
+    if isinstance(out, torch.Tensor) and out.dim() == 2:
+        cols = [out[:, v] for v in range(out.shape[1])]
+    elif isinstance(out, (tuple, list)):
+        cols = list(out)
+    else:
+        cols = [out]
+    if len(cols) != nvars:
+        raise ValueError(f"the {what} must return one value per variable: the field set has nvars = {nvars}, "
+                         f"it returned {len(cols)}")
+    res = torch.empty((npts, nvars), dtype=torch.float64, device=device)
+    for v, c in enumerate(cols):
+        res[:, v] = torch.as_tensor(c, dtype=torch.float64, device=device).expand(npts)
+    return res
+
+
+class _HookPoints:
+    """Every hook ghost point of a schedule's regions, in their numbering
+    (tah_boundary_region.first; i fastest within a region): block, stored
+    index, nearest evolved point, outward direction, and the positions of
+    the ghost and of that point as nodeposition forms them."""
+
+    def __init__(self, fs, schedule):
+        import torch
+        d = schedule.desc
+        D = fs.forest.D
+        P = int(d.boundary_points)
+        block = np.zeros(P, np.int64)
+        idx = np.zeros((P, 3), np.int64)
+        ridx = np.zeros((P, 3), np.int64)
+        dirs = np.zeros((P, 3), np.int64)
+        for r in range(d.nboundaries):
+            R = d.boundaries[r]
+            lo, hi = list(R.lo), list(R.hi)
+            k, j, i = np.meshgrid(np.arange(lo[2], hi[2]), np.arange(lo[1], hi[1]), np.arange(lo[0], hi[0]),
+                                  indexing="ij")
+            a, m = int(R.first), i.size
+            for dd, arr in enumerate((i, j, k)):
+                idx[a:a + m, dd] = arr.ravel()
+                ridx[a:a + m, dd] = R.clamps[dd] if R.clamps[dd] >= 0 else arr.ravel()
+                dirs[a:a + m, dd] = R.direction[dd]
+            block[a:a + m] = R.block
+        dev = fs.work.device
+        pos = [fs.positions(dd) for dd in range(D)]
+        self.npoints = P
+        self.block, self.idx, self.ridx = block, idx, ridx
+        self.xg = tuple(torch.tensor(pos[dd][block, idx[:, dd]], dtype=torch.float64, device=dev)
+                        for dd in range(D))
+        self.xr = tuple(torch.tensor(pos[dd][block, ridx[:, dd]], dtype=torch.float64, device=dev)
+                        for dd in range(D))
+        self.dirs = tuple(torch.tensor(dirs[:, dd], dtype=torch.int64, device=dev) for dd in range(D))
+        self.patch = torch.ones(P, dtype=torch.int64, device=dev)
+
+
+def _hook_points(fs, schedule):
+    cache = schedule.__dict__.setdefault("_hookpoints", {})
+    key = (str(fs.work.device), fs.G, fs.centering)
+    if key not in cache:
+        cache[key] = _HookPoints(fs, schedule)
+    return cache[key]
+
+
+def boundary_values(fs, schedule, hook):
+    """The hook's values at the schedule's hook points: (kind, values, fr),
+    kind _abi.TAH_HOOK_VALUES (values: g at every ghost, fr None) or
+    _abi.TAH_HOOK_RELATIVE (values = f(x_g), fr = f(x_r)), each a contiguous
+    (points, nvars) float64 tensor on the field set's device."""
+    if not isinstance(hook, CellBoundary):
+        raise ValueError("this boundary hook takes a whole region, which is a host form: it indexes the "
+                         "working array cell by cell. Wrap a per-cell function in CellBoundary(g) "
+                         "(g(x, v, delta) -> value), which the package evaluates at every hook point; a "
+                         "condition relative to the nearest evolved point is CellBoundary(Relative(f)), and "
+                         "a zero-gradient one an outflow face")
+    pts = _hook_points(fs, schedule)
+    P, nv, dev = pts.npoints, fs.nvars, fs.work.device
+    g = hook.g
+    if isinstance(g, Relative):
+        f = g.f
+        if isinstance(f, AllVariables):
+            fg = _values_of(f.f(pts.xg), P, nv, "boundary hook", dev)
+            fr = _values_of(f.f(pts.xr), P, nv, "boundary hook", dev)
+        else:
+            if nv != 1:
+                raise ValueError(f"CellBoundary(Relative(f)) sets one variable from f(x), but the field set has "
+                                 f"nvars = {nv}; for several, wrap the function in AllVariables, "
+                                 "CellBoundary(Relative(AllVariables(lambda x: (u1, u2, ...))))")
+            fg = _values_of(f(pts.xg), P, 1, "boundary hook", dev)
+            fr = _values_of(f(pts.xr), P, 1, "boundary hook", dev)
+        return _abi.TAH_HOOK_RELATIVE, fg, fr
+    patch = isinstance(g, WithPatch)
+    inner = g.f if patch else g
+    if isinstance(inner, AllVariables):
+        out = inner.f(pts.xg, pts.patch, pts.dirs) if patch else inner.f(pts.xg, pts.dirs)
+        return _abi.TAH_HOOK_VALUES, _values_of(out, P, nv, "boundary hook", dev), None
+    cols = [inner(pts.xg, pts.patch, v + 1, pts.dirs) if patch else inner(pts.xg, v + 1, pts.dirs)
+            for v in range(nv)]
+    return _abi.TAH_HOOK_VALUES, _values_of(cols, P, nv, "boundary hook", dev), None
+
+
+def fill_by_coordinates(f, fs):
+    """fill_by_coordinates!(f, fs) (src/storage.jl:1359-1418): every own point
+    of every variable from f(x, v), x the positions over (blocks, k, j, i) of
+    the own range as nodeposition forms them; or AllVariables(f), f(x) ->
+    nvars values; or WithPatch(f), f(x, p, v) / f(x, p)."""
+    import torch
+    dev = fs.work.device
+    own = fs.own_slices()
+    D = fs.forest.D
+    x = []
+    for d in range(D):
+        p = torch.tensor(fs.positions(d)[:, own[2 - d]], dtype=torch.float64, device=dev)
+        shape = [fs.nblocks, 1, 1, 1]
+        shape[3 - d] = p.shape[1]
+        x.append(p.reshape(shape))
+    ext = [own[2 - d].stop - own[2 - d].start if d < D else 1 for d in range(3)]
+    full = (fs.nblocks, ext[2], ext[1], ext[0])
+    x = tuple(t.expand(full) for t in x)
+    patch = torch.ones(full, dtype=torch.int64, device=dev)
+    withpatch = isinstance(f, WithPatch)
+    inner = f.f if withpatch else f
+    a = fs.array()
+    if isinstance(inner, AllVariables):
+        out = inner.f(x, patch) if withpatch else inner.f(x)
+        cols = [out[..., v] for v in range(out.shape[-1])] if isinstance(out, torch.Tensor) and \
+            out.dim() == 5 else list(out) if isinstance(out, (tuple, list)) else [out]
+        if len(cols) != fs.nvars:
+            raise ValueError(f"the fill callback must return one value per variable: the field set has nvars = "
+                             f"{fs.nvars}, it returned {len(cols)}")
+    else:
+        cols = [inner(x, patch, v + 1) if withpatch else inner(x, v + 1) for v in range(fs.nvars)]
+    for v, c in enumerate(cols):
+        a[(slice(None), v) + own] = torch.as_tensor(c, dtype=torch.float64, device=dev).expand(full)
+    return fs
+
+
 def fill_ghosts(fs, schedule, *, boundary=None, method="staged", ctx=None):
-    """fill_ghosts!(fs, schedule): every ghost of fs.work from the schedule, on the GPU.
+    """fill_ghosts!(fs, schedule; boundary): every ghost of fs.work from the
+    schedule, on the GPU.
 
     method="staged" (one launch per stage group, the default) or "plain" (one
-    launch per transfer group, the reference's device form).
+    launch per transfer group, the reference's device form). boundary: a
+    CellBoundary hook, evaluated at the schedule's hook points and run
+    between the stage groups as the reference runs it (tah_fill_boundary).
     """
-    if boundary is not None:
-        raise _abi.UnsupportedError(_abi.TAH_EUNSUPPORTED, "boundary hooks are not supported by "
-                                    "the device fill; pre-fill the outer-face regions instead")
     _check_fill(fs, schedule)
     if fs.work.device.type != "cuda":
         raise RuntimeError("fill_ghosts runs on the GPU only: this field set lives on "
                            f"{fs.work.device}; there is no CPU fallback")
     plan = schedule.plan(fs.layout, ctx)
     fptr = C.c_void_p(fs.factors.data_ptr()) if fs.factors is not None else None
+    if boundary is not None:
+        if method != "staged":
+            raise ValueError("the boundary hook runs with the staged fill (method='staged')")
+        kind, values, fr = boundary_values(fs, schedule, boundary)
+        check(lib().tah_fill_boundary(plan._h, C.c_void_p(fs.work.data_ptr()), fptr, kind,
+                                      C.c_void_p(values.data_ptr()),
+                                      C.c_void_p(fr.data_ptr()) if fr is not None else None, _stream_ptr(fs)))
+        return fs
     fn = lib().tah_fill if method == "staged" else lib().tah_fill_plain
     check(fn(plan._h, C.c_void_p(fs.work.data_ptr()), fptr, _stream_ptr(fs)))
     return fs
