@@ -150,6 +150,10 @@ def main():
     ap.add_argument("--no-unify", action="store_true")
     ap.add_argument("--config2-repeats", type=int, default=100)
     ap.add_argument("--batched-steps", type=int, default=3)
+    ap.add_argument("--only-batched", action="store_true",
+                    help="run the 64-forest batch alone (for a kernel trace of it) and print its record")
+    ap.add_argument("--batch-queue", type=int, default=0, choices=[0, 1],
+                    help="the batch's plans' option queue: 1 one persistent work-queue launch per stage position")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="at the end, copy a buffer of this many GiB once (a kernel of known "
                          "traffic, to scale HBM counters by)")
@@ -163,9 +167,48 @@ def main():
     import torch
     from treeamr_hip import _abi
     from treeamr_hip import (FieldSet, FillGraph, GhostSchedule, fill_ghosts, fill_ghosts_adjoint,
-                             fill_ghosts_timed, gather, ownextents, regrid_transfer, scatter, statelength,
-                             tune, Operators, RegridSchedule, unify)
+                             fill_ghosts_adjoint_timed, fill_ghosts_timed, gather, ownextents, regrid_transfer,
+                             scatter, statelength, tune, Operators, RegridSchedule, unify)
     from treeamr_hip import workloads as W
+
+    def run_batched(stream):
+        log("[bench] building the 64-forest batch")
+        tb = time.perf_counter()
+        pairs = W.config3(count=64, layout=args.layout)
+        for f2, s2 in pairs:
+            s2.plan(f2.layout).set_options(queue=args.batch_queue)
+        bvalues = sum(s.points() * f2.nvars for f2, s in pairs)
+        bblocks = sum({id(f2.forest): f2.nblocks for f2, _ in pairs}.values())
+        log(f"[bench] batch: {len(pairs)} field sets, {bblocks} blocks, {bvalues} values; "
+            f"{time.perf_counter() - tb:.1f}s")
+
+        def seq_plain():
+            for f2, s2 in pairs:
+                fill_ghosts(f2, s2, method="plain")
+
+        bplain = time_fill(seq_plain, args.batched_steps, 1, stream)
+        log(f"[bench] batch, 64 sequential plain fills: {bplain}")
+        bgraph_obj = FillGraph(pairs)
+        bgraph = time_fill(bgraph_obj.launch, args.batched_steps * 3, 1, stream)
+        log(f"[bench] batch, graph-captured batched fill: {bgraph}")
+        rec = {
+            "workload": "64 independent forests, seeds 2..65 (1^3-4^3 roots, N in {8,16,32}, 1-4 levels, "
+                        "G in {2,3}, cell or vertex, 1-3 field sets of 2-10 vars, PointValue(2G), periodic, "
+                        "<=2 GiB each)",
+            "field_sets": len(pairs), "blocks": bblocks, "values_per_fill": bvalues,
+            "queue": args.batch_queue,
+            "plain_sequential_ms": round(bplain["mean_ms"], 3),
+            "graph_batched_ms": round(bgraph["mean_ms"], 3),
+            "speedup": round(bplain["mean_ms"] / bgraph["mean_ms"], 2),
+            "graph_effective_GBs": round(16.0 * bvalues / (bgraph["mean_ms"] * 1e-3) / 1e9, 1),
+        }
+        del bgraph_obj, pairs
+        return rec
+
+    if args.only_batched:
+        rec = run_batched(torch.cuda.current_stream())
+        print(json.dumps({"batched": rec}), flush=True)
+        return
 
     def builder(nvars):
         f, ops, kw, _ = W.config5(nvars=nvars)
@@ -197,10 +240,22 @@ def main():
         return bytes_eff / (ms * 1e-3) / 1e9
 
     plain = None
+    plain_dense = None
     if not args.no_plain:
         log("[bench] timing the plain per-group fill")
         plain = time_fill(lambda: fill_ghosts(fs, sched, method="plain"), args.steps, args.warmup, stream)
         log(f"[bench] plain: {plain}")
+        if args.layout != "dense":
+            # the plain fill on the reference's own layout (work[i, j, k, var,
+            # block], i fastest), where its i-fastest threads read contiguous rows
+            log("[bench] timing the plain per-group fill on the dense layout")
+            fsd = FieldSet(f, nvars, device="cuda", layout="dense", fill=0.0, **kw)
+            W.fill_gaussians(fsd, 5)
+            plain_dense = time_fill(lambda: fill_ghosts(fsd, sched, method="plain"), args.steps, args.warmup,
+                                    stream)
+            log(f"[bench] plain (dense layout): {plain_dense}")
+            sched._plans.pop((tuple(fsd.layout.stride), fsd.layout.nvars, 0), None)
+            del fsd
     log("[bench] timing the staged fill")
     staged = time_fill(lambda: fill_ghosts(fs, sched, method="staged"), args.steps, args.warmup, stream)
     log(f"[bench] staged: {staged}")
@@ -245,11 +300,29 @@ def main():
         del tgraph
         sched.plan(fs.layout).set_options(fork=0, merge=1, defer=1, copy_per_thread=8, copy_wgs=0)
     adjoint = None
+    adj_kernels = {}
     if not args.no_adjoint:
         log("[bench] timing the adjoint fill")
         adj = torch.rand_like(fs.work)
         adjoint = time_fill(lambda: fill_ghosts_adjoint(fs, sched, adj), args.steps, args.warmup, stream)
         log(f"[bench] adjoint: {adjoint}")
+        log("[bench] timing every launch of the adjoint (in order, HIP events on its stream)")
+        for it in range(args.warmup + args.steps):
+            recs = fill_ghosts_adjoint_timed(fs, sched, adj)
+            if it < args.warmup:
+                continue
+            for name, nbytes, ms in recs:
+                k = adj_kernels.setdefault(name, dict(launches=0, bytes=0, ms=0.0))
+                k["launches"] += 1
+                k["bytes"] += nbytes
+                k["ms"] += ms
+        for k in adj_kernels.values():
+            k["avg_launch_ms"] = round(k["ms"] / k["launches"], 4)
+            k["bytes_per_launch"] = k["bytes"] // k["launches"]
+            k["GBs"] = round(k["bytes"] / (k["ms"] * 1e-3) / 1e9, 1)
+            k["ms_per_fill"] = round(k["ms"] / args.steps, 4)
+            k["ms"] = round(k["ms"], 3)
+        log(f"[bench] adjoint kernels: {adj_kernels}")
         del adj
     head = tg if tg is not None else graphed
     t_ms = head["mean_ms"]
@@ -257,34 +330,7 @@ def main():
 
     batched = None
     if not args.no_batched:
-        log("[bench] building the 64-forest batch")
-        tb = time.perf_counter()
-        pairs = W.config3(count=64, layout=args.layout)
-        bvalues = sum(s.points() * f2.nvars for f2, s in pairs)
-        bblocks = sum({id(f2.forest): f2.nblocks for f2, _ in pairs}.values())
-        log(f"[bench] batch: {len(pairs)} field sets, {bblocks} blocks, {bvalues} values; "
-            f"{time.perf_counter() - tb:.1f}s")
-
-        def seq_plain():
-            for f2, s2 in pairs:
-                fill_ghosts(f2, s2, method="plain")
-
-        bplain = time_fill(seq_plain, args.batched_steps, 1, stream)
-        log(f"[bench] batch, 64 sequential plain fills: {bplain}")
-        bgraph_obj = FillGraph(pairs)
-        bgraph = time_fill(bgraph_obj.launch, args.batched_steps * 3, 1, stream)
-        log(f"[bench] batch, graph-captured batched fill: {bgraph}")
-        batched = {
-            "workload": "64 independent forests, seeds 2..65 (1^3-4^3 roots, N in {8,16,32}, 1-4 levels, "
-                        "G in {2,3}, cell or vertex, 1-3 field sets of 2-10 vars, PointValue(2G), periodic, "
-                        "<=2 GiB each)",
-            "field_sets": len(pairs), "blocks": bblocks, "values_per_fill": bvalues,
-            "plain_sequential_ms": round(bplain["mean_ms"], 3),
-            "graph_batched_ms": round(bgraph["mean_ms"], 3),
-            "speedup": round(bplain["mean_ms"] / bgraph["mean_ms"], 2),
-            "graph_effective_GBs": round(16.0 * bvalues / (bgraph["mean_ms"] * 1e-3) / 1e9, 1),
-        }
-        del bgraph_obj, pairs
+        batched = run_batched(stream)
 
     nested = None
     if not args.no_config2:
@@ -479,6 +525,16 @@ def main():
     def r(d):
         return {k: round(v, 4) for k, v in d.items()} if d else None
 
+    def adjoint_roofline(ks):
+        # the adjoint's dominant kernel against HBM, as the fill's below
+        if not ks:
+            return None
+        name = max(ks, key=lambda n: ks[n]["ms"])
+        k = ks[name]
+        return {"kernel": name, "bound": "hbm", "achieved": k["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(k["GBs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes_per_launch": k["bytes_per_launch"], "avg_launch_ms": k["avg_launch_ms"]}
+
     copy_ref = device_copy_rate()
     log(f"[bench] device copy of 2 GiB: {copy_ref} GB/s")
 
@@ -509,10 +565,15 @@ def main():
         "graph_ms_default_options": r(graphed),
         "staged_ms": r(staged),
         "plain_ms": r(plain),
+        "plain_dense_layout_ms": r(plain_dense),
         "speedup_vs_plain": round(plain["mean_ms"] / t_ms, 3) if plain else None,
+        "speedup_vs_plain_dense_layout": round(plain_dense["mean_ms"] / t_ms, 3) if plain_dense else None,
         "tuned": tuned,
         "adjoint_ms": r(adjoint),
         "adjoint_over_forward": round(adjoint["mean_ms"] / t_ms, 3) if adjoint else None,
+        "adjoint_effective_GBs": round(gbs_of(adjoint["mean_ms"]), 1) if adjoint else None,
+        "adjoint_roofline": adjoint_roofline(adj_kernels),
+        "adjoint_kernels": adj_kernels or None,
         "batched": batched,
         "config2": nested,
         "reference_meshes": refmeshes,

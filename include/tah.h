@@ -365,6 +365,15 @@ tah_status tah_graph_destroy(tah_graph* g);
  * source accumulating w x adj[target] over a precomputed inverted schedule
  * in a fixed order (no float atomics), then the targets zeroed. */
 tah_status tah_fill_adjoint(tah_plan* p, double* adj, const double* factors, void* stream);
+/* tah_fill_adjoint with its launches timed one by one by HIP events on
+ * `stream` (for measurement; the same kernels and results). Call with kind,
+ * bytes and ms NULL for the launch count in *n; then with arrays of *n
+ * entries: per launch, in order, its kind (0, 1, 2: the z, y, x passes of
+ * the interpolations' transpose; 3: the gather; 4: the zeroing of
+ * interpolated targets), its algorithmic bytes (8 per value read and per
+ * value written, each once) and its duration in ms. */
+tah_status tah_fill_adjoint_timed(tah_plan* p, double* adj, const double* factors, void* stream, int32_t* n,
+                                  int32_t* kind, int64_t* bytes, float* ms);
 
 /* tah_fill_adjoint without any allocation. No counterpart in the reference:
  * this is for callers that cannot allocate device memory (code under HIP
@@ -410,14 +419,42 @@ typedef struct tah_plan_options {
   int32_t reserved_[3];
 } tah_plan_options;
 tah_status tah_plan_get_options(const tah_plan* p, tah_plan_options* out);
+/* Setting the plan-wide options sets every interpolation family's below to
+ * them again. */
 tah_status tah_plan_set_options(tah_plan* p, const tah_plan_options* o);
 
+/* The options of one interpolation family: the transfers of order `order`
+ * in every dimension that restrict (restriction = 1) or prolongate (0). No
+ * counterpart in the reference, whose one kernel serves every transfer
+ * (src/ghosts.jl:372-402). */
+typedef struct tah_family_options {
+  int32_t order;
+  int32_t restriction;
+  int32_t threads;  /* workgroup size of its launches: 256 (default), 128 or 64; a tile's units and
+                       variables fill that many threads */
+  int32_t merge;    /* every unit class of the family in one launch (1) or one launch each (0) */
+  int32_t pairs;    /* prolongation over pairs of neighbouring variables where the layout allows (1) or
+                       one variable per thread (0); 0 for restriction */
+  int32_t lds;      /* from LDS copies of the source boxes where the layout allows (1), or the unit
+                       kernels (0) */
+} tah_family_options;
+/* The families the plan's interpolations fall in and their options: call
+ * with out NULL for the count in *n, then with an array of *n entries. */
+tah_status tah_plan_get_family_options(const tah_plan* p, int32_t* n, tah_family_options* out);
+/* One family's options (named by order and restriction); rebuilds the
+ * plan's tiles (it waits for the device first). */
+tah_status tah_plan_set_family_options(tah_plan* p, const tah_family_options* o);
+
 /* The autotuner: times every combination of the plan's options on `work`
- * (fills it a few dozen times: a fill is idempotent) and keeps the fastest.
- * With cache_path, the choice is appended to that text file under a
- * signature of the mesh, layout and device, and a later call with the same
- * signature takes it from there without timing (best_ms then -1).
- * best_ms (may be NULL): the chosen options' median fill time. */
+ * (fills it a few dozen times: a fill is idempotent) and keeps the fastest;
+ * then, from there, each interpolation family's options in turn (workgroup
+ * size, merged launch, variable pairs, LDS staging), each judged by the
+ * time of that family's own launches, and keeps the family choices where
+ * the whole fill is no slower with them. With cache_path, the choice is
+ * appended to that text file under a signature of the mesh, layout and
+ * device, and a later call with the same signature takes it from there
+ * without timing (best_ms then -1). best_ms (may be NULL): the chosen
+ * options' median fill time. */
 tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char* cache_path,
                     double* best_ms);
 

@@ -13,7 +13,7 @@ from treeamr_hip import _abi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 STRUCTS = ["tah_operators", "tah_stencil1d", "tah_group_desc", "tah_schedule_desc", "tah_layout",
-           "tah_plan_options"]
+           "tah_plan_options", "tah_boundary_region", "tah_family_options"]
 
 
 def _c_layout(tmp_path):

@@ -162,3 +162,28 @@ def test_rk4_wave_gradient_with_checkpoints():
             fd = float((objective(u0 + eps * d, 0) - objective(u0 - eps * d, 0)) / (2 * eps))
             an = float((g_ck * d).sum())
             assert abs(fd - an) <= 1e-5 * abs(an), (fd, an)
+
+
+def test_adjoint_timed_bytes_closed_form():
+    """tah_fill_adjoint_timed on a uniform periodic cell mesh, where every
+    transfer is a copy: the launches are gathers only, their results are
+    tah_fill_adjoint's bit for bit, and their algorithmic bytes are the
+    closed form 16 x nvars x blocks x (the own points within G of a face,
+    each a source read and written once, plus the ghost points, each read
+    and zeroed once)."""
+    from treeamr_hip import Forest, Operators, fill_ghosts_adjoint_timed
+    N, G, nv = 8, 2, 3
+    f = Forest((2, 2, 2), N=N, periodic=(True,) * 3)
+    fs = FieldSet(f, nv, G=G, device="cuda", fill=0.0)
+    s = GhostSchedule(fs, Operators(prolongation=4, restriction=4))
+    y = torch.rand_like(fs.work)
+    a, b = y.clone(), y.clone()
+    fill_ghosts_adjoint(fs, s, a)
+    recs = fill_ghosts_adjoint_timed(fs, s, b)
+    torch.cuda.synchronize()
+    assert torch.equal(a.view(torch.int64), b.view(torch.int64))
+    assert recs and all(name == "adjoint_gather_kernel" for name, _, _ in recs)
+    own_near_face = N ** 3 - (N - 2 * G) ** 3
+    ghosts = (N + 2 * G) ** 3 - N ** 3
+    assert sum(nb for _, nb, _ in recs) == 16 * nv * f.nleaves * (own_near_face + ghosts)
+    assert all(ms > 0 for _, _, ms in recs)

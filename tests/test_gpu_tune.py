@@ -47,14 +47,69 @@ def test_every_option_combination_fills_alike(which):
     plan.set_options(fork=0, merge=1, fuse=0, defer=1, copy_per_thread=8, copy_wgs=0, interp_pairs=1, lds=0)
 
 
+@pytest.mark.parametrize("which", ["cell", "cell-even", "vertex"])
+def test_every_family_option_fills_alike(which):
+    """Every interpolation family's workgroup size, merged or per-class
+    launches, variable pairs and LDS staging, one family at a time, with
+    the copies forked beside them or not: bit for bit the defaults."""
+    fs, s = _case(which)
+    init = fs.work.clone()
+    fill_ghosts(fs, s)
+    torch.cuda.synchronize()
+    ref = fs.work.clone()
+    plan = s.plan(fs.layout)
+    fams = plan.families
+    assert fams and all(f["threads"] == 256 for f in fams)
+    for fork in (0, 1):
+        plan.set_options(fork=fork)
+        for fam in fams:
+            for threads, merge, pairs, lds in itertools.product((64, 128, 256), (0, 1), (0, 1), (0, 1)):
+                if fam["restriction"] and pairs or pairs and not merge:
+                    continue
+                print(f"[family] {which} fork={fork} order={fam['order']} restriction={fam['restriction']} "
+                      f"threads={threads} merge={merge} pairs={pairs} lds={lds}", flush=True)
+                plan.set_family(fam["order"], fam["restriction"], threads=threads, merge=merge, pairs=pairs, lds=lds)
+                fs.work.copy_(init)
+                fill_ghosts(fs, s)
+                torch.cuda.synchronize()
+                assert torch.equal(fs.work.view(torch.int64), ref.view(torch.int64)), (fam, threads, merge, pairs,
+                                                                                        lds)
+            plan.set_family(fam["order"], fam["restriction"], threads=256, merge=1,
+                            pairs=0 if fam["restriction"] else 1, lds=0)
+    # the plan-wide options set every family back
+    plan.set_family(fams[0]["order"], fams[0]["restriction"], threads=64)
+    plan.set_options(fork=0, merge=1, defer=1, copy_per_thread=8, copy_wgs=0, interp_pairs=1, lds=0)
+    assert all(f["threads"] == 256 for f in plan.families)
+
+
+def test_family_options_refused():
+    from treeamr_hip import _abi
+    fs, s = _case("cell")
+    plan = s.plan(fs.layout)
+    fam = plan.families[0]
+    before = plan.families
+    for bad in (dict(threads=96), dict(threads=512), dict(merge=2), dict(lds=-1)):
+        with pytest.raises(ValueError):
+            plan.set_family(fam["order"], fam["restriction"], **bad)
+        assert plan.families == before, bad
+    with pytest.raises(ValueError):
+        plan.set_family(7, 0)
+    with pytest.raises(ValueError):
+        plan.set_family(fam["order"], 1, pairs=1)
+    assert _abi.tah_family_options().__class__.__name__ == "tah_family_options"
+
+
 def test_tune_caches_by_signature(tmp_path):
     fs, s = _case("cell")
     path = tmp_path / "tune.txt"
     opts, ms = tune(fs, s, path)
     assert ms is not None and ms > 0
     lines = path.read_text().splitlines()
-    assert len(lines) == 1 and lines[0].startswith("tah-tune-4 ")
+    assert len(lines) == 1 and lines[0].startswith("tah-tune-5 ")
+    assert opts["families"] and all(f["threads"] in (64, 128, 256) for f in opts["families"])
     s.plan(fs.layout).set_options(fork=1 - opts["fork"])
+    s.plan(fs.layout).set_family(opts["families"][0]["order"], opts["families"][0]["restriction"],
+                                 threads=64 if opts["families"][0]["threads"] != 64 else 128)
     opts2, ms2 = tune(fs, s, path)
     assert ms2 is None and opts2 == opts
     assert len(path.read_text().splitlines()) == 1

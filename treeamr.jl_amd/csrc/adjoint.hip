@@ -109,6 +109,13 @@ struct AdjStage {
   int64_t gather[2];
   int64_t zero[2];
   int64_t scratch_off[3];   // outputs of the passes, in the scratch buffer
+  // Algorithmic bytes of the stage's launches (passes z, y, x, gather,
+  // zeroing): 8 per value read and 8 per value written, once each -- a
+  // pass reads its input box and writes its output box; the gather reads
+  // and writes each source value it owns (writes only, where the cell's
+  // points are targets too) and reads each entry's values, zeroing a copy's
+  // targets as it reads them; the zeroing writes the interpolated targets.
+  int64_t bytes[5];
 };
 
 struct AdjointTables {
@@ -575,6 +582,7 @@ void build_adjoint_tables(tah_plan* p) {
         if (copy) continue;  // the gather zeroes a copy's targets
         interp.push_back({int32_t(gi), t});
         const int64_t elems = int64_t(g.points) * nvars;
+        S.bytes[4] += 8 * elems;
         for (int64_t e = 0; e < elems; e += kCopyTile) zeros.push_back(Tile{0, int32_t(gi), t, int32_t(e), 0, 0, 0, 0});
       }
     }
@@ -634,6 +642,12 @@ void build_adjoint_tables(tah_plan* p) {
           P.o[2] = e[2];
         }
         P.out_off = S.scratch_off[q] + off_of[3 * k + q];
+        {
+          const int64_t in = q == 0 ? int64_t(g.n[0]) * g.n[1] * g.n[2]
+                             : q == 1 ? int64_t(g.n[0]) * g.n[1] * e[2]
+                                      : int64_t(g.n[0]) * e[1] * e[2];
+          S.bytes[q] += 8 * nvars * (in + int64_t(P.o[0]) * P.o[1] * P.o[2]);
+        }
         const int D = 2 - q, DL = D == 0 ? 1 : 0, DU = D == 2 ? 1 : 2;
         const int64_t plane = int64_t(P.o[DL]) * P.o[DU] * nvars;
         if (plane > INT32_MAX) fail("an adjoint pass plane exceeds 2^31 values", kUnsupported);
@@ -712,8 +726,16 @@ void build_adjoint_tables(tah_plan* p) {
       }
       const size_t c0 = cells.size();
       cut_cells(es[i][0], readers, boxes, cells, entries);
-      for (size_t c = c0; c < cells.size(); ++c)
-        push_tiles(tiles, int32_t(c), int64_t(cells[c].npts) * nvars);
+      for (size_t c = c0; c < cells.size(); ++c) {
+        const AdjCell& C = cells[c];
+        push_tiles(tiles, int32_t(c), int64_t(C.npts) * nvars);
+        int64_t per = C.target ? 8 : 16;
+        for (int32_t q = C.first; q < C.first + C.count; ++q) {
+          const AdjEntry& E = entries[size_t(q)];
+          per += E.buf == 0 ? 16 * int64_t(E.c[0]) * E.c[1] * E.c[2] : 8;
+        }
+        S.bytes[3] += per * int64_t(C.npts) * nvars;
+      }
       i = j;
     }
     S.gather[1] = int64_t(tiles.size()) - S.gather[0];
@@ -787,25 +809,38 @@ void check_adjoint_call(tah_plan* p, double* adj, const double* factors) {
 
 // Every stage in reverse: passes, gather, zero. Each pass writes every
 // staged value the gather after it reads, so the buffer needs no clearing.
-void run_adjoint(tah_plan* p, double* adj, const double* factors, hipStream_t s, double* staged) {
+// `after` (may be null) is called after each launch with its kind (0-2 the
+// z, y, x passes, 3 the gather, 4 the zeroing) and algorithmic bytes.
+template <class After>
+void run_adjoint_with(tah_plan* p, double* adj, const double* factors, hipStream_t s, double* staged,
+                      const After& after) {
   plan_upload_job(p, plan_job(p, adj, factors), s);
   const AdjointTables& A = *p->adj;
   for (int32_t st = p->nstages - 1; st >= 0; --st) {
     const AdjStage& S = A.stages[st];
     launch_pass<2>(p, S, 0, s, staged);
+    if (S.pass_tiles[0][1] > 0) after(0, S.bytes[0]);
     launch_pass<1>(p, S, 1, s, staged);
+    if (S.pass_tiles[1][1] > 0) after(1, S.bytes[1]);
     launch_pass<0>(p, S, 2, s, staged);
+    if (S.pass_tiles[2][1] > 0) after(2, S.bytes[2]);
     if (S.gather[1] > 0) {
       hipLaunchKernelGGL(adjoint_gather_kernel, dim3(uint32_t(S.gather[1])), dim3(kThreads), 0, s, p->djob,
                          A.dcells, A.dtiles + S.gather[0], A.dentries, uint32_t(S.gather[1]), staged);
       HIPCHECK(hipGetLastError());
+      after(3, S.bytes[3]);
     }
     if (S.zero[1] > 0) {
       hipLaunchKernelGGL(zero_kernel, dim3(uint32_t(S.zero[1])), dim3(kThreads), 0, s, p->djob,
                          A.dzero + S.zero[0]);
       HIPCHECK(hipGetLastError());
+      after(4, S.bytes[4]);
     }
   }
+}
+
+void run_adjoint(tah_plan* p, double* adj, const double* factors, hipStream_t s, double* staged) {
+  run_adjoint_with(p, adj, factors, s, staged, [](int, int64_t) {});
 }
 
 }  // namespace
@@ -856,6 +891,52 @@ extern "C" tah_status tah_fill_adjoint(tah_plan* p, double* adj, const double* f
     if (!A.scratch && A.scratch_values > 0) HIPCHECK(hipMalloc(&A.scratch, sizeof(double) * A.scratch_values));
     run_adjoint(p, adj, factors, s, A.scratch);
     plan_end(p, s);
+  });
+}
+
+extern "C" tah_status tah_fill_adjoint_timed(tah_plan* p, double* adj, const double* factors, void* stream,
+                                             int32_t* n, int32_t* kind, int64_t* bytes, float* ms) {
+  return guard([&] {
+    if (!n) fail("null argument");
+    check_adjoint_call(p, adj, factors);
+    HIPCHECK(hipSetDevice(p->ctx->device));
+    if (!p->adj) build_adjoint(p);
+    AdjointTables& A = *p->adj;
+    int32_t total = 0;
+    for (const AdjStage& S : A.stages)
+      total += (S.pass_tiles[0][1] > 0) + (S.pass_tiles[1][1] > 0) + (S.pass_tiles[2][1] > 0) + (S.gather[1] > 0) +
+               (S.zero[1] > 0);
+    if (!kind && !bytes && !ms) {
+      *n = total;
+      return;
+    }
+    if (!kind || !bytes || !ms) fail("null argument");
+    if (*n < total)
+      fail("the record arrays hold " + std::to_string(*n) + " launches; the adjoint makes " + std::to_string(total));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    plan_begin(p, s);
+    if (!A.scratch && A.scratch_values > 0) HIPCHECK(hipMalloc(&A.scratch, sizeof(double) * A.scratch_values));
+    std::vector<hipEvent_t> ev(size_t(total) + 1);
+    for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
+    try {
+      HIPCHECK(hipEventRecord(ev[0], s));
+      int32_t k = 0;
+      run_adjoint_with(p, adj, factors, s, A.scratch, [&](int kd, int64_t b) {
+        if (k >= total) fail("adjoint launch count changed", kHip);
+        HIPCHECK(hipEventRecord(ev[size_t(k) + 1], s));
+        kind[k] = kd;
+        bytes[k] = b;
+        ++k;
+      });
+      plan_end(p, s);
+      HIPCHECK(hipEventSynchronize(ev[size_t(total)]));
+      for (int32_t i = 0; i < total; ++i) HIPCHECK(hipEventElapsedTime(&ms[i], ev[size_t(i)], ev[size_t(i) + 1]));
+    } catch (...) {
+      for (auto& e : ev) (void)hipEventDestroy(e);
+      throw;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    *n = total;
   });
 }
 

@@ -124,7 +124,7 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
     return (g.kind == 2 ? g.sblocks : g.tblocks)[size_t(t.transfer)];
   };
   for (size_t pos = 0; pos < npos; ++pos) {
-    std::map<int, std::vector<Tile>> bycls;
+    std::map<std::pair<int, int32_t>, std::vector<Tile>> bycls;  // by (class, workgroup size)
     std::map<int, std::vector<LdsTile>> ldscls;  // LDS-staged classes
     std::map<int, int32_t> ldsbytes;
     for (int32_t i = 0; i < n; ++i) {
@@ -144,7 +144,7 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
         for (int64_t t = L.first; t < L.first + L.count; ++t) {
           Tile tl = p->tiles[t];
           tl.job = i;
-          bycls[L.cls].push_back(tl);
+          bycls[{L.cls, L.threads}].push_back(tl);
         }
       }
     }
@@ -154,7 +154,7 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
       // where the tiles that read or write its blocks are
       std::vector<std::pair<uint64_t, Tile>> q;
       for (auto it = bycls.begin(); it != bycls.end();) {
-        const int kind = queue_kind(it->first);
+        const int kind = queue_kind(it->first.first);
         if (kind < 0) {
           ++it;
           continue;
@@ -176,7 +176,7 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
     }
     std::vector<Launch> ls;
     for (auto& kv : bycls) {
-      ls.push_back(Launch{kv.first, int64_t(tiles.size()), int64_t(kv.second.size())});
+      ls.push_back(Launch{kv.first.first, int64_t(tiles.size()), int64_t(kv.second.size()), 0, 0, 0, kv.first.second});
       tiles.insert(tiles.end(), kv.second.begin(), kv.second.end());
     }
     for (auto& kv : ldscls) {

@@ -35,8 +35,7 @@ int interp_index(int P, int shift) {
 namespace {
 
 template <int P, int S>
-void launch_interp_r(int rbits, dim3 grid, hipStream_t s, const Job* j, const Tile* t) {
-  dim3 blk(kThreads);
+void launch_interp_r(int rbits, dim3 grid, dim3 blk, hipStream_t s, const Job* j, const Tile* t) {
   switch (rbits) {
     case 0: hipLaunchKernelGGL((interp_kernel<P, S, 1, 1, 1>), grid, blk, 0, s, j, t); break;
     case 1: hipLaunchKernelGGL((interp_kernel<P, S, 2, 1, 1>), grid, blk, 0, s, j, t); break;
@@ -51,8 +50,17 @@ void launch_interp_r(int rbits, dim3 grid, hipStream_t s, const Job* j, const Ti
 
 }  // namespace
 
-void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles, int32_t bounded) {
+void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles, int32_t bounded,
+                  int32_t threads) {
   dim3 grid(ntiles), blk(kThreads);
+  // an interpolation family's workgroup size: its tiles fill that many
+  // threads (the kernels are bounded for 256 and let threads past a tile's
+  // work go)
+  const bool interp = cls >= kClsInterp && cls != copy_launch(4) && cls != copy_launch(8) && cls != copy_launch(16);
+  if (threads != kThreads) {
+    if (!interp || (threads != 64 && threads != 128)) fail("bad workgroup size for class " + std::to_string(cls));
+    blk = dim3(uint32_t(threads));
+  }
   if (bounded > 0) {
     // a copy class from a bounded grid (a multiple of 8, at most ntiles)
     if (bounded % 8 != 0 || uint32_t(bounded) > ntiles) fail("bad bounded grid for a copy launch");
@@ -95,14 +103,14 @@ void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, cons
   } else {
     int id = cls - kClsInterp, ps = id / 8, rbits = id % 8;
     switch (ps) {
-      case 0: launch_interp_r<2, 0>(rbits, grid, s, jobs, tiles); break;
-      case 1: launch_interp_r<3, 0>(rbits, grid, s, jobs, tiles); break;
-      case 2: launch_interp_r<4, 0>(rbits, grid, s, jobs, tiles); break;
-      case 3: launch_interp_r<5, 0>(rbits, grid, s, jobs, tiles); break;
-      case 4: launch_interp_r<6, 0>(rbits, grid, s, jobs, tiles); break;
-      case 5: launch_interp_r<2, 2>(rbits, grid, s, jobs, tiles); break;
-      case 6: launch_interp_r<4, 2>(rbits, grid, s, jobs, tiles); break;
-      default: launch_interp_r<6, 2>(rbits, grid, s, jobs, tiles); break;
+      case 0: launch_interp_r<2, 0>(rbits, grid, blk, s, jobs, tiles); break;
+      case 1: launch_interp_r<3, 0>(rbits, grid, blk, s, jobs, tiles); break;
+      case 2: launch_interp_r<4, 0>(rbits, grid, blk, s, jobs, tiles); break;
+      case 3: launch_interp_r<5, 0>(rbits, grid, blk, s, jobs, tiles); break;
+      case 4: launch_interp_r<6, 0>(rbits, grid, blk, s, jobs, tiles); break;
+      case 5: launch_interp_r<2, 2>(rbits, grid, blk, s, jobs, tiles); break;
+      case 6: launch_interp_r<4, 2>(rbits, grid, blk, s, jobs, tiles); break;
+      default: launch_interp_r<6, 2>(rbits, grid, blk, s, jobs, tiles); break;
     }
   }
   HIPCHECK(hipGetLastError());

@@ -433,6 +433,7 @@ void build_plan_device(tah_plan* p) {
   std::memset(&p->last_job, 0, sizeof(Job));
   HIPCHECK(hipEventCreateWithFlags(&p->done, hipEventDisableTiming));
   build_hooks(p);
+  families_from_options(p);
   if (p->regrid) {
     HIPCHECK(hipMalloc(&p->djob2, sizeof(Job)));
     std::memset(&p->last_job2, 0, sizeof(Job));
@@ -533,13 +534,20 @@ void build_launches(tah_plan* p) {
   // allows: variables innermost, an even count, every other stride even.
   // (Restriction over pairs measured slower than over single variables.)
   const int64_t* st = p->layout.stride;
-  const bool pair_layout = p->interp_pairs && st[3] == 1 && nvars % 2 == 0 && st[0] % 2 == 0 &&
-                           st[1] % 2 == 0 && st[2] % 2 == 0 && st[4] % 2 == 0;
+  const bool pair_layout = st[3] == 1 && nvars % 2 == 0 && st[0] % 2 == 0 && st[1] % 2 == 0 && st[2] % 2 == 0 &&
+                           st[4] % 2 == 0;
   auto launch_of = [&](int cls) {
-    if (!p->merge || cls < kClsInterp || cls >= kClsMerged) return cls;
+    if (cls < kClsInterp || cls >= kClsMerged) return cls;
     const int index = (cls - kClsInterp) / 8;
-    const bool pair = pair_layout && kInterpTable[size_t(index)][1] == 0;
+    if (!p->fam_merge[size_t(index)]) return cls;
+    const bool pair = pair_layout && p->fam_pairs[size_t(index)] && kInterpTable[size_t(index)][1] == 0;
     return (pair ? kClsMergedPair : kClsMerged) + index;
+  };
+  // The family of a group (-1: an order no specialised kernel takes).
+  auto family_of = [&](const PlanGroup& g) { return interp_index(g.order[0], g.kind == 2 ? 2 : 0); };
+  auto fam_lds = [&](const PlanGroup& g) {
+    const int f = family_of(g);
+    return f >= 0 && p->fam_lds[size_t(f)] != 0;
   };
   // The launch a copy or generic group runs in.
   auto group_launch = [&](int cls) { return cls == kClsCopy ? copy_launch(p->copy_per_thread) : cls; };
@@ -550,10 +558,10 @@ void build_launches(tah_plan* p) {
                             st[2] % 2 == 0 && st[4] % 2 == 0;
   p->lds_chunks.clear();
   std::vector<std::array<int32_t, 2>> lds_of(p->groups.size(), {0, 0});  // first chunk, count
-  if ((p->lds & 1) && pair_capable)
+  if (pair_capable)
     for (size_t gi = 0; gi < p->groups.size(); ++gi) {
       const PlanGroup& g = p->groups[gi];
-      if (g.cls == kClsInterp && g.kind == 2) {
+      if (g.cls == kClsInterp && g.kind == 2 && fam_lds(g)) {
         const auto cs = lds_chunks_of(g, int32_t(gi));
         lds_of[gi] = {int32_t(p->lds_chunks.size()), int32_t(cs.size())};
         p->lds_chunks.insert(p->lds_chunks.end(), cs.begin(), cs.end());
@@ -568,10 +576,10 @@ void build_launches(tah_plan* p) {
   p->lds_units.clear();
   std::vector<char> lds_cls(p->wclasses.size(), 0);
   std::vector<std::array<int32_t, 2>> ulds_of(p->groups.size(), {0, 0});  // first chunk, count
-  if ((p->lds & 2) && pair_capable)
+  if (pair_capable)
     for (size_t gi = 0; gi < p->groups.size(); ++gi) {
       const PlanGroup& g = p->groups[gi];
-      if (g.cls != kClsInterp || g.kind != 3) continue;
+      if (g.cls != kClsInterp || g.kind != 3 || !fam_lds(g)) continue;
       std::vector<int32_t> cls;
       for (int32_t ci : g.wclasses)
         if (p->wclasses[size_t(ci)].nunits() >= 64) cls.push_back(ci);
@@ -688,16 +696,17 @@ void build_launches(tah_plan* p) {
             const int64_t per_unit = (1 + (rb & 1)) * (1 + ((rb >> 1) & 1)) * (1 + ((rb >> 2) & 1));
             const int32_t nun = c.nunits();
             int32_t nu, nvt;
+            const int32_t T = p->fam_threads[size_t((c.cls - kClsInterp) / 8)];
             if (vfast) {
               // a pair tile's thread holds two variables
-              const int32_t per_wg = cls >= kClsMergedPair ? 2 * kThreads : kThreads;
+              const int32_t per_wg = cls >= kClsMergedPair ? 2 * T : T;
               nvt = std::min(nvars, per_wg);
               nu = std::max(1, per_wg / nvt);
             } else {
               // units fastest: as many as the class has per transfer, the
               // rest of the workgroup over variables
-              nu = std::min(nun, kThreads);
-              nvt = std::max(1, std::min(nvars, kThreads / nu));
+              nu = std::min(nun, T);
+              nvt = std::max(1, std::min(nvars, T / nu));
             }
             const int32_t nt = int32_t(g.tblocks.size());
             for (int32_t t = 0; t < nt;) {
@@ -764,6 +773,14 @@ void build_launches(tah_plan* p) {
         continue;
       }
       Launch L{B.cls, int64_t(p->tiles.size()), int64_t(B.tiles.size()), B.values, 0};
+      {
+        // an interpolation family's workgroup size
+        int fam = -1;
+        if (B.cls >= kClsMergedPair && B.cls < kClsMergedPair + kFamilies) fam = B.cls - kClsMergedPair;
+        else if (B.cls >= kClsMerged && B.cls < kClsMerged + kFamilies) fam = B.cls - kClsMerged;
+        else if (B.cls >= kClsInterp && B.cls < kClsInterp + 8 * kFamilies) fam = (B.cls - kClsInterp) / 8;
+        if (fam >= 0) L.threads = p->fam_threads[size_t(fam)];
+      }
       if (p->copy_wgs > 0 && B.cls == copy_launch(p->copy_per_thread) && int64_t(B.tiles.size()) >= 8)
         L.grid = int32_t(std::min<int64_t>(p->copy_wgs, int64_t(B.tiles.size())) / 8 * 8);
       for (const auto& e : B.tiles) p->tiles.push_back(e.second);
@@ -868,7 +885,17 @@ const std::vector<hipStream_t>& ctx_streams(tah_ctx* c) {
 
 void launch_one(const Launch& L, hipStream_t s, const Job* jobs, const Tile* tiles, const LdsTile* ltiles) {
   if (L.lds > 0) launch_lds(L.cls, uint32_t(L.count), s, jobs, ltiles + L.first, L.lds);
-  else launch_class(L.cls, uint32_t(L.count), s, jobs, tiles + L.first, L.grid);
+  else launch_class(L.cls, uint32_t(L.count), s, jobs, tiles + L.first, L.grid, L.threads);
+}
+
+void families_from_options(tah_plan* p) {
+  for (int f = 0; f < kFamilies; ++f) {
+    const bool restriction = kInterpTable[size_t(f)][1] != 0;
+    p->fam_threads[size_t(f)] = kThreads;
+    p->fam_merge[size_t(f)] = p->merge;
+    p->fam_pairs[size_t(f)] = restriction ? 0 : p->interp_pairs;
+    p->fam_lds[size_t(f)] = restriction ? (p->lds & 1) : (p->lds >> 1) & 1;
+  }
 }
 
 void run_launches(tah_ctx* c, const std::vector<Launch>& ls, hipStream_t s, const Job* jobs,

@@ -75,7 +75,12 @@ struct Launch {
   int64_t values = 0;  // values the launch writes
   int32_t grid = 0;    // copies: workgroups of a bounded grid (0: one per tile)
   int32_t lds = 0;     // LDS-staged: bytes of LDS per workgroup; first/count index LdsTiles
+  int32_t threads = kThreads;  // workgroup size (interpolation families: 64, 128 or 256)
 };
+
+// The interpolation families: the index into kInterpTable of order P and
+// shift (restriction 2, prolongation 0).
+constexpr int kFamilies = 8;
 
 struct AdjointTables;
 
@@ -155,6 +160,12 @@ struct tah_plan {
   // queue took 26.4 ms against 21.5 ms, its one register budget holding the
   // copies to four waves per SIMD).
   int32_t queue = 0;
+  // Per interpolation family (kInterpTable index), what the plan-wide
+  // options above set for every family until tah_plan_set_class_options
+  // (or the tuner) sets one apart: its workgroup size (the units and
+  // variables of a tile fill that many threads), one launch for all its
+  // unit classes, variable pairs (prolongation), LDS staging.
+  std::array<int32_t, tah::kFamilies> fam_threads{}, fam_merge{}, fam_pairs{}, fam_lds{};
   // The outer-face hook (tah_fill_boundary): the regions in HBM and their
   // tiles -- every region's first (the values form, after phase 1's stage
   // group), then per stage group those of the regions whose slot stage it
@@ -221,7 +232,10 @@ void launch_lds(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const 
 // Launch one Launch (copy, interpolation or LDS-staged) on stream s.
 void launch_one(const Launch& L, hipStream_t s, const Job* jobs, const Tile* tiles, const LdsTile* ltiles);
 // Launch one class over ntiles tiles (fill.hip); grid > 0: a bounded grid.
-void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles, int32_t grid = 0);
+void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const Tile* tiles, int32_t grid = 0,
+                  int32_t threads = kThreads);
+// Every family's options from the plan-wide ones.
+void families_from_options(tah_plan* p);
 // Order a call on plan p and stream s after the plan's previous call (on
 // whatever stream it ran), before the call rewrites the plan's device Job
 // or staging buffer; plan_end marks the call's last launch. A stream under

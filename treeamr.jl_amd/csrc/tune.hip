@@ -1,16 +1,19 @@
 // Plan options and the autotuner.
 //
-// A plan's launches have six knobs: stage groups forked over the
-// context's streams or run in order on one; the unit classes of one
-// interpolation order in one launch or a launch each; phase-1 transfers
-// deferred into the last stage group or not; the copy kernel's tile (4, 8 or
-// 16 values per thread); the copies from one workgroup per tile or from a
-// bounded grid that leaves wave slots to the interpolations beside them;
-// merged interpolations over variable pairs or single variables. tah_tune times every
-// combination on the caller's working array and keeps the fastest, cached
-// in a text file under a signature of the mesh, the layout and the device,
-// so that a second run over the same mesh skips the timing. The reference
-// has no autotuner (its launches are fixed, src/ghosts.jl:404-521).
+// A plan's launches have plan-wide knobs -- stage groups forked over the
+// context's streams or run in order on one; phase-1 transfers deferred into
+// the last stage group or not; the copy kernel's tile (4, 8 or 16 values per
+// thread); the copies from one workgroup per tile or from a bounded grid
+// that leaves wave slots to the interpolations beside them -- and knobs per
+// interpolation family (order and direction): the workgroup size, the unit
+// classes in one launch or a launch each, variable pairs, LDS staging.
+// tah_tune times every plan-wide combination on the caller's working array
+// and keeps the fastest, then searches each family's knobs by the time of
+// that family's own launches, and keeps those where the whole fill is no
+// slower; cached in a text file under a signature of the mesh, the layout
+// and the device, so that a second run over the same mesh skips the timing.
+// The reference has no autotuner (its launches are fixed,
+// src/ghosts.jl:404-521).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -85,8 +88,8 @@ void set_options(tah_plan* p, const tah_plan_options& o) {
   if (o.lds < 0 || o.lds > 3) fail("plan option lds is 0 to 3 (bit 0 restriction, bit 1 prolongation)");
   if (o.queue < 0 || o.queue > 1) fail("plan option queue is 0 or 1");
   if (o.reserved_[0] || o.reserved_[1] || o.reserved_[2]) fail("tah_plan_options.reserved_ must be zero");
-  const bool rebuild = o.merge != p->merge || o.defer != p->defer || o.copy_per_thread != p->copy_per_thread ||
-                       wgs != p->copy_wgs || o.interp_pairs != p->interp_pairs || o.lds != p->lds;
+  bool rebuild = o.merge != p->merge || o.defer != p->defer || o.copy_per_thread != p->copy_per_thread ||
+                 wgs != p->copy_wgs || o.interp_pairs != p->interp_pairs || o.lds != p->lds;
   p->fork = o.fork;
   p->merge = o.merge;
   p->defer = o.defer;
@@ -99,10 +102,70 @@ void set_options(tah_plan* p, const tah_plan_options& o) {
     p->queue = o.queue;
     plan_register(p);
   }
+  // every family to the plan-wide options
+  const auto th = p->fam_threads, me = p->fam_merge, pa = p->fam_pairs, ld = p->fam_lds;
+  families_from_options(p);
+  rebuild = rebuild || th != p->fam_threads || me != p->fam_merge || pa != p->fam_pairs || ld != p->fam_lds;
   if (rebuild) build_launches(p);
 }
 
-bool lookup(const std::string& path, const std::string& key, tah_plan_options& o) {
+// The family of a launch class (-1: copies, generic transfers).
+int family_of_class(int cls) {
+  if (cls >= kClsLdsProlong && cls < kClsLdsProlong + kFamilies) return cls - kClsLdsProlong;
+  if (cls >= kClsLds && cls < kClsLds + kFamilies) return cls - kClsLds;
+  if (cls >= kClsMergedPair && cls < kClsMergedPair + kFamilies) return cls - kClsMergedPair;
+  if (cls >= kClsMerged && cls < kClsMerged + kFamilies) return cls - kClsMerged;
+  if (cls >= kClsInterp && cls < kClsInterp + 8 * kFamilies) return (cls - kClsInterp) / 8;
+  return -1;
+}
+
+// The families the plan's interpolation groups fall in.
+std::vector<int> plan_families(const tah_plan* p) {
+  std::vector<int> out;
+  for (const PlanWClass& c : p->wclasses) {
+    const int f = family_of_class(c.cls);
+    if (f >= 0 && std::find(out.begin(), out.end(), f) == out.end()) out.push_back(f);
+  }
+  std::sort(out.begin(), out.end());
+  return out;
+}
+
+tah_family_options family_options(const tah_plan* p, int f) {
+  tah_family_options o{};
+  o.order = kInterpTable[size_t(f)][0];
+  o.restriction = kInterpTable[size_t(f)][1] != 0;
+  o.threads = p->fam_threads[size_t(f)];
+  o.merge = p->fam_merge[size_t(f)];
+  o.pairs = p->fam_pairs[size_t(f)];
+  o.lds = p->fam_lds[size_t(f)];
+  return o;
+}
+
+void set_family(tah_plan* p, const tah_family_options& o) {
+  if (o.restriction < 0 || o.restriction > 1) fail("family option restriction is 0 or 1");
+  const int f = interp_index(o.order, o.restriction ? 2 : 0);
+  if (f < 0)
+    fail("no interpolation family of order " + std::to_string(o.order) +
+         (o.restriction ? " restricts" : " prolongates") + " here");
+  if (o.threads != 64 && o.threads != 128 && o.threads != 256) fail("a family's workgroup size is 64, 128 or 256");
+  if (o.merge < 0 || o.merge > 1 || o.pairs < 0 || o.pairs > 1 || o.lds < 0 || o.lds > 1)
+    fail("family options merge, pairs and lds are 0 or 1");
+  if (o.restriction && o.pairs) fail("variable pairs are for prolongation (pairs must be 0 for restriction)");
+  const size_t i = size_t(f);
+  if (p->fam_threads[i] == o.threads && p->fam_merge[i] == o.merge && p->fam_pairs[i] == o.pairs &&
+      p->fam_lds[i] == o.lds)
+    return;
+  p->fam_threads[i] = o.threads;
+  p->fam_merge[i] = o.merge;
+  p->fam_pairs[i] = o.pairs;
+  p->fam_lds[i] = o.lds;
+  build_launches(p);
+}
+
+// A cache line: "tah-tune-5 <signature> fork merge defer cpt wgs pairs lds
+// <families> (order restriction threads merge pairs lds)... <ms>".
+bool lookup(const std::string& path, const std::string& key, tah_plan_options& o,
+            std::vector<tah_family_options>& fams) {
   std::ifstream in(path);
   if (!in) return false;
   std::string line;
@@ -110,30 +173,55 @@ bool lookup(const std::string& path, const std::string& key, tah_plan_options& o
   while (std::getline(in, line)) {
     std::istringstream ss(line);
     std::string tag, k;
-    int f, m, d, c, w, ip, l;
-    if (!(ss >> tag >> k >> f >> m >> d >> c >> w >> ip >> l) || tag != "tah-tune-4" || k != key) continue;
-    o = tah_plan_options{f, m, d, c, 0, w, ip, l};
+    int f, m, d, c, w, ip, l, nf;
+    if (!(ss >> tag >> k >> f >> m >> d >> c >> w >> ip >> l >> nf) || tag != "tah-tune-5" || k != key || nf < 0 ||
+        nf > kFamilies)
+      continue;
+    std::vector<tah_family_options> fs(static_cast<size_t>(nf));
+    bool ok = true;
+    for (auto& e : fs) ok = ok && bool(ss >> e.order >> e.restriction >> e.threads >> e.merge >> e.pairs >> e.lds);
+    if (!ok) continue;
+    o = tah_plan_options{f, m, d, c, 0, w, ip, l, 0, {0, 0, 0}};
+    fams = std::move(fs);
     found = true;  // the last entry wins
   }
   return found;
 }
 
-// The candidates: every combination, the bounded copy grid only beside
-// forked interpolations (it has nothing to leave its wave slots to
-// otherwise), variable pairs only for merged launches (the only ones that
-// take them).
-std::vector<tah_plan_options> candidates() {
+// The plan-wide candidates: forked or not, deferral, the copy tile and
+// the copies' bounded grid (only beside forked interpolations: it has
+// nothing to leave its wave slots to otherwise). The interpolations' own
+// options are the families' and searched after these.
+std::vector<tah_plan_options> candidates(const tah_plan* p) {
   std::vector<tah_plan_options> out;
   for (int f = 0; f <= 1; ++f)
-    for (int m = 0; m <= 1; ++m)
-      for (int ip = m; ip >= 0; --ip)
-        for (int d = 0; d <= 1; ++d)
-          for (int c : {4, 8, 16})
-            for (int w : {0, 512, 1024, 2048})
-              for (int l = 3; l >= 0; --l) {
-                if (w && !f) continue;
-                out.push_back(tah_plan_options{f, m, d, c, 0, w, m ? ip : 1, l});
-              }
+    for (int d = 0; d <= 1; ++d)
+      for (int c : {4, 8, 16})
+        for (int w : {0, 512, 1024, 2048}) {
+          if (w && !f) continue;
+          out.push_back(tah_plan_options{f, p->merge, d, c, 0, w, p->interp_pairs, p->lds, p->queue, {0, 0, 0}});
+        }
+  return out;
+}
+
+// One family's candidates: workgroup sizes, one launch or one per unit
+// class, variable pairs (prolongation, where the layout allows), LDS
+// staging (where the layout allows and a staged kernel exists).
+std::vector<tah_family_options> family_candidates(const tah_plan* p, int f) {
+  const int64_t* st = p->layout.stride;
+  const bool pairs_ok = st[3] == 1 && p->layout.nvars % 2 == 0 && st[0] % 2 == 0 && st[1] % 2 == 0 &&
+                        st[2] % 2 == 0 && st[4] % 2 == 0;
+  const int P = kInterpTable[size_t(f)][0];
+  const bool restriction = kInterpTable[size_t(f)][1] != 0;
+  const bool lds_ok = pairs_ok && (restriction ? (P == 2 || P == 4 || P == 6) : (P >= 2 && P <= 6));
+  std::vector<tah_family_options> out;
+  for (int t : {256, 128, 64})
+    for (int m = 1; m >= 0; --m)
+      for (int pr = (!restriction && pairs_ok) ? 1 : 0; pr >= 0; --pr)
+        for (int l = 0; l <= (lds_ok ? 1 : 0); ++l) {
+          if (pr && !m) continue;  // pairs are a merged launch's
+          out.push_back(tah_family_options{P, restriction ? 1 : 0, t, m, pr, l});
+        }
   return out;
 }
 
@@ -165,6 +253,29 @@ tah_status tah_plan_set_options(tah_plan* p, const tah_plan_options* o) {
   });
 }
 
+tah_status tah_plan_get_family_options(const tah_plan* p, int32_t* n, tah_family_options* out) {
+  return guard([&] {
+    if (!p || !n) fail("null argument");
+    const std::vector<int> fams = plan_families(p);
+    if (!out) {
+      *n = int32_t(fams.size());
+      return;
+    }
+    if (*n < int32_t(fams.size()))
+      fail("the array holds " + std::to_string(*n) + " families; the plan has " + std::to_string(fams.size()));
+    for (size_t k = 0; k < fams.size(); ++k) out[k] = family_options(p, fams[k]);
+    *n = int32_t(fams.size());
+  });
+}
+
+tah_status tah_plan_set_family_options(tah_plan* p, const tah_family_options* o) {
+  return guard([&] {
+    if (!p || !o) fail("null argument");
+    HIPCHECK(hipSetDevice(p->ctx->device));
+    set_family(p, *o);
+  });
+}
+
 tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char* cache_path,
                     double* best_ms) {
   return guard([&] {
@@ -175,9 +286,11 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
     HIPCHECK(hipSetDevice(p->ctx->device));
     const std::string key = signature(p);
     tah_plan_options o{};
-    if (cache_path && lookup(cache_path, key, o)) {
+    std::vector<tah_family_options> cached;
+    if (cache_path && lookup(cache_path, key, o, cached)) {
       o.queue = p->queue;  // a batch setting: the tuner times single fills
       set_options(p, o);
+      for (const auto& f : cached) set_family(p, f);
       if (best_ms) *best_ms = -1.0;
       return;
     }
@@ -186,34 +299,100 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
-    tah_plan_options best{p->fork, p->merge, p->defer, p->copy_per_thread, 0, p->copy_wgs, p->interp_pairs, p->lds,
-                          p->queue, {0, 0, 0}};
-    float best_t = 1e30f;
     auto run = [&]() {
       tah_status st = tah_fill(p, work, factors, s);
       if (st != TAH_OK) fail(std::string("tah_tune: a trial fill failed: ") + tah_last_error(), st);
     };
+    // the median of five fills after two unmeasured ones
+    auto time_fill = [&]() {
+      for (int w = 0; w < 2; ++w) run();
+      std::vector<float> ts;
+      for (int r = 0; r < 5; ++r) {
+        HIPCHECK(hipEventRecord(e0, s));
+        run();
+        HIPCHECK(hipEventRecord(e1, s));
+        HIPCHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+        ts.push_back(ms);
+      }
+      std::sort(ts.begin(), ts.end());
+      return ts[2];
+    };
+    // the median over five fills of family f's own launches, timed one by one
+    auto time_family = [&](int f) {
+      int32_t n = 0;
+      tah_status st = tah_fill_timed(p, work, factors, s, &n, nullptr, nullptr, nullptr);
+      if (st != TAH_OK) fail(std::string("tah_tune: ") + tah_last_error(), st);
+      std::vector<int32_t> cls(size_t(std::max(1, n)));
+      std::vector<int64_t> vals(cls.size());
+      std::vector<float> ms(cls.size());
+      std::vector<float> ts;
+      for (int r = 0; r < 6; ++r) {
+        int32_t m = n;
+        st = tah_fill_timed(p, work, factors, s, &m, cls.data(), vals.data(), ms.data());
+        if (st != TAH_OK) fail(std::string("tah_tune: a trial fill failed: ") + tah_last_error(), st);
+        if (r == 0) continue;  // unmeasured
+        float t = 0.f;
+        for (int32_t k = 0; k < m; ++k)
+          if (family_of_class(cls[size_t(k)]) == f) t += ms[size_t(k)];
+        ts.push_back(t);
+      }
+      std::sort(ts.begin(), ts.end());
+      return ts[2];
+    };
+    tah_plan_options best{p->fork, p->merge, p->defer, p->copy_per_thread, 0, p->copy_wgs, p->interp_pairs, p->lds,
+                          p->queue, {0, 0, 0}};
+    std::vector<tah_family_options> best_fams;
+    float best_t = 1e30f;
     try {
-      for (tah_plan_options cand : candidates()) {
-        cand.queue = p->queue;
+      // 1. the plan-wide options, every family at the plan's own
+      for (const tah_plan_options& cand : candidates(p)) {
         set_options(p, cand);
-        for (int w = 0; w < 2; ++w) run();
-        std::vector<float> ts;
-        for (int r = 0; r < 5; ++r) {
-          HIPCHECK(hipEventRecord(e0, s));
-          run();
-          HIPCHECK(hipEventRecord(e1, s));
-          HIPCHECK(hipEventSynchronize(e1));
-          float ms = 0.f;
-          HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
-          ts.push_back(ms);
-        }
-        std::sort(ts.begin(), ts.end());
-        if (ts[2] < best_t) {
-          best_t = ts[2];
+        const float t = time_fill();
+        if (t < best_t) {
+          best_t = t;
           best = cand;
         }
       }
+      set_options(p, best);
+      // 2. from there, each family by the time of its own launches
+      for (int f : plan_families(p)) {
+        tah_family_options keep = family_options(p, f);
+        float keep_t = time_family(f);
+        for (const tah_family_options& c : family_candidates(p, f)) {
+          set_family(p, c);
+          const float t = time_family(f);
+          if (t < keep_t) {
+            keep_t = t;
+            keep = c;
+          }
+        }
+        set_family(p, keep);
+      }
+      // 3. each family's choice where the whole fill is no slower with it
+      // than with the family at the plan-wide options: the two timed in
+      // turn, three times each, the better of each's medians compared
+      // (the fill's time is what counts when launches overlap)
+      for (int f : plan_families(p)) {
+        const tah_family_options chosen = family_options(p, f);
+        tah_family_options base = chosen;
+        base.threads = kThreads;
+        base.merge = p->merge;
+        base.pairs = base.restriction ? 0 : p->interp_pairs;
+        base.lds = base.restriction ? (p->lds & 1) : (p->lds >> 1) & 1;
+        if (std::memcmp(&chosen, &base, sizeof base) == 0) continue;
+        float tc = 1e30f, tb = 1e30f;
+        for (int r = 0; r < 3; ++r) {
+          set_family(p, chosen);
+          tc = std::min(tc, time_fill());
+          set_family(p, base);
+          tb = std::min(tb, time_fill());
+        }
+        set_family(p, tc < tb ? chosen : base);
+      }
+      best_t = time_fill();
+      for (int f : plan_families(p)) best_fams.push_back(family_options(p, f));
     } catch (...) {
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
@@ -225,13 +404,18 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
     HIPCHECK(hipStreamSynchronize(s));
     (void)hipStreamDestroy(s);
     set_options(p, best);
+    for (const auto& f : best_fams) set_family(p, f);
     if (best_ms) *best_ms = best_t;
     if (cache_path) {
       std::ofstream out(cache_path, std::ios::app);
       if (!out) fail(std::string("tah_tune: cannot write the cache file ") + cache_path);
-      out << "tah-tune-4 " << key << " " << best.fork << " " << best.merge << " " << best.defer << " "
+      out << "tah-tune-5 " << key << " " << best.fork << " " << best.merge << " " << best.defer << " "
           << best.copy_per_thread << " " << best.copy_wgs << " " << best.interp_pairs << " " << best.lds << " "
-          << best_t << "\n";
+          << best_fams.size();
+      for (const auto& f : best_fams)
+        out << " " << f.order << " " << f.restriction << " " << f.threads << " " << f.merge << " " << f.pairs << " "
+            << f.lds;
+      out << " " << best_t << "\n";
     }
   });
 }

@@ -20,6 +20,6 @@ from .core import (  # noqa: F401
     ownextents, statelength, statevector, staterange, stateview, statearray, scatter, gather,
     RegridSchedule, regrid, regrid_transfer, UnifySchedule, unify, unify_desc,
     AllVariables, WithPatch, Relative, CellBoundary, boundary_by_coordinates, boundary_values,
-    fill_by_coordinates,
+    fill_by_coordinates, fill_ghosts_adjoint_timed,
 )
 from ._abi import TahError, StaleScheduleError, LayoutError, UnsupportedError  # noqa: F401

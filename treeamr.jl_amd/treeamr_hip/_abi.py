@@ -55,6 +55,11 @@ P = C.c_void_p
 I32P = C.POINTER(C.c_int32)
 DP = C.POINTER(C.c_double)
 
+class tah_family_options(C.Structure):
+    _fields_ = [("order", C.c_int32), ("restriction", C.c_int32), ("threads", C.c_int32), ("merge", C.c_int32),
+                ("pairs", C.c_int32), ("lds", C.c_int32)]
+
+
 class tah_plan_options(C.Structure):
     _fields_ = [("fork", C.c_int32), ("merge", C.c_int32), ("defer", C.c_int32),
                 ("copy_per_thread", C.c_int32), ("fuse", C.c_int32), ("copy_wgs", C.c_int32),
@@ -101,8 +106,11 @@ SIGNATURES = {
     "tah_kernel_class_name": [C.c_int32, C.c_char_p, C.c_int32],
     "tah_fill_adjoint": [P, P, P, P],
     "tah_fill_adjoint_noalloc": [P, P, P, P],
+    "tah_fill_adjoint_timed": [P, P, P, P, I32P, I32P, C.POINTER(C.c_int64), C.POINTER(C.c_float)],
     "tah_plan_get_options": [P, C.POINTER(tah_plan_options)],
     "tah_plan_set_options": [P, C.POINTER(tah_plan_options)],
+    "tah_plan_get_family_options": [P, I32P, C.POINTER(tah_family_options)],
+    "tah_plan_set_family_options": [P, C.POINTER(tah_family_options)],
     "tah_tune": [P, P, P, C.c_char_p, C.POINTER(C.c_double)],
     "tah_state_length": [C.POINTER(tah_layout), I32P, C.POINTER(C.c_int64)],
     "tah_forest_coarsen": [P, C.c_int32, I32P, I32P],

@@ -358,6 +358,27 @@ class Plan:
                                   pick("lds", lds), pick("queue", queue))
         check(lib().tah_plan_set_options(self._h, C.byref(o)))
 
+    @property
+    def families(self):
+        """The interpolation families of the plan and their options: a list of
+        dict(order=, restriction=, threads=, merge=, pairs=, lds=)."""
+        n = C.c_int32()
+        check(lib().tah_plan_get_family_options(self._h, C.byref(n), None))
+        arr = (_abi.tah_family_options * max(1, n.value))()
+        check(lib().tah_plan_get_family_options(self._h, C.byref(n), arr))
+        return [dict(order=e.order, restriction=e.restriction, threads=e.threads, merge=e.merge, pairs=e.pairs,
+                     lds=e.lds) for e in arr[:n.value]]
+
+    def set_family(self, order, restriction, **kw):
+        """Set one family's options (threads 64, 128 or 256; merge, pairs,
+        lds 0 or 1); the others keep theirs."""
+        cur = next((f for f in self.families if f["order"] == order and f["restriction"] == int(restriction)),
+                   dict(order=order, restriction=int(restriction), threads=256, merge=1, pairs=0, lds=0))
+        cur.update({k: int(v) for k, v in kw.items()})
+        o = _abi.tah_family_options(int(order), int(restriction), cur["threads"], cur["merge"], cur["pairs"],
+                                    cur["lds"])
+        check(lib().tah_plan_set_family_options(self._h, C.byref(o)))
+
 
 def _layout_of(desc_ptr, nvars, kind):
     L = _abi.tah_layout()
@@ -744,10 +765,11 @@ def fill_ghosts(fs, schedule, *, boundary=None, method="staged", ctx=None):
 
 
 def tune(fs, schedule, cache_path=None, *, ctx=None):
-    """tah_tune: time every combination of the plan's launch options on
-    fs.work and keep the fastest (cached in cache_path by mesh signature).
-    Returns (options, median ms of the choice, or None when taken from the
-    cache)."""
+    """tah_tune: time every combination of the plan-wide launch options on
+    fs.work and keep the fastest, then each interpolation family's options
+    (cached in cache_path by mesh signature). Returns (options with the
+    families' under "families", median ms of the choice, or None when taken
+    from the cache)."""
     _check_fill(fs, schedule)
     if fs.work.device.type != "cuda":
         raise RuntimeError("tune runs on the GPU only")
@@ -756,7 +778,9 @@ def tune(fs, schedule, cache_path=None, *, ctx=None):
     ms = C.c_double()
     path = None if cache_path is None else str(cache_path).encode()
     check(lib().tah_tune(plan._h, C.c_void_p(fs.work.data_ptr()), fptr, path, C.byref(ms)))
-    return plan.options, (None if ms.value < 0 else ms.value)
+    opts = dict(plan.options)
+    opts["families"] = plan.families
+    return opts, (None if ms.value < 0 else ms.value)
 
 
 def kernel_class_name(cls):
@@ -791,6 +815,26 @@ def fill_ghosts_adjoint(fs, schedule, adj, *, ctx=None):
     fptr = C.c_void_p(fs.factors.data_ptr()) if fs.factors is not None else None
     check(lib().tah_fill_adjoint(plan._h, C.c_void_p(adj.data_ptr()), fptr, _stream_ptr(fs)))
     return adj
+
+
+ADJOINT_KINDS = ("adjoint_pass_kernel<2>", "adjoint_pass_kernel<1>", "adjoint_pass_kernel<0>",
+                 "adjoint_gather_kernel", "zero_kernel")
+
+
+def fill_ghosts_adjoint_timed(fs, schedule, adj, *, ctx=None):
+    """The adjoint with its launches timed one by one (tah_fill_adjoint_timed):
+    a list of (kernel name, algorithmic bytes, ms) in launch order."""
+    _check_fill(fs, schedule)
+    plan = schedule.plan(fs.layout, ctx)
+    fptr = C.c_void_p(fs.factors.data_ptr()) if fs.factors is not None else None
+    n = C.c_int32()
+    a = C.c_void_p(adj.data_ptr())
+    check(lib().tah_fill_adjoint_timed(plan._h, a, fptr, _stream_ptr(fs), C.byref(n), None, None, None))
+    kind = (C.c_int32 * max(1, n.value))()
+    nbytes = (C.c_int64 * max(1, n.value))()
+    ms = (C.c_float * max(1, n.value))()
+    check(lib().tah_fill_adjoint_timed(plan._h, a, fptr, _stream_ptr(fs), C.byref(n), kind, nbytes, ms))
+    return [(ADJOINT_KINDS[kind[i]], int(nbytes[i]), float(ms[i])) for i in range(n.value)]
 
 
 # ---------------------------------------------------------------------------
