@@ -437,6 +437,9 @@ typedef struct tah_family_options {
                        one variable per thread (0); 0 for restriction */
   int32_t lds;      /* from LDS copies of the source boxes where the layout allows (1), or the unit
                        kernels (0) */
+  int32_t vars;     /* where the layout holds variables innermost: the variables of a tile, 0 (default)
+                       for every variable, else at most this many (even, over pairs), the tile's other
+                       threads over more units */
 } tah_family_options;
 /* The families the plan's interpolations fall in and their options: call
  * with out NULL for the count in *n, then with an array of *n entries. */

@@ -68,14 +68,24 @@ def test_every_family_option_fills_alike(which):
                     continue
                 print(f"[family] {which} fork={fork} order={fam['order']} restriction={fam['restriction']} "
                       f"threads={threads} merge={merge} pairs={pairs} lds={lds}", flush=True)
-                plan.set_family(fam["order"], fam["restriction"], threads=threads, merge=merge, pairs=pairs, lds=lds)
+                plan.set_family(fam["order"], fam["restriction"], threads=threads, merge=merge, pairs=pairs, lds=lds,
+                                vars=0)
                 fs.work.copy_(init)
                 fill_ghosts(fs, s)
                 torch.cuda.synchronize()
                 assert torch.equal(fs.work.view(torch.int64), ref.view(torch.int64)), (fam, threads, merge, pairs,
                                                                                         lds)
+            for threads, nv in itertools.product((64, 256), (1, 2, 3, 4)):
+                print(f"[family] {which} fork={fork} order={fam['order']} restriction={fam['restriction']} "
+                      f"threads={threads} vars={nv}", flush=True)
+                plan.set_family(fam["order"], fam["restriction"], threads=threads, merge=1, lds=0, vars=nv,
+                                pairs=0 if fam["restriction"] else 1)
+                fs.work.copy_(init)
+                fill_ghosts(fs, s)
+                torch.cuda.synchronize()
+                assert torch.equal(fs.work.view(torch.int64), ref.view(torch.int64)), (fam, threads, nv)
             plan.set_family(fam["order"], fam["restriction"], threads=256, merge=1,
-                            pairs=0 if fam["restriction"] else 1, lds=0)
+                            pairs=0 if fam["restriction"] else 1, lds=0, vars=0)
     # the plan-wide options set every family back
     plan.set_family(fams[0]["order"], fams[0]["restriction"], threads=64)
     plan.set_options(fork=0, merge=1, defer=1, copy_per_thread=8, copy_wgs=0, interp_pairs=1, lds=0)
@@ -88,7 +98,7 @@ def test_family_options_refused():
     plan = s.plan(fs.layout)
     fam = plan.families[0]
     before = plan.families
-    for bad in (dict(threads=96), dict(threads=512), dict(merge=2), dict(lds=-1)):
+    for bad in (dict(threads=96), dict(threads=512), dict(merge=2), dict(lds=-1), dict(vars=-1), dict(vars=300)):
         with pytest.raises(ValueError):
             plan.set_family(fam["order"], fam["restriction"], **bad)
         assert plan.families == before, bad
@@ -105,7 +115,7 @@ def test_tune_caches_by_signature(tmp_path):
     opts, ms = tune(fs, s, path)
     assert ms is not None and ms > 0
     lines = path.read_text().splitlines()
-    assert len(lines) == 1 and lines[0].startswith("tah-tune-5 ")
+    assert len(lines) == 1 and lines[0].startswith("tah-tune-6 ")
     assert opts["families"] and all(f["threads"] in (64, 128, 256) for f in opts["families"])
     s.plan(fs.layout).set_options(fork=1 - opts["fork"])
     s.plan(fs.layout).set_family(opts["families"][0]["order"], opts["families"][0]["restriction"],

@@ -696,11 +696,15 @@ void build_launches(tah_plan* p) {
             const int64_t per_unit = (1 + (rb & 1)) * (1 + ((rb >> 1) & 1)) * (1 + ((rb >> 2) & 1));
             const int32_t nun = c.nunits();
             int32_t nu, nvt;
-            const int32_t T = p->fam_threads[size_t((c.cls - kClsInterp) / 8)];
+            const size_t fam = size_t((c.cls - kClsInterp) / 8);
+            const int32_t T = p->fam_threads[fam];
             if (vfast) {
               // a pair tile's thread holds two variables
-              const int32_t per_wg = cls >= kClsMergedPair ? 2 * T : T;
+              const bool pairs = cls >= kClsMergedPair;
+              const int32_t per_wg = pairs ? 2 * T : T;
               nvt = std::min(nvars, per_wg);
+              if (p->fam_vars[fam] > 0)
+                nvt = std::min(nvt, pairs ? std::max(2, p->fam_vars[fam] + (p->fam_vars[fam] & 1)) : p->fam_vars[fam]);
               nu = std::max(1, per_wg / nvt);
             } else {
               // units fastest: as many as the class has per transfer, the
@@ -895,6 +899,7 @@ void families_from_options(tah_plan* p) {
     p->fam_merge[size_t(f)] = p->merge;
     p->fam_pairs[size_t(f)] = restriction ? 0 : p->interp_pairs;
     p->fam_lds[size_t(f)] = restriction ? (p->lds & 1) : (p->lds >> 1) & 1;
+    p->fam_vars[size_t(f)] = 0;
   }
 }
 

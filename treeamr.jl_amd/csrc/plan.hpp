@@ -164,8 +164,9 @@ struct tah_plan {
   // options above set for every family until tah_plan_set_class_options
   // (or the tuner) sets one apart: its workgroup size (the units and
   // variables of a tile fill that many threads), one launch for all its
-  // unit classes, variable pairs (prolongation), LDS staging.
-  std::array<int32_t, tah::kFamilies> fam_threads{}, fam_merge{}, fam_pairs{}, fam_lds{};
+  // unit classes, variable pairs (prolongation), LDS staging, and the
+  // variables of a tile (0: every one) where variables are innermost.
+  std::array<int32_t, tah::kFamilies> fam_threads{}, fam_merge{}, fam_pairs{}, fam_lds{}, fam_vars{};
   // The outer-face hook (tah_fill_boundary): the regions in HBM and their
   // tiles -- every region's first (the values form, after phase 1's stage
   // group), then per stage group those of the regions whose slot stage it

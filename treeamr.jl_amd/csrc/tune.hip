@@ -103,9 +103,10 @@ void set_options(tah_plan* p, const tah_plan_options& o) {
     plan_register(p);
   }
   // every family to the plan-wide options
-  const auto th = p->fam_threads, me = p->fam_merge, pa = p->fam_pairs, ld = p->fam_lds;
+  const auto th = p->fam_threads, me = p->fam_merge, pa = p->fam_pairs, ld = p->fam_lds, va = p->fam_vars;
   families_from_options(p);
-  rebuild = rebuild || th != p->fam_threads || me != p->fam_merge || pa != p->fam_pairs || ld != p->fam_lds;
+  rebuild = rebuild || th != p->fam_threads || me != p->fam_merge || pa != p->fam_pairs || ld != p->fam_lds ||
+            va != p->fam_vars;
   if (rebuild) build_launches(p);
 }
 
@@ -138,6 +139,7 @@ tah_family_options family_options(const tah_plan* p, int f) {
   o.merge = p->fam_merge[size_t(f)];
   o.pairs = p->fam_pairs[size_t(f)];
   o.lds = p->fam_lds[size_t(f)];
+  o.vars = p->fam_vars[size_t(f)];
   return o;
 }
 
@@ -151,19 +153,21 @@ void set_family(tah_plan* p, const tah_family_options& o) {
   if (o.merge < 0 || o.merge > 1 || o.pairs < 0 || o.pairs > 1 || o.lds < 0 || o.lds > 1)
     fail("family options merge, pairs and lds are 0 or 1");
   if (o.restriction && o.pairs) fail("variable pairs are for prolongation (pairs must be 0 for restriction)");
+  if (o.vars < 0 || o.vars > o.threads) fail("a family's variables per tile are 0 (every one) to its threads");
   const size_t i = size_t(f);
   if (p->fam_threads[i] == o.threads && p->fam_merge[i] == o.merge && p->fam_pairs[i] == o.pairs &&
-      p->fam_lds[i] == o.lds)
+      p->fam_lds[i] == o.lds && p->fam_vars[i] == o.vars)
     return;
   p->fam_threads[i] = o.threads;
   p->fam_merge[i] = o.merge;
   p->fam_pairs[i] = o.pairs;
   p->fam_lds[i] = o.lds;
+  p->fam_vars[i] = o.vars;
   build_launches(p);
 }
 
-// A cache line: "tah-tune-5 <signature> fork merge defer cpt wgs pairs lds
-// <families> (order restriction threads merge pairs lds)... <ms>".
+// A cache line: "tah-tune-6 <signature> fork merge defer cpt wgs pairs lds
+// <families> (order restriction threads merge pairs lds vars)... <ms>".
 bool lookup(const std::string& path, const std::string& key, tah_plan_options& o,
             std::vector<tah_family_options>& fams) {
   std::ifstream in(path);
@@ -174,12 +178,13 @@ bool lookup(const std::string& path, const std::string& key, tah_plan_options& o
     std::istringstream ss(line);
     std::string tag, k;
     int f, m, d, c, w, ip, l, nf;
-    if (!(ss >> tag >> k >> f >> m >> d >> c >> w >> ip >> l >> nf) || tag != "tah-tune-5" || k != key || nf < 0 ||
+    if (!(ss >> tag >> k >> f >> m >> d >> c >> w >> ip >> l >> nf) || tag != "tah-tune-6" || k != key || nf < 0 ||
         nf > kFamilies)
       continue;
     std::vector<tah_family_options> fs(static_cast<size_t>(nf));
     bool ok = true;
-    for (auto& e : fs) ok = ok && bool(ss >> e.order >> e.restriction >> e.threads >> e.merge >> e.pairs >> e.lds);
+    for (auto& e : fs)
+      ok = ok && bool(ss >> e.order >> e.restriction >> e.threads >> e.merge >> e.pairs >> e.lds >> e.vars);
     if (!ok) continue;
     o = tah_plan_options{f, m, d, c, 0, w, ip, l, 0, {0, 0, 0}};
     fams = std::move(fs);
@@ -204,25 +209,34 @@ std::vector<tah_plan_options> candidates(const tah_plan* p) {
   return out;
 }
 
-// One family's candidates: workgroup sizes, one launch or one per unit
-// class, variable pairs (prolongation, where the layout allows), LDS
-// staging (where the layout allows and a staged kernel exists).
-std::vector<tah_family_options> family_candidates(const tah_plan* p, int f) {
+// One family's knobs and their values: workgroup size, variables per tile
+// (where variables are innermost), one launch or one per unit class,
+// variable pairs (prolongation, where the layout allows), LDS staging
+// (where the layout allows and a staged kernel exists).
+std::vector<std::vector<int32_t>> family_knobs(const tah_plan* p, int f) {
   const int64_t* st = p->layout.stride;
+  const bool vfast = st[3] < st[0];
   const bool pairs_ok = st[3] == 1 && p->layout.nvars % 2 == 0 && st[0] % 2 == 0 && st[1] % 2 == 0 &&
                         st[2] % 2 == 0 && st[4] % 2 == 0;
   const int P = kInterpTable[size_t(f)][0];
   const bool restriction = kInterpTable[size_t(f)][1] != 0;
   const bool lds_ok = pairs_ok && (restriction ? (P == 2 || P == 4 || P == 6) : (P >= 2 && P <= 6));
-  std::vector<tah_family_options> out;
-  for (int t : {256, 128, 64})
-    for (int m = 1; m >= 0; --m)
-      for (int pr = (!restriction && pairs_ok) ? 1 : 0; pr >= 0; --pr)
-        for (int l = 0; l <= (lds_ok ? 1 : 0); ++l) {
-          if (pr && !m) continue;  // pairs are a merged launch's
-          out.push_back(tah_family_options{P, restriction ? 1 : 0, t, m, pr, l});
-        }
-  return out;
+  std::vector<int32_t> vars{0};
+  if (vfast)
+    for (int v : {8, 4, 2, 1})
+      if (v < p->layout.nvars) vars.push_back(v);
+  return {{256, 128, 64}, vars, {1, 0}, (!restriction && pairs_ok) ? std::vector<int32_t>{1, 0} : std::vector<int32_t>{0},
+          lds_ok ? std::vector<int32_t>{0, 1} : std::vector<int32_t>{0}};
+}
+
+int32_t& knob(tah_family_options& o, int k) {
+  switch (k) {
+    case 0: return o.threads;
+    case 1: return o.vars;
+    case 2: return o.merge;
+    case 3: return o.pairs;
+    default: return o.lds;
+  }
 }
 
 }  // namespace
@@ -356,18 +370,26 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
         }
       }
       set_options(p, best);
-      // 2. from there, each family by the time of its own launches
+      // 2. from there, each family by the time of its own launches: one
+      // knob at a time over its values, the others at their best so far,
+      // twice round
       for (int f : plan_families(p)) {
         tah_family_options keep = family_options(p, f);
         float keep_t = time_family(f);
-        for (const tah_family_options& c : family_candidates(p, f)) {
-          set_family(p, c);
-          const float t = time_family(f);
-          if (t < keep_t) {
-            keep_t = t;
-            keep = c;
-          }
-        }
+        const auto knobs = family_knobs(p, f);
+        for (int round = 0; round < 2; ++round)
+          for (int k = 0; k < int(knobs.size()); ++k)
+            for (int32_t v : knobs[size_t(k)]) {
+              tah_family_options c = keep;
+              knob(c, k) = v;
+              if (knob(c, k) == knob(keep, k) || (c.pairs && !c.merge)) continue;  // pairs are a merged launch's
+              set_family(p, c);
+              const float t = time_family(f);
+              if (t < keep_t) {
+                keep_t = t;
+                keep = c;
+              }
+            }
         set_family(p, keep);
       }
       // 3. each family's choice where the whole fill is no slower with it
@@ -381,6 +403,7 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
         base.merge = p->merge;
         base.pairs = base.restriction ? 0 : p->interp_pairs;
         base.lds = base.restriction ? (p->lds & 1) : (p->lds >> 1) & 1;
+        base.vars = 0;
         if (std::memcmp(&chosen, &base, sizeof base) == 0) continue;
         float tc = 1e30f, tb = 1e30f;
         for (int r = 0; r < 3; ++r) {
@@ -409,12 +432,12 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
     if (cache_path) {
       std::ofstream out(cache_path, std::ios::app);
       if (!out) fail(std::string("tah_tune: cannot write the cache file ") + cache_path);
-      out << "tah-tune-5 " << key << " " << best.fork << " " << best.merge << " " << best.defer << " "
+      out << "tah-tune-6 " << key << " " << best.fork << " " << best.merge << " " << best.defer << " "
           << best.copy_per_thread << " " << best.copy_wgs << " " << best.interp_pairs << " " << best.lds << " "
           << best_fams.size();
       for (const auto& f : best_fams)
         out << " " << f.order << " " << f.restriction << " " << f.threads << " " << f.merge << " " << f.pairs << " "
-            << f.lds;
+            << f.lds << " " << f.vars;
       out << " " << best_t << "\n";
     }
   });

@@ -57,7 +57,7 @@ DP = C.POINTER(C.c_double)
 
 class tah_family_options(C.Structure):
     _fields_ = [("order", C.c_int32), ("restriction", C.c_int32), ("threads", C.c_int32), ("merge", C.c_int32),
-                ("pairs", C.c_int32), ("lds", C.c_int32)]
+                ("pairs", C.c_int32), ("lds", C.c_int32), ("vars", C.c_int32)]
 
 
 class tah_plan_options(C.Structure):

@@ -361,22 +361,23 @@ class Plan:
     @property
     def families(self):
         """The interpolation families of the plan and their options: a list of
-        dict(order=, restriction=, threads=, merge=, pairs=, lds=)."""
+        dict(order=, restriction=, threads=, merge=, pairs=, lds=, vars=)."""
         n = C.c_int32()
         check(lib().tah_plan_get_family_options(self._h, C.byref(n), None))
         arr = (_abi.tah_family_options * max(1, n.value))()
         check(lib().tah_plan_get_family_options(self._h, C.byref(n), arr))
         return [dict(order=e.order, restriction=e.restriction, threads=e.threads, merge=e.merge, pairs=e.pairs,
-                     lds=e.lds) for e in arr[:n.value]]
+                     lds=e.lds, vars=e.vars) for e in arr[:n.value]]
 
     def set_family(self, order, restriction, **kw):
         """Set one family's options (threads 64, 128 or 256; merge, pairs,
-        lds 0 or 1); the others keep theirs."""
+        lds 0 or 1; vars 0 or a cap on a tile's variables); the others keep
+        theirs."""
         cur = next((f for f in self.families if f["order"] == order and f["restriction"] == int(restriction)),
-                   dict(order=order, restriction=int(restriction), threads=256, merge=1, pairs=0, lds=0))
+                   dict(order=order, restriction=int(restriction), threads=256, merge=1, pairs=0, lds=0, vars=0))
         cur.update({k: int(v) for k, v in kw.items()})
         o = _abi.tah_family_options(int(order), int(restriction), cur["threads"], cur["merge"], cur["pairs"],
-                                    cur["lds"])
+                                    cur["lds"], cur["vars"])
         check(lib().tah_plan_set_family_options(self._h, C.byref(o)))
 
 
