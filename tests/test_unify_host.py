@@ -136,3 +136,11 @@ def test_face_centred_unify_averages_across_the_face():
         face[cz * 8:(cz + 1) * 8, cy * 8:(cy + 1) * 8] = before[b, 0, :, :, 0]
     want = face.view(8, 2, 8, 2).mean(dim=(1, 3))
     torch.testing.assert_close(arr[coarse, 0, :, :, 8], want, rtol=0, atol=1e-15)
+
+
+def test_unify_gpu_seed_list_draws_vertex_forests():
+    """The seed list the GPU unify test loops over is exactly the vertex-centred
+    draws among seeds 20..39, so that its coverage cannot shrink unnoticed."""
+    from treeamr_hip import workloads as W
+    from test_gpu_unify import VERTEX_SEEDS
+    assert VERTEX_SEEDS == tuple(s for s in range(20, 40) if W.random_forest(s, max_blocks=300)[4] == "vertex")
