@@ -433,8 +433,11 @@ typedef struct tah_family_options {
   int32_t threads;  /* workgroup size of its launches: 256 (default), 128 or 64; a tile's units and
                        variables fill that many threads */
   int32_t merge;    /* every unit class of the family in one launch (1) or one launch each (0) */
-  int32_t pairs;    /* prolongation over pairs of neighbouring variables where the layout allows (1) or
-                       one variable per thread (0); 0 for restriction */
+  int32_t pairs;    /* prolongation over pairs of neighbouring variables where variables are innermost
+                       (1; 16-byte loads where the count is even and the array aligned, else pairs of
+                       single loads, the last variable of an odd count alone) or one variable per
+                       thread (0); 0 for restriction. The plan-wide interp_pairs sets it for even
+                       counts only: over an odd count pairs measured slower */
   int32_t lds;      /* from LDS copies of the source boxes where the layout allows (1), or the unit
                        kernels (0) */
   int32_t vars;     /* where the layout holds variables innermost: the variables of a tile, 0 (default)

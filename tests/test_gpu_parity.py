@@ -126,6 +126,32 @@ def test_variable_pair_tiles_at_the_extremes(nvars):
     _check(fs, sched, "staged")
 
 
+@pytest.mark.parametrize("nvars", [3, 5, 9, 515])
+@pytest.mark.parametrize("centering", ["cell", "vertex"])
+def test_variable_pairs_over_odd_counts(nvars, centering):
+    """Arena layout with an odd variable count: prolongation over pairs of
+    single loads, the last variable of each tile alone (515: a tile of 512
+    variables and one of 3), at workgroups of 256 and 64 threads, and with
+    the pairs off, each bit for bit the oracle."""
+    from treeamr_hip import Forest, Operators
+    f = Forest((2, 2, 2), N=8, periodic=(True,) * 3)
+    f.refine([0])
+    f.balance()
+    f.refine([2])
+    f.balance()
+    fs, sched = _prepare(f, Operators(prolongation=4, restriction=4), dict(nvars=nvars, G=2, centering=centering),
+                         layout="arena", seed=nvars)
+    plan = sched.plan(fs.layout)
+    init = fs.work.clone()
+    fam = next(x for x in plan.families if not x["restriction"])
+    for threads, pairs in ((256, 1), (64, 1), (256, 0)):
+        print(f"[odd pairs] nvars={nvars} {centering} threads={threads} pairs={pairs}", flush=True)
+        plan.set_family(fam["order"], 0, threads=threads, pairs=pairs, merge=1)
+        fs.work.copy_(init)
+        _check(fs, sched, "staged")
+    plan.set_family(fam["order"], 0, threads=256, pairs=1, merge=1)
+
+
 @pytest.mark.parametrize("seed", [2, 3, 4, 5, 6, 12])
 def test_random_forests_bit_exact(seed):
     """Forests of the batched configuration, cell (seeds 2, 4, 5) and vertex

@@ -395,7 +395,9 @@ __device__ __forceinline__ void interp_body(const Job& job, const Tile& tile, co
   typedef double pair_t __attribute__((ext_vector_type(2)));
   const DevGroup& g = job.groups[wc.group];
   const int32_t t = int32_t(threadIdx.x);
-  const int32_t lanes = MODE == 0 ? tile.nvt : tile.nvt / 2;  // threads per unit
+  // threads per unit: one per variable, or per pair (the last one of an odd
+  // count alone)
+  const int32_t lanes = MODE == 0 ? tile.nvt : (tile.nvt + 1) / 2;
   if (t >= tile.nu * lanes) return;
   int32_t ul, vl;
   if (MODE != 0 || job.vfast) {
@@ -406,6 +408,10 @@ __device__ __forceinline__ void interp_body(const Job& job, const Tile& tile, co
     ul = t - vl * tile.nu;
   }
   const int32_t v = tile.v0 + NV * vl;
+  // MODE 1 over an odd count: the last lane's second variable is none; it
+  // loads its own variable twice and stores one.
+  const bool has2 = MODE != 1 || v + 1 < tile.v0 + tile.nvt;
+  const int32_t o1 = has2 ? 1 : 0;
   // The tile's units run on across consecutive transfers of the group.
   int32_t u = tile.e0 + ul, tr = tile.transfer;
   const int32_t nun = wc.nunits;
@@ -446,7 +452,7 @@ __device__ __forceinline__ void interp_body(const Job& job, const Tile& tile, co
         row[NV - 1][m0] = x.y;
       } else {
         row[0][m0] = p[s0 * m0];
-        if (MODE == 1) row[NV - 1][m0] = p[s0 * m0 + 1];
+        if (MODE == 1) row[NV - 1][m0] = p[s0 * m0 + o1];
       }
     }
   };
@@ -526,7 +532,7 @@ __device__ __forceinline__ void interp_body(const Job& job, const Tile& tile, co
           *reinterpret_cast<pair_t*>(q) = out;
         } else {
           q[0] = x;
-          if (MODE == 1) q[1] = scaled(acc[NV - 1][a][b][c], job.factors, g.factorcol, nvars, v + 1);
+          if (MODE == 1 && has2) q[1] = scaled(acc[NV - 1][a][b][c], job.factors, g.factorcol, nvars, v + 1);
         }
       }
 }

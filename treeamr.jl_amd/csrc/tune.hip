@@ -216,11 +216,12 @@ std::vector<tah_plan_options> candidates(const tah_plan* p) {
 std::vector<std::vector<int32_t>> family_knobs(const tah_plan* p, int f) {
   const int64_t* st = p->layout.stride;
   const bool vfast = st[3] < st[0];
-  const bool pairs_ok = st[3] == 1 && p->layout.nvars % 2 == 0 && st[0] % 2 == 0 && st[1] % 2 == 0 &&
-                        st[2] % 2 == 0 && st[4] % 2 == 0;
+  const bool staged_ok = st[3] == 1 && p->layout.nvars % 2 == 0 && st[0] % 2 == 0 && st[1] % 2 == 0 &&
+                         st[2] % 2 == 0 && st[4] % 2 == 0;
+  const bool pairs_ok = st[3] == 1 && p->layout.nvars > 1;
   const int P = kInterpTable[size_t(f)][0];
   const bool restriction = kInterpTable[size_t(f)][1] != 0;
-  const bool lds_ok = pairs_ok && (restriction ? (P == 2 || P == 4 || P == 6) : (P >= 2 && P <= 6));
+  const bool lds_ok = staged_ok && (restriction ? (P == 2 || P == 4 || P == 6) : (P >= 2 && P <= 6));
   std::vector<int32_t> vars{0};
   if (vfast)
     for (int v : {8, 4, 2, 1})
@@ -401,7 +402,7 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
         tah_family_options base = chosen;
         base.threads = kThreads;
         base.merge = p->merge;
-        base.pairs = base.restriction ? 0 : p->interp_pairs;
+        base.pairs = base.restriction || p->layout.nvars % 2 ? 0 : p->interp_pairs;
         base.lds = base.restriction ? (p->lds & 1) : (p->lds >> 1) & 1;
         base.vars = 0;
         if (std::memcmp(&chosen, &base, sizeof base) == 0) continue;
