@@ -410,9 +410,11 @@ typedef struct tah_plan_options {
   int32_t fuse;            /* must be 0: the fused copy-and-interpolation launch is switched off */
   int32_t copy_wgs;        /* copies from a bounded grid of this many workgroups (a multiple of 8), or 0 */
   int32_t interp_pairs;    /* merged interpolations over variable pairs where the layout allows (1, default) */
-  int32_t lds;             /* interpolation from LDS copies of its source boxes, chunk by chunk, where the
-                              layout holds variables innermost in even number: bit 0 restriction, bit 1
-                              prolongation's unit classes of 64 units or more (0, the default: unit kernels) */
+  int32_t lds;             /* interpolation from LDS copies of its source boxes: bit 2 restriction streamed
+                              along z through LDS, every layout (4, the default); else, where the layout
+                              holds variables innermost in even number, bit 0 restriction and bit 1
+                              prolongation's unit classes of 64 units or more chunk by chunk; unit kernels
+                              for the rest */
   int32_t queue;           /* in a batched fill of plans that all set it: one persistent launch per stage
                               position over a work queue of every plan's tiles (1), or a launch per kernel
                               class (0, default); reserved_ must be zero */
@@ -438,8 +440,8 @@ typedef struct tah_family_options {
                        single loads, the last variable of an odd count alone) or one variable per
                        thread (0); 0 for restriction. The plan-wide interp_pairs sets it for even
                        counts only: over an odd count pairs measured slower */
-  int32_t lds;      /* from LDS copies of the source boxes where the layout allows (1), or the unit
-                       kernels (0) */
+  int32_t lds;      /* restriction streamed along z through LDS (2, any layout), from LDS copies of the
+                       source boxes chunk by chunk where the layout allows (1), or the unit kernels (0) */
   int32_t vars;     /* where the layout holds variables innermost: the variables of a tile, 0 (default)
                        for every variable, else at most this many (even, over pairs), the tile's other
                        threads over more units */

@@ -75,11 +75,11 @@ def test_hook_fill_under_launch_options(C):
     plan = s.plan(fs.layout)
     hook = _hooks(3, 4)["relative"]
     for opts in (dict(defer=1), dict(defer=0), dict(fork=1, defer=1), dict(merge=0, copy_per_thread=4),
-                 dict(lds=3, defer=1)):
+                 dict(lds=3, defer=1), dict(lds=0, defer=1), dict(lds=4, fork=1, defer=0)):
         plan.set_options(**opts)
         print(f"[hooks options] {C} {opts}", flush=True)
         _against_oracle(fs, s, hook, str(opts))
-    plan.set_options(fork=0, merge=1, defer=1, copy_per_thread=8, lds=0)
+    plan.set_options(fork=0, merge=1, defer=1, copy_per_thread=8, lds=4)
 
 
 @pytest.mark.parametrize("C", ["cell", "vertex"])

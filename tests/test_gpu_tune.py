@@ -34,7 +34,7 @@ def test_every_option_combination_fills_alike(which):
     ref = fs.work.clone()
     plan = s.plan(fs.layout)
     for fork, merge, defer, cpt, wgs, ip, lds in itertools.product((0, 1), (0, 1), (0, 1), (4, 8, 16),
-                                                                   (0, 64, 512), (0, 1), (0, 1, 2, 3)):
+                                                                   (0, 64, 512), (0, 1), (0, 1, 2, 3, 4, 6)):
         print(f"[tune] {which} fork={fork} merge={merge} defer={defer} copy_per_thread={cpt} copy_wgs={wgs} "
               f"interp_pairs={ip} lds={lds}", flush=True)
         plan.set_options(fork=fork, merge=merge, fuse=0, defer=defer, copy_per_thread=cpt, copy_wgs=wgs,
@@ -44,7 +44,7 @@ def test_every_option_combination_fills_alike(which):
         torch.cuda.synchronize()
         assert torch.equal(fs.work.view(torch.int64), ref.view(torch.int64)), (fork, merge, defer, cpt, wgs, ip,
                                                                                 lds)
-    plan.set_options(fork=0, merge=1, fuse=0, defer=1, copy_per_thread=8, copy_wgs=0, interp_pairs=1, lds=0)
+    plan.set_options(fork=0, merge=1, fuse=0, defer=1, copy_per_thread=8, copy_wgs=0, interp_pairs=1, lds=4)
 
 
 @pytest.mark.parametrize("which", ["cell", "cell-even", "vertex"])
@@ -63,8 +63,8 @@ def test_every_family_option_fills_alike(which):
     for fork in (0, 1):
         plan.set_options(fork=fork)
         for fam in fams:
-            for threads, merge, pairs, lds in itertools.product((64, 128, 256), (0, 1), (0, 1), (0, 1)):
-                if fam["restriction"] and pairs or pairs and not merge:
+            for threads, merge, pairs, lds in itertools.product((64, 128, 256), (0, 1), (0, 1), (0, 1, 2)):
+                if fam["restriction"] and pairs or pairs and not merge or lds == 2 and not fam["restriction"]:
                     continue
                 print(f"[family] {which} fork={fork} order={fam['order']} restriction={fam['restriction']} "
                       f"threads={threads} merge={merge} pairs={pairs} lds={lds}", flush=True)
@@ -85,10 +85,10 @@ def test_every_family_option_fills_alike(which):
                 torch.cuda.synchronize()
                 assert torch.equal(fs.work.view(torch.int64), ref.view(torch.int64)), (fam, threads, nv)
             plan.set_family(fam["order"], fam["restriction"], threads=256, merge=1,
-                            pairs=0 if fam["restriction"] else 1, lds=0, vars=0)
+                            pairs=0 if fam["restriction"] else 1, lds=2 if fam["restriction"] else 0, vars=0)
     # the plan-wide options set every family back
     plan.set_family(fams[0]["order"], fams[0]["restriction"], threads=64)
-    plan.set_options(fork=0, merge=1, defer=1, copy_per_thread=8, copy_wgs=0, interp_pairs=1, lds=0)
+    plan.set_options(fork=0, merge=1, defer=1, copy_per_thread=8, copy_wgs=0, interp_pairs=1, lds=4)
     assert all(f["threads"] == 256 for f in plan.families)
 
 
@@ -98,7 +98,8 @@ def test_family_options_refused():
     plan = s.plan(fs.layout)
     fam = plan.families[0]
     before = plan.families
-    for bad in (dict(threads=96), dict(threads=512), dict(merge=2), dict(lds=-1), dict(vars=-1), dict(vars=300)):
+    for bad in (dict(threads=96), dict(threads=512), dict(merge=2), dict(lds=-1), dict(lds=3), dict(vars=-1),
+                dict(vars=300)):
         with pytest.raises(ValueError):
             plan.set_family(fam["order"], fam["restriction"], **bad)
         assert plan.families == before, bad
@@ -152,7 +153,7 @@ def test_option_values_refused():
     plan = s.plan(fs.layout)
     before = plan.options
     for bad in (dict(interp_pairs=2), dict(interp_pairs=-1), dict(copy_per_thread=5), dict(copy_wgs=12),
-                dict(fork=2), dict(lds=4), dict(queue=2), dict(queue=-1)):
+                dict(fork=2), dict(lds=8), dict(lds=-1), dict(queue=2), dict(queue=-1)):
         with pytest.raises(ValueError):
             plan.set_options(**bad)
         assert plan.options == before, bad

@@ -123,6 +123,45 @@ struct alignas(16) LdsUnitChunk {
 // 16-byte pairs of variables).
 constexpr int kLdsMaxBytes = 64 * 1024;
 
+// The staging slots of a streamed chunk's plane per thread: a plane holds at
+// most kStreamSlots * kThreads pairs of variables.
+constexpr int kStreamSlots = 8;
+// The most z windows of a streamed restriction of order P that may hold one
+// plane (the kernel's accumulators per thread): P/2 where windows start two
+// rows apart, one more where the first two windows share a start (clamped
+// into the fine block's interior); kStreamKA for every order here.
+constexpr int stream_ka(int P) { return P / 2 + 1; }
+constexpr int kStreamKA = 4;
+
+// One source plane of a streamed restriction's group, the planes zlo ..
+// zlo + nplanes - 1 in ascending order: the z rows whose windows hold it are
+// slots 0 .. nact-1, in window order, and c2[q] is slot q's weight along z
+// at this plane; slots had .. nact-1 start here (at +0); the first nret
+// slots end here -- rows k[0 .. nret-1], stored after the plane -- and the
+// rest then move down by nret.
+struct alignas(16) StreamPlane {
+  double c2[kStreamKA];
+  int32_t k[kStreamKA];
+  int32_t had, nact, nret, pad;
+};
+
+// A chunk of a restriction's target box streamed along z: target rows
+// tlo[d] .. tlo[d]+tn[d]-1 along x and y (d = 0, 1) and every row along z.
+// Each source plane z (rows slo[d] .. slo[d]+sext[d]-1 along x and y, the
+// union of the chunk's windows) is staged in LDS with a tile's variables
+// innermost, point (x, y) at (y*sext[0] + x) * nvp doubles, nvp the tile's
+// variable count rounded up to even. Its group's planes are
+// planes[plane0 .. plane0 + nplanes - 1] of the job's list, the first at
+// source row zlo. nvt: variables per tile.
+struct alignas(16) StreamChunk {
+  int32_t group;
+  int32_t tlo[2], tn[2];
+  int32_t slo[2], sext[2];
+  int32_t plane0, nplanes, zlo;
+  int32_t nvt;
+  int32_t pad[3];
+};
+
 // One workgroup's share of an LDS-staged launch: chunk `chunk` (of the job's
 // chunk list) of transfer `transfer`, variables v0 and v0+1.
 struct alignas(16) LdsTile {
@@ -141,6 +180,8 @@ struct alignas(16) Job {
   const LdsUnitChunk* uchunks;  // ... of prolongation, their class waves and units
   const LdsWave* waves;
   const uint32_t* units;
+  const StreamChunk* schunks;  // the plan's restriction chunks streamed along z
+  const StreamPlane* planes;   // ... and their groups' planes
   double* work;        // the targets' array
   const double* src;   // the sources' array: work, or a regrid's old array
   const double* factors;

@@ -120,6 +120,16 @@ void launch_lds(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const 
   if (bytes <= 0 || bytes > kLdsMaxBytes) fail("bad LDS size for an LDS-staged launch");
   const dim3 grid(ntiles), blk(kThreads);
   const size_t sh = size_t(bytes);
+  if (cls >= kClsStream) {
+    switch (cls - kClsStream) {
+      case 5: hipLaunchKernelGGL((interp_stream_kernel<2, stream_ka(2)>), grid, blk, sh, s, jobs, tiles); break;
+      case 6: hipLaunchKernelGGL((interp_stream_kernel<4, stream_ka(4)>), grid, blk, sh, s, jobs, tiles); break;
+      case 7: hipLaunchKernelGGL((interp_stream_kernel<6, stream_ka(6)>), grid, blk, sh, s, jobs, tiles); break;
+      default: fail("no streamed restriction kernel for class " + std::to_string(cls));
+    }
+    HIPCHECK(hipGetLastError());
+    return;
+  }
   if (cls >= kClsLdsProlong) {
     const dim3 pblk(kLdsProlongThreads);
     switch (cls - kClsLdsProlong) {
@@ -622,6 +632,9 @@ tah_status tah_kernel_class_name(int32_t cls, char* buf, int32_t len) {
     } else if (cls >= kClsMergedPair && cls - kClsMergedPair < int(kInterpTable.size())) {
       const auto& e = kInterpTable[cls - kClsMergedPair];
       name = "interp_pair_kernel<" + std::to_string(e[0]) + ", " + std::to_string(e[1]) + ">";
+    } else if (cls >= kClsStream && cls - kClsStream < int(kInterpTable.size())) {
+      name = "interp_stream_kernel<" + std::to_string(kInterpTable[cls - kClsStream][0]) + ", " +
+             std::to_string(stream_ka(kInterpTable[cls - kClsStream][0])) + ">";
     } else if (cls >= kClsLdsProlong && cls - kClsLdsProlong < int(kInterpTable.size())) {
       name = "interp_lds_prolong_kernel<" + std::to_string(kInterpTable[cls - kClsLdsProlong][0]) + ">";
     } else if (cls >= kClsLds && cls - kClsLds < int(kInterpTable.size())) {

@@ -965,4 +965,255 @@ interp_lds_prolong_kernel(const Job* __restrict__ jobs, const LdsTile* __restric
 }
 #endif  // TAH_DEFINE_KERNELS
 
+// ---------------------------------------------------------------------------
+// Restriction of order P streamed along z through LDS. A workgroup takes one
+// chunk of one transfer (target rows along x and y, every row along z) for a
+// tile of variables and walks the source planes of its windows in ascending
+// z. Each plane -- the union of the chunk's windows along x and y, every
+// variable of the tile of each point, so that the global loads read whole
+// runs of a point's variables -- is staged in LDS once; a thread holds one
+// target (i, j) for a pair of variables, with one accumulator per z row
+// whose window holds the plane (at most KA: the rows sorted by window
+// start enter and leave in order, as the group's plane records say). For
+// each such row k the plane is tap m3 = z - srcstart[2][k] of its window;
+// the thread adds the plane's taps in the order m2, then m1, each as
+// acc += ((w1*w2)*w3)*u. So every target sees its taps in the reference's
+// order (m3, m2, m1 ascending, m1 innermost), from +0, every operation
+// rounded on its own (stencil_sum, src/ghosts.jl:58-118), and each fine
+// value crosses the vector memory path once per chunk instead of once per
+// tap.
+//
+// ASYNC (16-byte aligned pairs of variables innermost): two plane buffers;
+// the plane after next is loaded straight into LDS (global_load_lds, no
+// registers) while the threads sum the current one, one barrier a plane.
+// Otherwise one buffer, staged through registers between two barriers.
+#ifdef TAH_DEFINE_KERNELS
+template <int P, int KA, bool ASYNC>
+__device__ __forceinline__ void stream_body(const Job& job, const LdsTile& tile, double* __restrict__ lds) {
+  typedef double pair_t __attribute__((ext_vector_type(2)));
+  const StreamChunk& ch = job.schunks[tile.chunk];
+  const DevGroup& g = job.groups[ch.group];
+  const int32_t s0 = int32_t(job.stride[0]), s1 = int32_t(job.stride[1]), s2 = int32_t(job.stride[2]),
+                s3 = int32_t(job.stride[3]);
+  const int64_t s4 = job.stride[4];
+  const int32_t nvars = job.nvars;
+  const int32_t v0 = tile.v0;
+  const int32_t nv = min(ch.nvt, nvars - v0);
+  const int32_t L = (nv + 1) >> 1;  // threads per target: one per pair of variables
+  const int32_t nvp = 2 * L;        // doubles per staged point
+  const int32_t e0 = ch.sext[0], e1 = ch.sext[1];
+  const int32_t t = int32_t(threadIdx.x);
+  const bool pairs = job.pairs != 0;
+
+  // The plane's pairs, numbered with a point's pairs fastest where variables
+  // are innermost, else x fastest; thread t stages pairs t, t + kThreads, ...
+  // (at most kStreamSlots), pair vp of point (x, y) at LDS double
+  // (y*e0 + x)*nvp + 2*vp -- in the numbering's own order where variables
+  // are innermost. Odd counts: the last pair's second value is staged as
+  // zero.
+  const int32_t npairs = e0 * e1 * L;
+  const int32_t plane = e0 * e1 * nvp;  // doubles of one plane buffer
+  const bool vfast = job.vfast != 0;
+  const int32_t A = vfast ? L : e0, B = vfast ? e0 : e1;
+  const FastDiv dA{uint32_t(A)}, dB{uint32_t(B)};
+  const double* __restrict__ src = job.src + s4 * g.sblocks[tile.transfer] +
+                                   (s3 * v0 + s0 * ch.slo[0] + s1 * ch.slo[1]);
+  // pair q's offset from the plane's first staged point and the tile's
+  // first variable, and its LDS double
+  auto slot_of = [&](int32_t q, int32_t& go, int32_t& at, int32_t& vp) {
+    const int32_t r = int32_t(dA.div(uint32_t(q))), c0 = q - r * A;
+    const int32_t c2 = int32_t(dB.div(uint32_t(r))), c1 = r - c2 * B;
+    vp = vfast ? c0 : c2;
+    const int32_t x = vfast ? c1 : c0, y = vfast ? c2 : c1;
+    go = s1 * y + s0 * x + s3 * (2 * vp);
+    at = (y * e0 + x) * nvp + 2 * vp;
+  };
+  auto stage_sync = [&](int32_t z) {
+    const __attribute__((address_space(1))) double* __restrict__ p =
+        (const __attribute__((address_space(1))) double*)(src + int64_t(s2) * z);
+    pair_t val[kStreamSlots];
+    int32_t at[kStreamSlots];
+    // (the slots' offsets formed anew on every plane: kept across planes,
+    // they would hold three registers a slot)
+    int32_t tt = t;
+    asm volatile("" : "+v"(tt));
+#pragma unroll
+    for (int s = 0; s < kStreamSlots; ++s) {
+      const int32_t q = tt + s * kThreads;
+      at[s] = -1;
+      if (q >= npairs) continue;
+      int32_t go, vp;
+      slot_of(q, go, at[s], vp);
+      if (pairs) {
+        val[s] = *(const __attribute__((address_space(1))) pair_t*)(p + go);
+      } else {
+        val[s].x = p[go];
+        val[s].y = 2 * vp + 1 < nv ? p[go + s3] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < kStreamSlots; ++s)
+      if (at[s] >= 0) *reinterpret_cast<pair_t*>(lds + at[s]) = val[s];
+  };
+  // ASYNC: each wave loads its 64 consecutive pairs of a slot straight into
+  // LDS, lane l's 16 bytes at the wave's base + 16*l (the numbering's order).
+  auto stage_async = [&](int32_t z, int32_t buf) {
+    const double* __restrict__ p = src + int64_t(s2) * z;
+    int32_t tt = t;
+    asm volatile("" : "+v"(tt));
+    const int32_t wave = tt & ~63;
+#pragma unroll
+    for (int s = 0; s < kStreamSlots; ++s) {
+      const int32_t q0 = wave + s * kThreads;  // the wave's first pair: uniform
+      if (q0 >= npairs) break;
+      const int32_t q = tt + s * kThreads;
+      if (q < npairs) {
+        int32_t go, at, vp;
+        slot_of(q, go, at, vp);
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(p + go),
+                                         (__attribute__((address_space(3))) void*)(lds + buf * plane + 2 * q0), 16, 0,
+                                         0);
+      }
+    }
+  };
+
+  // This thread's target and pair of variables; threads past the chunk's
+  // targets only stage.
+  const int32_t item = t / L, vl = t - item * L;
+  const bool active = item < ch.tn[0] * ch.tn[1];
+  const int32_t b = active ? item / ch.tn[0] : 0;
+  const int32_t a = active ? item - b * ch.tn[0] : 0;
+  const int32_t i = ch.tlo[0] + a, j = ch.tlo[1] + b;
+  const int32_t ox = g.srcstart[0][i] - ch.slo[0], oy = g.srcstart[1][j] - ch.slo[1];
+  double w0[P], w1[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    w0[m] = g.weights[0][P * i + m];
+    w1[m] = g.weights[1][P * j + m];
+  }
+  const int32_t tap0 = (oy * e0 + ox) * nvp + 2 * vl;  // the window's first tap in a plane buffer (doubles)
+  const int32_t v = v0 + 2 * vl;
+  const bool has2 = 2 * vl + 1 < nv;
+  double* __restrict__ dst = job.work + s4 * g.tblocks[tile.transfer] +
+                             (s3 * v + s0 * (g.tfirst[0] + i) + s1 * (g.tfirst[1] + j));
+  int32_t tz = g.tfirst[2];
+  asm volatile("" : "+v"(tz));  // read once, not at each store
+  // the parity factors of the thread's variables, read once (scaled():
+  // acc*f + 0 where the group has a factor column)
+  const int32_t fcol = g.factorcol;
+  double fx = 1.0, fy = 1.0;
+  if (fcol != 0) {
+    fx = job.factors[v + int64_t(nvars) * (fcol - 1)];
+    fy = has2 ? job.factors[v + 1 + int64_t(nvars) * (fcol - 1)] : 0.0;
+  }
+  // the plane records by scalar loads (uniform)
+  const __attribute__((address_space(4))) StreamPlane* __restrict__ planes = constant(job.planes + ch.plane0);
+  const int32_t np = ch.nplanes, zlo = ch.zlo;
+
+  double ax[KA], ay[KA];
+#pragma unroll
+  for (int q = 0; q < KA; ++q) ax[q] = ay[q] = 0.0;
+  if (ASYNC) stage_async(zlo, 0);
+  for (int32_t pz = 0; pz < np; ++pz) {
+    const int32_t z = zlo + pz;
+    const __attribute__((address_space(4))) StreamPlane& pl = planes[pz];
+    const int32_t had = pl.had, nact = pl.nact, nret = pl.nret;
+    // rows whose windows start here join, at +0
+#pragma unroll
+    for (int q = 0; q < KA; ++q)
+      if (q >= had && q < nact) ax[q] = ay[q] = 0.0;
+    int32_t buf = 0;
+    if (ASYNC) {
+      // this thread's loads of plane z are done; after the barrier every
+      // thread's are, and every thread is done with plane z - 1, whose
+      // buffer plane z + 1 then fills
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+      __syncthreads();
+      buf = pz & 1;
+      if (pz + 1 < np) stage_async(z + 1, buf ^ 1);
+    } else {
+      __syncthreads();  // every thread is done with the previous plane
+      if (nact > 0) stage_sync(z);
+      __syncthreads();
+    }
+    if (active && nact > 0) {
+      // (the tap addresses formed anew on every plane: kept across planes,
+      // they would hold P*P registers)
+      int32_t taps = tap0 + buf * plane;
+      asm volatile("" : "+v"(taps));
+      double c2[KA];
+#pragma unroll
+      for (int q = 0; q < KA; ++q) c2[q] = pl.c2[q];
+#pragma unroll
+      for (int m2 = 0; m2 < P; ++m2) {
+        const double* __restrict__ row = lds + (taps + m2 * e0 * nvp);
+        pair_t u[P];
+        double w01[P];
+        // (formed anew on every plane: kept across planes, the row
+        // products would hold 2*P*P registers)
+        double c1 = w1[m2];
+        asm volatile("" : "+v"(c1));
+#pragma unroll
+        for (int m1 = 0; m1 < P; ++m1) {
+          u[m1] = *reinterpret_cast<const pair_t*>(row + m1 * nvp);
+          w01[m1] = __dmul_rn(w0[m1], c1);
+        }
+#pragma unroll
+        for (int q = 0; q < KA; ++q) {
+          if (q >= nact) continue;
+#pragma unroll
+          for (int m1 = 0; m1 < P; ++m1) {
+            const double w = __dmul_rn(w01[m1], c2[q]);
+            ax[q] = __dadd_rn(ax[q], __dmul_rn(w, u[m1].x));
+            ay[q] = __dadd_rn(ay[q], __dmul_rn(w, u[m1].y));
+          }
+        }
+        // Row by row: without this the compiler hoists every row's LDS
+        // reads to the top of the plane and keeps them all live.
+#pragma unroll
+        for (int q = 0; q < KA; ++q) asm volatile("" : "+v"(ax[q]), "+v"(ay[q]));
+      }
+    }
+    // rows whose windows end here are done: the first nret slots, stored,
+    // then the rest move down
+    if (nret > 0) {
+#pragma unroll
+      for (int q = 0; q < KA; ++q) {
+        if (q >= nret || !active) continue;
+        double* __restrict__ o = dst + int64_t(s2) * (tz + pl.k[q]);
+        pair_t out;
+        out.x = fcol != 0 ? __dadd_rn(__dmul_rn(ax[q], fx), 0.0) : ax[q];
+        out.y = fcol != 0 ? __dadd_rn(__dmul_rn(ay[q], fy), 0.0) : ay[q];
+        if (pairs) {
+          *reinterpret_cast<pair_t*>(o) = out;
+        } else {
+          o[0] = out.x;
+          if (has2) o[s3] = out.y;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < KA; ++r) {
+        if (r >= nret) break;
+#pragma unroll
+        for (int q = 0; q + 1 < KA; ++q) {
+          ax[q] = ax[q + 1];
+          ay[q] = ay[q + 1];
+        }
+      }
+    }
+  }
+}
+
+// LDS: two plane buffers (the launch's largest plane twice).
+template <int P, int KA>
+__global__ void __launch_bounds__(kThreads)
+interp_stream_kernel(const Job* __restrict__ jobs, const LdsTile* __restrict__ tiles) {
+  extern __shared__ __align__(16) double lds[];
+  const LdsTile tile = tiles[xcd_tile(blockIdx.x, gridDim.x)];
+  const Job& job = jobs[tile.job];
+  if (job.pairs && job.vfast) stream_body<P, KA, true>(job, tile, lds);
+  else stream_body<P, KA, false>(job, tile, lds);
+}
+#endif  // TAH_DEFINE_KERNELS
+
 }  // namespace tah

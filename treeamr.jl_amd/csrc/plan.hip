@@ -189,6 +189,129 @@ bool lds_unit_chunks_of(const tah_plan* p, const PlanGroup& g, int32_t gi, const
   return true;
 }
 
+// The plane records of restriction group g streamed along z (StreamPlane):
+// its z rows sorted by window start (stable) enter as their windows begin
+// and leave, stored, as they end. Appends to planes; *zlo is the first
+// plane's source row. False, with nothing appended, where more than
+// stream_ka(P) windows hold one plane.
+bool stream_planes_of(const PlanGroup& g, std::vector<StreamPlane>& planes, int32_t* zlo) {
+  const int32_t n2 = g.n[2], P = g.order[2];
+  std::vector<int32_t> order(static_cast<size_t>(n2));
+  for (int32_t k = 0; k < n2; ++k) order[size_t(k)] = k;
+  const auto& s = g.srcstart[2];
+  std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return s[size_t(x)] < s[size_t(y)]; });
+  auto start = [&](int32_t o) { return s[size_t(order[size_t(o)])]; };
+  const size_t first = planes.size();
+  *zlo = start(0);
+  int32_t ol = 0, oh = -1;
+  for (int32_t z = start(0); z <= start(n2 - 1) + P - 1; ++z) {
+    StreamPlane pl{};
+    pl.had = oh - ol + 1;
+    while (oh + 1 < n2 && start(oh + 1) <= z) ++oh;
+    pl.nact = oh - ol + 1;
+    if (pl.nact > stream_ka(P) || pl.nact > kStreamKA) {
+      planes.resize(first);
+      return false;
+    }
+    for (int32_t q = 0; q < pl.nact; ++q) {
+      const int32_t k = order[size_t(ol + q)];
+      pl.c2[q] = g.weights[2][size_t(P) * k + size_t(z - s[size_t(k)])];
+    }
+    while (ol + pl.nret <= oh && start(ol + pl.nret) + P - 1 == z) {
+      pl.k[pl.nret] = order[size_t(ol + pl.nret)];
+      ++pl.nret;
+    }
+    ol += pl.nret;
+    planes.push_back(pl);
+  }
+  return true;
+}
+
+// The chunks of restriction group g (index gi) streamed along z, for tiles
+// of nvt variables: the target rows along x and along y cut into balanced
+// runs, so that a chunk's targets, each taking one thread per pair of
+// variables, fill at most a workgroup, and its plane (the union of its
+// windows along x and y, every variable of a tile) takes at most
+// kStreamSlots pairs per thread. Of the cuts that fit, the one with the
+// fewest chunks, then the fewest staged points. None where none fits.
+std::vector<StreamChunk> stream_chunks_of(const PlanGroup& g, int32_t gi, int32_t nvt, int32_t plane0,
+                                          int32_t nplanes, int32_t zlo) {
+  const int32_t L = (nvt + 1) / 2;
+  const int32_t items = kThreads / L;
+  if (items < 1) return {};
+  const int32_t n0 = g.n[0], n1 = g.n[1];
+  auto part = [](int32_t n, int32_t m, int32_t p) { return int32_t(int64_t(n) * p / m); };
+  auto extent = [&](int d, int32_t r0, int32_t r1, int32_t* lo) {
+    int32_t smin = INT32_MAX, smax = INT32_MIN;
+    for (int32_t r = r0; r < r1; ++r) {
+      smin = std::min(smin, g.srcstart[d][size_t(r)]);
+      smax = std::max(smax, g.srcstart[d][size_t(r)]);
+    }
+    *lo = smin;
+    return smax + g.order[d] - smin;
+  };
+  int64_t best_chunks = INT64_MAX, best_staged = INT64_MAX;
+  int32_t best_m0 = 0, best_m1 = 0;
+  for (int32_t c0 = 1; c0 <= std::min(n0, items); ++c0) {
+    const int32_t c1 = std::min(n1, items / c0);
+    if (c1 < 1) continue;
+    const int32_t m0 = (n0 + c0 - 1) / c0, m1 = (n1 + c1 - 1) / c1;
+    int64_t sum0 = 0, sum1 = 0;
+    int32_t max0 = 0, max1 = 0, lo;
+    for (int32_t q = 0; q < m0; ++q) {
+      const int32_t e = extent(0, part(n0, m0, q), part(n0, m0, q + 1), &lo);
+      sum0 += e;
+      max0 = std::max(max0, e);
+    }
+    for (int32_t q = 0; q < m1; ++q) {
+      const int32_t e = extent(1, part(n1, m1, q), part(n1, m1, q + 1), &lo);
+      sum1 += e;
+      max1 = std::max(max1, e);
+    }
+    if (int64_t(max0) * max1 * L > int64_t(kStreamSlots) * kThreads) continue;
+    if (2 * int64_t(max0) * max1 * 2 * L * 8 > kLdsMaxBytes) continue;
+    const int64_t chunks = int64_t(m0) * m1, staged = sum0 * sum1;
+    if (chunks < best_chunks || (chunks == best_chunks && staged < best_staged)) {
+      best_chunks = chunks;
+      best_staged = staged;
+      best_m0 = m0;
+      best_m1 = m1;
+    }
+  }
+  std::vector<StreamChunk> out;
+  if (best_m0 == 0) return out;
+  for (int32_t qy = 0; qy < best_m1; ++qy)
+    for (int32_t qx = 0; qx < best_m0; ++qx) {
+      StreamChunk ch{};
+      ch.group = gi;
+      ch.tlo[0] = part(n0, best_m0, qx);
+      ch.tn[0] = part(n0, best_m0, qx + 1) - ch.tlo[0];
+      ch.tlo[1] = part(n1, best_m1, qy);
+      ch.tn[1] = part(n1, best_m1, qy + 1) - ch.tlo[1];
+      ch.sext[0] = extent(0, ch.tlo[0], ch.tlo[0] + ch.tn[0], &ch.slo[0]);
+      ch.sext[1] = extent(1, ch.tlo[1], ch.tlo[1] + ch.tn[1], &ch.slo[1]);
+      ch.plane0 = plane0;
+      ch.nplanes = nplanes;
+      ch.zlo = zlo;
+      ch.nvt = nvt;
+      // What interp_stream_kernel assumes of its chunk, checked before any
+      // launch: its targets' threads fit the workgroup, its plane fits the
+      // staging slots and LDS, and every target's window lies in the plane.
+      const int32_t P = g.order[0];
+      bool ok = ch.tn[0] >= 1 && ch.tn[1] >= 1 && int64_t(ch.tn[0]) * ch.tn[1] * L <= kThreads &&
+                int64_t(ch.sext[0]) * ch.sext[1] * L <= int64_t(kStreamSlots) * kThreads &&
+                2 * int64_t(ch.sext[0]) * ch.sext[1] * 2 * L * 8 <= kLdsMaxBytes && ch.slo[0] >= 0 && ch.slo[1] >= 0;
+      for (int d = 0; d < 2 && ok; ++d)
+        for (int32_t r = ch.tlo[d]; r < ch.tlo[d] + ch.tn[d]; ++r) {
+          const int32_t o = g.srcstart[d][size_t(r)] - ch.slo[d];
+          ok = ok && o >= 0 && o + P <= ch.sext[d];
+        }
+      if (!ok) fail("internal: a streamed restriction chunk does not fit its kernel");
+      out.push_back(ch);
+    }
+  return out;
+}
+
 // Bitwise key of a vector of doubles.
 std::vector<uint64_t> bits_of(const double* w, size_t n) {
   std::vector<uint64_t> k(n);
@@ -546,9 +669,11 @@ void build_launches(tah_plan* p) {
   };
   // The family of a group (-1: an order no specialised kernel takes).
   auto family_of = [&](const PlanGroup& g) { return interp_index(g.order[0], g.kind == 2 ? 2 : 0); };
+  // a group's family's LDS staging: 0 none, 1 LDS-staged chunks, 2
+  // (restriction) streamed along z
   auto fam_lds = [&](const PlanGroup& g) {
     const int f = family_of(g);
-    return f >= 0 && p->fam_lds[size_t(f)] != 0;
+    return f >= 0 ? p->fam_lds[size_t(f)] : 0;
   };
   // The launch a copy or generic group runs in.
   auto group_launch = [&](int cls) { return cls == kClsCopy ? copy_launch(p->copy_per_thread) : cls; };
@@ -562,13 +687,36 @@ void build_launches(tah_plan* p) {
   if (pair_capable)
     for (size_t gi = 0; gi < p->groups.size(); ++gi) {
       const PlanGroup& g = p->groups[gi];
-      if (g.cls == kClsInterp && g.kind == 2 && fam_lds(g)) {
+      if (g.cls == kClsInterp && g.kind == 2 && fam_lds(g) == 1) {
         const auto cs = lds_chunks_of(g, int32_t(gi));
         lds_of[gi] = {int32_t(p->lds_chunks.size()), int32_t(cs.size())};
         p->lds_chunks.insert(p->lds_chunks.end(), cs.begin(), cs.end());
       }
     }
   auto lds_launch = [&](const PlanGroup& g) { return kClsLds + interp_index(g.order[0], g.kind == 2 ? 2 : 0); };
+  // Restriction streamed along z: the groups whose z windows overlap at
+  // most stream_ka(P) to a plane and whose boxes cut into chunks that fit,
+  // in any layout; their chunks and sorted z rows.
+  p->stream_chunks.clear();
+  p->stream_planes.clear();
+  std::vector<std::array<int32_t, 2>> str_of(p->groups.size(), {0, 0});  // first chunk, count
+  const int32_t stream_nvt = std::min<int32_t>(nvars, 2 * kThreads);
+  for (size_t gi = 0; gi < p->groups.size(); ++gi) {
+    const PlanGroup& g = p->groups[gi];
+    if (g.cls != kClsInterp || g.kind != 2 || fam_lds(g) != 2 || g.n[2] < 1) continue;
+    const int32_t plane0 = int32_t(p->stream_planes.size());
+    int32_t zlo = 0;
+    if (!stream_planes_of(g, p->stream_planes, &zlo)) continue;
+    const int32_t nplanes = int32_t(p->stream_planes.size()) - plane0;
+    const auto cs = stream_chunks_of(g, int32_t(gi), stream_nvt, plane0, nplanes, zlo);
+    if (cs.empty()) {
+      p->stream_planes.resize(size_t(plane0));
+      continue;
+    }
+    str_of[gi] = {int32_t(p->stream_chunks.size()), int32_t(cs.size())};
+    p->stream_chunks.insert(p->stream_chunks.end(), cs.begin(), cs.end());
+  }
+  auto stream_launch = [&](const PlanGroup& g) { return kClsStream + interp_index(g.order[0], 2); };
   // LDS-staged prolongation by units: the unit classes of at least 64 units
   // (a face's inner units); a group's other classes stay with the unit
   // kernels.
@@ -580,7 +728,7 @@ void build_launches(tah_plan* p) {
   if (pair_capable)
     for (size_t gi = 0; gi < p->groups.size(); ++gi) {
       const PlanGroup& g = p->groups[gi];
-      if (g.cls != kClsInterp || g.kind != 3 || !fam_lds(g)) continue;
+      if (g.cls != kClsInterp || g.kind != 3 || fam_lds(g) != 1) continue;
       std::vector<int32_t> cls;
       for (int32_t ci : g.wclasses)
         if (p->wclasses[size_t(ci)].nunits() >= 64) cls.push_back(ci);
@@ -597,7 +745,9 @@ void build_launches(tah_plan* p) {
     for (size_t gi = 0; gi < p->groups.size(); ++gi) {
       const PlanGroup& g = p->groups[gi];
       if (g.points == 0 || std::find(run_in[gi].begin(), run_in[gi].end(), k) == run_in[gi].end()) continue;
-      if (lds_of[gi][1] > 0) {
+      if (str_of[gi][1] > 0) {
+        classes.push_back(stream_launch(g));
+      } else if (lds_of[gi][1] > 0) {
         classes.push_back(lds_launch(g));
       } else if (g.cls == kClsInterp) {
         if (ulds_of[gi][1] > 0) classes.push_back(ulds_launch(g));
@@ -618,6 +768,35 @@ void build_launches(tah_plan* p) {
     };
     std::vector<Built> built;
     for (int cls : classes) {
+      if (cls >= kClsStream) {
+        // A transfer's chunks one after another, each for every tile of
+        // variables in turn, ordered by source block as the unit tiles of
+        // restriction are.
+        Built B{cls, {}, 0, {}, 0};
+        for (size_t gi = 0; gi < p->groups.size(); ++gi) {
+          const PlanGroup& g = p->groups[gi];
+          if (g.points == 0 || str_of[gi][1] == 0 || stream_launch(g) != cls) continue;
+          for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t) {
+            if (run_in[gi][t] != k) continue;
+            for (int32_t c = str_of[gi][0]; c < str_of[gi][0] + str_of[gi][1]; ++c) {
+              const StreamChunk& ch = p->stream_chunks[size_t(c)];
+              for (int32_t v0 = 0; v0 < nvars; v0 += ch.nvt) {
+                const int32_t nv = std::min(ch.nvt, nvars - v0);
+                // two plane buffers
+                B.lds_bytes = std::max(B.lds_bytes, int32_t(2 * ch.sext[0] * ch.sext[1] * 2 * ((nv + 1) / 2) * 8));
+                B.ltiles.push_back({g.sblocks[t], LdsTile{0, c, t, v0}});
+                B.values += int64_t(ch.tn[0]) * ch.tn[1] * g.n[2] * nv;
+              }
+            }
+          }
+        }
+        std::stable_sort(B.ltiles.begin(), B.ltiles.end(),
+                         [](const std::pair<int32_t, LdsTile>& x, const std::pair<int32_t, LdsTile>& y) {
+                           return x.first < y.first;
+                         });
+        if (!B.ltiles.empty()) built.push_back(std::move(B));
+        continue;
+      }
       if (cls >= kClsLdsProlong) {
         // Every chunk of a transfer for every pair of variables in turn,
         // ordered by target block as the unit tiles of prolongation are.
@@ -682,7 +861,7 @@ void build_launches(tah_plan* p) {
       int64_t values = 0;
       for (size_t gi = 0; gi < p->groups.size(); ++gi) {
         const PlanGroup& g = p->groups[gi];
-        if (g.points == 0 || lds_of[gi][1] > 0) continue;
+        if (g.points == 0 || lds_of[gi][1] > 0 || str_of[gi][1] > 0) continue;
         if (g.cls == kClsInterp) {
           // A transfer's classes one after another, so that a merged
           // launch runs the tiles that share its windows together.
@@ -798,7 +977,7 @@ void build_launches(tah_plan* p) {
   }
   // A new state of the plan: batches built from an earlier one are stale.
   plan_register(p);
-  void* old[] = {p->dtiles, p->dchunks, p->dlds_tiles, p->duchunks, p->dwaves, p->dunits};
+  void* old[] = {p->dtiles, p->dchunks, p->dlds_tiles, p->duchunks, p->dwaves, p->dunits, p->dschunks, p->dplanes};
   if (std::any_of(std::begin(old), std::end(old), [](void* q) { return q != nullptr; })) {
     // a previous tile list may still be read by queued fills
     HIPCHECK(hipDeviceSynchronize());
@@ -810,6 +989,8 @@ void build_launches(tah_plan* p) {
     p->duchunks = nullptr;
     p->dwaves = nullptr;
     p->dunits = nullptr;
+    p->dschunks = nullptr;
+    p->dplanes = nullptr;
   }
   auto upload = [](auto*& dev, const auto& host) {
     if (host.empty()) return;
@@ -823,6 +1004,8 @@ void build_launches(tah_plan* p) {
   upload(p->duchunks, p->lds_uchunks);
   upload(p->dwaves, p->lds_waves);
   upload(p->dunits, p->lds_units);
+  upload(p->dschunks, p->stream_chunks);
+  upload(p->dplanes, p->stream_planes);
 }
 
 Job plan_job(const tah_plan* p, double* work, const double* factors, const double* src) {
@@ -834,6 +1017,8 @@ Job plan_job(const tah_plan* p, double* work, const double* factors, const doubl
   job.uchunks = p->duchunks;
   job.waves = p->dwaves;
   job.units = p->dunits;
+  job.schunks = p->dschunks;
+  job.planes = p->dplanes;
   job.work = work;
   job.src = src ? src : work;
   job.factors = factors;
@@ -903,7 +1088,7 @@ void families_from_options(tah_plan* p) {
     // single loads, the last variable alone) prolongation measured slower
     // than single variables (the 64-forest batch: 24.4 against 22.7 ms)
     p->fam_pairs[size_t(f)] = restriction || p->layout.nvars % 2 ? 0 : p->interp_pairs;
-    p->fam_lds[size_t(f)] = restriction ? (p->lds & 1) : (p->lds >> 1) & 1;
+    p->fam_lds[size_t(f)] = restriction ? ((p->lds & 4) ? 2 : (p->lds & 1)) : (p->lds >> 1) & 1;
     p->fam_vars[size_t(f)] = 0;
   }
 }
@@ -952,6 +1137,8 @@ tah_plan::~tah_plan() {
   if (duchunks) (void)hipFree(duchunks);
   if (dwaves) (void)hipFree(dwaves);
   if (dunits) (void)hipFree(dunits);
+  if (dschunks) (void)hipFree(dschunks);
+  if (dplanes) (void)hipFree(dplanes);
   if (djob) (void)hipFree(djob);
   if (djob2) (void)hipFree(djob2);
   if (dhooks) (void)hipFree(dhooks);

@@ -42,6 +42,7 @@ constexpr int kClsMerged = 1000;  // + index into kInterpTable: every rbits in o
 constexpr int kClsMergedPair = 1100;  // + index: the same over tiles of variable pairs
 constexpr int kClsLds = 1200;  // + index: LDS-staged chunks (interp_lds_kernel)
 constexpr int kClsLdsProlong = 1300;  // + index: LDS-staged prolongation by units
+constexpr int kClsStream = 1400;  // + index: restriction streamed along z (interp_stream_kernel)
 
 // (P, SHIFT) of each specialised interpolation kernel family.
 extern const std::array<std::array<int, 2>, 8> kInterpTable;
@@ -118,6 +119,11 @@ struct tah_plan {
   tah::LdsUnitChunk* duchunks = nullptr;
   tah::LdsWave* dwaves = nullptr;
   uint32_t* dunits = nullptr;
+  // ... of restriction streamed along z: chunks and their groups' planes
+  std::vector<tah::StreamChunk> stream_chunks;
+  std::vector<tah::StreamPlane> stream_planes;
+  tah::StreamChunk* dschunks = nullptr;
+  tah::StreamPlane* dplanes = nullptr;
   tah::Job* djob = nullptr;
   tah::Job last_job{};
   // A regrid transfer (stage tags TAH_STAGE_REGRID, then
@@ -148,12 +154,13 @@ struct tah_plan {
   // per thread where the layout has variables innermost and an even count
   // (1, the default), or of one variable per thread (0).
   int32_t interp_pairs = 1;
-  // Interpolation from LDS copies of its source boxes, chunk by chunk, where
-  // the layout holds variables innermost in even number: bit 0 restriction,
-  // bit 1 the prolongation of the units classes of at least 64 units (faces);
-  // the rest through the unit kernels. Off by default: on the large forest
-  // neither beat the unit kernels (the tuner tries them per mesh).
-  int32_t lds = 0;
+  // Interpolation from LDS copies of its source boxes. Bit 2 (the default):
+  // restriction streamed along z through LDS (interp_stream_kernel), in any
+  // layout. Bits 0 and 1, where the layout holds variables innermost in even
+  // number: restriction (without bit 2) and the prolongation of the unit
+  // classes of at least 64 units (faces) from chunks of their source boxes.
+  // The rest through the unit kernels.
+  int32_t lds = 4;
   // In a batched fill of plans that all have it set: one persistent launch
   // per stage position over a work queue of every plan's tiles (1), or a
   // launch per kernel class (0, the default: on the 64-forest batch the

@@ -339,9 +339,11 @@ class Plan:
 
     @property
     def options(self):
-        """The launch options: dict(fork=, merge=, defer=, fuse=, interp_pairs=,
-        lds=) each 0 or 1 (fuse always 0), copy_per_thread 4, 8 or 16,
-        copy_wgs 0 or a multiple of 8."""
+        """The launch options: dict(fork=, merge=, defer=, fuse=, interp_pairs=)
+        each 0 or 1 (fuse always 0), lds 0 to 7 (bit 2, the default:
+        restriction streamed along z; bits 0 and 1: restriction and
+        prolongation from LDS chunks), copy_per_thread 4, 8 or 16, copy_wgs 0
+        or a multiple of 8."""
         o = _abi.tah_plan_options()
         check(lib().tah_plan_get_options(self._h, C.byref(o)))
         return dict(fork=o.fork, merge=o.merge, defer=o.defer, copy_per_thread=o.copy_per_thread,
@@ -370,11 +372,12 @@ class Plan:
                      lds=e.lds, vars=e.vars) for e in arr[:n.value]]
 
     def set_family(self, order, restriction, **kw):
-        """Set one family's options (threads 64, 128 or 256; merge, pairs,
-        lds 0 or 1; vars 0 or a cap on a tile's variables); the others keep
-        theirs."""
+        """Set one family's options (threads 64, 128 or 256; merge, pairs 0 or
+        1; lds 0 or 1, or 2 for restriction streamed along z; vars 0 or a cap
+        on a tile's variables); the others keep theirs."""
         cur = next((f for f in self.families if f["order"] == order and f["restriction"] == int(restriction)),
-                   dict(order=order, restriction=int(restriction), threads=256, merge=1, pairs=0, lds=0, vars=0))
+                   dict(order=order, restriction=int(restriction), threads=256, merge=1, pairs=0,
+                        lds=2 if restriction else 0, vars=0))
         cur.update({k: int(v) for k, v in kw.items()})
         o = _abi.tah_family_options(int(order), int(restriction), cur["threads"], cur["merge"], cur["pairs"],
                                     cur["lds"], cur["vars"])
