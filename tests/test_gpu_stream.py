@@ -1,11 +1,11 @@
-"""Restriction streamed along z through LDS (plan option lds bit 2, family
-option lds 2: interp_stream_kernel) against the oracle, bit for bit: on a mesh
-of 32^3 blocks (faces, edges and corners of the large forest's shape, with a
-reflecting face: parity factors), in the arena layout 16-byte aligned (paired
-loads and stores) and 8 bytes off (single values), with an odd variable count
-and in the dense layout; on config 1 (Conservative restriction of order 2), on
-PointValue(4,4), and on forty random forests of the batched configuration in
-both layouts."""
+"""Interpolation streamed along z through LDS (plan option lds bit 2 for
+restriction, bit 3 for prolongation; family option lds 2: interp_stream_kernel)
+against the oracle, bit for bit: on a mesh of 32^3 blocks (faces, edges and
+corners of the large forest's shape, with a reflecting face: parity factors),
+in the arena layout 16-byte aligned (paired loads and stores) and 8 bytes off
+(single values), with an odd variable count and in the dense layout; on config
+1 (Conservative(3,2)), on PointValue(4,4), on vertex-centred meshes, and on
+forty random forests of the batched configuration in both layouts."""
 import pytest
 import torch
 
@@ -16,7 +16,7 @@ from treeamr_hip import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-STREAM = 4  # plan option lds: bit 2, restriction streamed along z
+STREAM = 12  # plan option lds: bits 2 and 3, restriction and prolongation streamed along z
 
 
 def _oracle(fs, sched):
@@ -63,9 +63,11 @@ def test_stream_on_32_cubed_blocks(layout, nvars, offset):
     print(f"[stream] 32^3 blocks layout={layout} nvars={nv} offset={offset} blocks={f.nleaves}", flush=True)
     sched.plan(fs.layout).set_options(lds=STREAM)
     names = _names(fs, sched)
-    assert "interp_stream_kernel<6, 4>" in names
-    assert not any(n.startswith("interp_all_kernel<6, 2>") for n in names)
+    assert "interp_stream_kernel<6, 4>" in names and "interp_stream_kernel<6, 12>" in names
+    assert not any(n.startswith("interp_all_kernel") or n.startswith("interp_pair_kernel") for n in names)
     assert _mismatches(fs, sched) == 0
+    for lds in (4, 8):
+        assert _mismatches(fs, sched, lds=lds) == 0, lds
 
 
 @pytest.mark.parametrize("layout", ["arena", "dense"])
@@ -76,7 +78,8 @@ def test_stream_conservative_order_2(layout):
     sched = GhostSchedule(fs, ops)
     print(f"[stream] config 1 layout={layout}", flush=True)
     sched.plan(fs.layout).set_options(lds=STREAM)
-    assert "interp_stream_kernel<2, 2>" in _names(fs, sched)
+    names = _names(fs, sched)
+    assert "interp_stream_kernel<2, 2>" in names and "interp_stream_kernel<3, 6>" in names
     assert _mismatches(fs, sched) == 0
 
 
@@ -90,7 +93,8 @@ def test_stream_order_4():
     sched = GhostSchedule(fs, Operators(prolongation=4, restriction=4))
     print(f"[stream] PointValue(4,4) blocks={f.nleaves}", flush=True)
     sched.plan(fs.layout).set_options(lds=STREAM)
-    assert "interp_stream_kernel<4, 3>" in _names(fs, sched)
+    names = _names(fs, sched)
+    assert "interp_stream_kernel<4, 3>" in names and "interp_stream_kernel<4, 8>" in names
     assert _mismatches(fs, sched) == 0
 
 
@@ -104,12 +108,33 @@ def test_stream_family_option_and_refusal():
     plan = sched.plan(fs.layout)
     plan.set_options(lds=0)
     plan.set_family(6, True, lds=2)
-    assert "interp_stream_kernel<6, 4>" in _names(fs, sched)
+    names = _names(fs, sched)
+    assert "interp_stream_kernel<6, 4>" in names and "interp_stream_kernel<6, 12>" not in names
     assert _mismatches(fs, sched, lds=None) == 0
+    plan.set_family(6, False, lds=2)
+    assert "interp_stream_kernel<6, 12>" in _names(fs, sched)
+    assert _mismatches(fs, sched, lds=None) == 0
+    before = plan.families
     with pytest.raises(Exception):
-        plan.set_family(6, False, lds=2)  # prolongation has no streamed form
+        plan.set_family(6, False, lds=3)
+    assert plan.families == before
     with pytest.raises(Exception):
-        plan.set_options(lds=8)
+        plan.set_options(lds=16)
+
+
+@pytest.mark.parametrize("layout,offset", [("arena", 0), ("arena", 1), ("dense", 0)])
+def test_stream_vertex_centred(layout, offset):
+    f, ops, kw, _ = W.config2(nvars=8, roots=4, N=16, radii=(0.35, 0.2))
+    fs = FieldSet(f, kw["nvars"], G=kw["G"], centering=kw["centering"], parity=kw["parity"], device="cuda",
+                  layout=layout, fill=float("nan"))
+    W.fill_waves(fs, 1)
+    if offset:
+        _unaligned(fs)
+    sched = GhostSchedule(fs, ops)
+    print(f"[stream] vertex layout={layout} offset={offset} blocks={f.nleaves}", flush=True)
+    sched.plan(fs.layout).set_options(lds=STREAM)
+    assert "interp_stream_kernel<6, 12>" in _names(fs, sched)
+    assert _mismatches(fs, sched) == 0
 
 
 @pytest.mark.parametrize("layout", ["arena", "dense"])

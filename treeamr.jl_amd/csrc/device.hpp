@@ -126,12 +126,15 @@ constexpr int kLdsMaxBytes = 64 * 1024;
 // The staging slots of a streamed chunk's plane per thread: a plane holds at
 // most kStreamSlots * kThreads pairs of variables.
 constexpr int kStreamSlots = 8;
-// The most z windows of a streamed restriction of order P that may hold one
-// plane (the kernel's accumulators per thread): P/2 where windows start two
-// rows apart, one more where the first two windows share a start (clamped
-// into the fine block's interior); kStreamKA for every order here.
-constexpr int stream_ka(int P) { return P / 2 + 1; }
-constexpr int kStreamKA = 4;
+// The most z windows of a streamed interpolation of order P that may hold
+// one plane (the kernel's accumulators per thread). Restriction (shift 2):
+// P/2 where windows start two rows apart, one more where the first two
+// windows share a start (clamped into the fine block's interior).
+// Prolongation (shift 0): the two fine rows of a coarse row share its
+// window, and windows start one coarse row apart: 2P. kStreamKA bounds them
+// all.
+constexpr int stream_ka(int P, int shift) { return shift != 0 ? P / 2 + 1 : 2 * P; }
+constexpr int kStreamKA = 12;
 
 // One source plane of a streamed restriction's group, the planes zlo ..
 // zlo + nplanes - 1 in ascending order: the z rows whose windows hold it are

@@ -122,9 +122,14 @@ void launch_lds(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const 
   const size_t sh = size_t(bytes);
   if (cls >= kClsStream) {
     switch (cls - kClsStream) {
-      case 5: hipLaunchKernelGGL((interp_stream_kernel<2, stream_ka(2)>), grid, blk, sh, s, jobs, tiles); break;
-      case 6: hipLaunchKernelGGL((interp_stream_kernel<4, stream_ka(4)>), grid, blk, sh, s, jobs, tiles); break;
-      case 7: hipLaunchKernelGGL((interp_stream_kernel<6, stream_ka(6)>), grid, blk, sh, s, jobs, tiles); break;
+      case 0: hipLaunchKernelGGL((interp_stream_kernel<2, stream_ka(2, 0)>), grid, blk, sh, s, jobs, tiles); break;
+      case 1: hipLaunchKernelGGL((interp_stream_kernel<3, stream_ka(3, 0)>), grid, blk, sh, s, jobs, tiles); break;
+      case 2: hipLaunchKernelGGL((interp_stream_kernel<4, stream_ka(4, 0)>), grid, blk, sh, s, jobs, tiles); break;
+      case 3: hipLaunchKernelGGL((interp_stream_kernel<5, stream_ka(5, 0)>), grid, blk, sh, s, jobs, tiles); break;
+      case 4: hipLaunchKernelGGL((interp_stream_kernel<6, stream_ka(6, 0)>), grid, blk, sh, s, jobs, tiles); break;
+      case 5: hipLaunchKernelGGL((interp_stream_kernel<2, stream_ka(2, 2)>), grid, blk, sh, s, jobs, tiles); break;
+      case 6: hipLaunchKernelGGL((interp_stream_kernel<4, stream_ka(4, 2)>), grid, blk, sh, s, jobs, tiles); break;
+      case 7: hipLaunchKernelGGL((interp_stream_kernel<6, stream_ka(6, 2)>), grid, blk, sh, s, jobs, tiles); break;
       default: fail("no streamed restriction kernel for class " + std::to_string(cls));
     }
     HIPCHECK(hipGetLastError());
@@ -633,8 +638,8 @@ tah_status tah_kernel_class_name(int32_t cls, char* buf, int32_t len) {
       const auto& e = kInterpTable[cls - kClsMergedPair];
       name = "interp_pair_kernel<" + std::to_string(e[0]) + ", " + std::to_string(e[1]) + ">";
     } else if (cls >= kClsStream && cls - kClsStream < int(kInterpTable.size())) {
-      name = "interp_stream_kernel<" + std::to_string(kInterpTable[cls - kClsStream][0]) + ", " +
-             std::to_string(stream_ka(kInterpTable[cls - kClsStream][0])) + ">";
+      const auto& e = kInterpTable[cls - kClsStream];
+      name = "interp_stream_kernel<" + std::to_string(e[0]) + ", " + std::to_string(stream_ka(e[0], e[1])) + ">";
     } else if (cls >= kClsLdsProlong && cls - kClsLdsProlong < int(kInterpTable.size())) {
       name = "interp_lds_prolong_kernel<" + std::to_string(kInterpTable[cls - kClsLdsProlong][0]) + ">";
     } else if (cls >= kClsLds && cls - kClsLds < int(kInterpTable.size())) {

@@ -189,11 +189,11 @@ bool lds_unit_chunks_of(const tah_plan* p, const PlanGroup& g, int32_t gi, const
   return true;
 }
 
-// The plane records of restriction group g streamed along z (StreamPlane):
+// The plane records of interpolating group g streamed along z (StreamPlane):
 // its z rows sorted by window start (stable) enter as their windows begin
 // and leave, stored, as they end. Appends to planes; *zlo is the first
 // plane's source row. False, with nothing appended, where more than
-// stream_ka(P) windows hold one plane.
+// stream_ka(P, shift) windows hold one plane.
 bool stream_planes_of(const PlanGroup& g, std::vector<StreamPlane>& planes, int32_t* zlo) {
   const int32_t n2 = g.n[2], P = g.order[2];
   std::vector<int32_t> order(static_cast<size_t>(n2));
@@ -209,7 +209,7 @@ bool stream_planes_of(const PlanGroup& g, std::vector<StreamPlane>& planes, int3
     pl.had = oh - ol + 1;
     while (oh + 1 < n2 && start(oh + 1) <= z) ++oh;
     pl.nact = oh - ol + 1;
-    if (pl.nact > stream_ka(P) || pl.nact > kStreamKA) {
+    if (pl.nact > stream_ka(P, g.kind == 2 ? 2 : 0) || pl.nact > kStreamKA) {
       planes.resize(first);
       return false;
     }
@@ -227,7 +227,7 @@ bool stream_planes_of(const PlanGroup& g, std::vector<StreamPlane>& planes, int3
   return true;
 }
 
-// The chunks of restriction group g (index gi) streamed along z, for tiles
+// The chunks of interpolating group g (index gi) streamed along z, for tiles
 // of nvt variables: the target rows along x and along y cut into balanced
 // runs, so that a chunk's targets, each taking one thread per pair of
 // variables, fill at most a workgroup, and its plane (the union of its
@@ -694,16 +694,16 @@ void build_launches(tah_plan* p) {
       }
     }
   auto lds_launch = [&](const PlanGroup& g) { return kClsLds + interp_index(g.order[0], g.kind == 2 ? 2 : 0); };
-  // Restriction streamed along z: the groups whose z windows overlap at
-  // most stream_ka(P) to a plane and whose boxes cut into chunks that fit,
-  // in any layout; their chunks and sorted z rows.
+  // Interpolation streamed along z (family option lds 2): the groups whose z
+  // windows overlap at most stream_ka(P, shift) to a plane and whose boxes
+  // cut into chunks that fit, in any layout; their chunks and plane records.
   p->stream_chunks.clear();
   p->stream_planes.clear();
   std::vector<std::array<int32_t, 2>> str_of(p->groups.size(), {0, 0});  // first chunk, count
   const int32_t stream_nvt = std::min<int32_t>(nvars, 2 * kThreads);
   for (size_t gi = 0; gi < p->groups.size(); ++gi) {
     const PlanGroup& g = p->groups[gi];
-    if (g.cls != kClsInterp || g.kind != 2 || fam_lds(g) != 2 || g.n[2] < 1) continue;
+    if (g.cls != kClsInterp || fam_lds(g) != 2 || g.n[2] < 1) continue;
     const int32_t plane0 = int32_t(p->stream_planes.size());
     int32_t zlo = 0;
     if (!stream_planes_of(g, p->stream_planes, &zlo)) continue;
@@ -716,7 +716,7 @@ void build_launches(tah_plan* p) {
     str_of[gi] = {int32_t(p->stream_chunks.size()), int32_t(cs.size())};
     p->stream_chunks.insert(p->stream_chunks.end(), cs.begin(), cs.end());
   }
-  auto stream_launch = [&](const PlanGroup& g) { return kClsStream + interp_index(g.order[0], 2); };
+  auto stream_launch = [&](const PlanGroup& g) { return kClsStream + family_of(g); };
   // LDS-staged prolongation by units: the unit classes of at least 64 units
   // (a face's inner units); a group's other classes stay with the unit
   // kernels.
@@ -1088,7 +1088,7 @@ void families_from_options(tah_plan* p) {
     // single loads, the last variable alone) prolongation measured slower
     // than single variables (the 64-forest batch: 24.4 against 22.7 ms)
     p->fam_pairs[size_t(f)] = restriction || p->layout.nvars % 2 ? 0 : p->interp_pairs;
-    p->fam_lds[size_t(f)] = restriction ? ((p->lds & 4) ? 2 : (p->lds & 1)) : (p->lds >> 1) & 1;
+    p->fam_lds[size_t(f)] = restriction ? ((p->lds & 4) ? 2 : (p->lds & 1)) : ((p->lds & 8) ? 2 : (p->lds >> 1) & 1);
     p->fam_vars[size_t(f)] = 0;
   }
 }

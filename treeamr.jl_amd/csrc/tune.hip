@@ -85,8 +85,9 @@ void set_options(tah_plan* p, const tah_plan_options& o) {
   const int32_t wgs = o.copy_wgs;
   if (wgs < 0 || wgs % 8 != 0) fail("copy_wgs is 0 or a positive multiple of 8");
   if (o.interp_pairs < 0 || o.interp_pairs > 1) fail("plan option interp_pairs is 0 or 1");
-  if (o.lds < 0 || o.lds > 7)
-    fail("plan option lds is 0 to 7 (bit 0 restriction, bit 1 prolongation, bit 2 restriction streamed along z)");
+  if (o.lds < 0 || o.lds > 15)
+    fail("plan option lds is 0 to 15 (bits 0 and 1: restriction and prolongation from LDS chunks; bits 2 and "
+         "3: restriction and prolongation streamed along z)");
   if (o.queue < 0 || o.queue > 1) fail("plan option queue is 0 or 1");
   if (o.reserved_[0] || o.reserved_[1] || o.reserved_[2]) fail("tah_plan_options.reserved_ must be zero");
   bool rebuild = o.merge != p->merge || o.defer != p->defer || o.copy_per_thread != p->copy_per_thread ||
@@ -152,8 +153,8 @@ void set_family(tah_plan* p, const tah_family_options& o) {
     fail("no interpolation family of order " + std::to_string(o.order) +
          (o.restriction ? " restricts" : " prolongates") + " here");
   if (o.threads != 64 && o.threads != 128 && o.threads != 256) fail("a family's workgroup size is 64, 128 or 256");
-  if (o.merge < 0 || o.merge > 1 || o.pairs < 0 || o.pairs > 1 || o.lds < 0 || o.lds > (o.restriction ? 2 : 1))
-    fail("family options merge and pairs are 0 or 1; lds is 0 or 1, or 2 for restriction (streamed along z)");
+  if (o.merge < 0 || o.merge > 1 || o.pairs < 0 || o.pairs > 1 || o.lds < 0 || o.lds > 2)
+    fail("family options merge and pairs are 0 or 1; lds is 0, 1 or 2 (streamed along z)");
   if (o.restriction && o.pairs) fail("variable pairs are for prolongation (pairs must be 0 for restriction)");
   if (o.vars < 0 || o.vars > o.threads) fail("a family's variables per tile are 0 (every one) to its threads");
   const size_t i = size_t(f);
@@ -229,8 +230,7 @@ std::vector<std::vector<int32_t>> family_knobs(const tah_plan* p, int f) {
     for (int v : {8, 4, 2, 1})
       if (v < p->layout.nvars) vars.push_back(v);
   return {{256, 128, 64}, vars, {1, 0}, (!restriction && pairs_ok) ? std::vector<int32_t>{1, 0} : std::vector<int32_t>{0},
-          restriction ? (lds_ok ? std::vector<int32_t>{2, 0, 1} : std::vector<int32_t>{2, 0})
-                      : (lds_ok ? std::vector<int32_t>{0, 1} : std::vector<int32_t>{0})};
+          lds_ok ? std::vector<int32_t>{2, 0, 1} : std::vector<int32_t>{2, 0}};
 }
 
 int32_t& knob(tah_family_options& o, int k) {
@@ -406,7 +406,7 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
         base.threads = kThreads;
         base.merge = p->merge;
         base.pairs = base.restriction || p->layout.nvars % 2 ? 0 : p->interp_pairs;
-        base.lds = base.restriction ? ((p->lds & 4) ? 2 : (p->lds & 1)) : (p->lds >> 1) & 1;
+        base.lds = base.restriction ? ((p->lds & 4) ? 2 : (p->lds & 1)) : ((p->lds & 8) ? 2 : (p->lds >> 1) & 1);
         base.vars = 0;
         if (std::memcmp(&chosen, &base, sizeof base) == 0) continue;
         float tc = 1e30f, tb = 1e30f;
