@@ -125,8 +125,8 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
   };
   for (size_t pos = 0; pos < npos; ++pos) {
     std::map<std::pair<int, int32_t>, std::vector<Tile>> bycls;  // by (class, workgroup size)
-    std::map<int, std::vector<LdsTile>> ldscls;  // LDS-staged classes
-    std::map<int, int32_t> ldsbytes;
+    std::map<std::pair<int, int32_t>, std::vector<LdsTile>> ldscls;  // LDS-staged, by (class, workgroup size)
+    std::map<std::pair<int, int32_t>, int32_t> ldsbytes;
     for (int32_t i = 0; i < n; ++i) {
       tah_plan* p = b->plans[i];
       if (pos >= p->launches.size()) continue;
@@ -136,9 +136,9 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
           for (int64_t t = L.first; t < L.first + L.count; ++t) {
             LdsTile tl = p->lds_tiles[size_t(t)];
             tl.job = i;
-            ldscls[L.cls].push_back(tl);
+            ldscls[{L.cls, L.threads}].push_back(tl);
           }
-          ldsbytes[L.cls] = std::max(ldsbytes[L.cls], L.lds);
+          ldsbytes[{L.cls, L.threads}] = std::max(ldsbytes[{L.cls, L.threads}], L.lds);
           continue;
         }
         for (int64_t t = L.first; t < L.first + L.count; ++t) {
@@ -180,7 +180,8 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
       tiles.insert(tiles.end(), kv.second.begin(), kv.second.end());
     }
     for (auto& kv : ldscls) {
-      ls.push_back(Launch{kv.first, int64_t(ltiles.size()), int64_t(kv.second.size()), 0, 0, ldsbytes[kv.first]});
+      ls.push_back(Launch{kv.first.first, int64_t(ltiles.size()), int64_t(kv.second.size()), 0, 0, ldsbytes[kv.first],
+                          kv.first.second});
       ltiles.insert(ltiles.end(), kv.second.begin(), kv.second.end());
     }
     std::stable_sort(ls.begin(), ls.end(), [](const Launch& x, const Launch& y) { return x.count > y.count; });

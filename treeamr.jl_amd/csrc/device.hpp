@@ -124,7 +124,7 @@ struct alignas(16) LdsUnitChunk {
 constexpr int kLdsMaxBytes = 64 * 1024;
 
 // The staging slots of a streamed chunk's plane per thread: a plane holds at
-// most kStreamSlots * kThreads pairs of variables.
+// most kStreamSlots pairs of variables per thread of its workgroup.
 constexpr int kStreamSlots = 8;
 // The most z windows of a streamed interpolation of order P that may hold
 // one plane (the kernel's accumulators per thread). Restriction (shift 2):
@@ -162,7 +162,8 @@ struct alignas(16) StreamChunk {
   int32_t slo[2], sext[2];
   int32_t plane0, nplanes, zlo;
   int32_t nvt;
-  int32_t pad[3];
+  int32_t threads;  // the workgroup size its launch takes: 64, 128 or 256
+  int32_t pad[2];
 };
 
 // One workgroup's share of an LDS-staged launch: chunk `chunk` (of the job's

@@ -116,11 +116,14 @@ void launch_class(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, cons
   HIPCHECK(hipGetLastError());
 }
 
-void launch_lds(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const LdsTile* tiles, int32_t bytes) {
+void launch_lds(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const LdsTile* tiles, int32_t bytes,
+                int32_t threads) {
   if (bytes <= 0 || bytes > kLdsMaxBytes) fail("bad LDS size for an LDS-staged launch");
   const dim3 grid(ntiles), blk(kThreads);
   const size_t sh = size_t(bytes);
   if (cls >= kClsStream) {
+    if (threads != 64 && threads != 128 && threads != kThreads) fail("bad workgroup size for a streamed launch");
+    const dim3 blk(threads);
     switch (cls - kClsStream) {
       case 0: hipLaunchKernelGGL((interp_stream_kernel<2, stream_ka(2, 0)>), grid, blk, sh, s, jobs, tiles); break;
       case 1: hipLaunchKernelGGL((interp_stream_kernel<3, stream_ka(3, 0)>), grid, blk, sh, s, jobs, tiles); break;

@@ -1002,11 +1002,12 @@ __device__ __forceinline__ void stream_body(const Job& job, const LdsTile& tile,
   const int32_t L = (nv + 1) >> 1;  // threads per target: one per pair of variables
   const int32_t nvp = 2 * L;        // doubles per staged point
   const int32_t e0 = ch.sext[0], e1 = ch.sext[1];
-  const int32_t t = int32_t(threadIdx.x);
+  const int32_t t = int32_t(threadIdx.x), T = int32_t(blockDim.x);
   const bool pairs = job.pairs != 0;
 
   // The plane's pairs, numbered with a point's pairs fastest where variables
-  // are innermost, else x fastest; thread t stages pairs t, t + kThreads, ...
+  // are innermost, else x fastest; thread t stages pairs t, t + T, ... (T
+  // the workgroup's threads)
   // (at most kStreamSlots), pair vp of point (x, y) at LDS double
   // (y*e0 + x)*nvp + 2*vp -- in the numbering's own order where variables
   // are innermost. Odd counts: the last pair's second value is staged as
@@ -1039,7 +1040,7 @@ __device__ __forceinline__ void stream_body(const Job& job, const LdsTile& tile,
     asm volatile("" : "+v"(tt));
 #pragma unroll
     for (int s = 0; s < kStreamSlots; ++s) {
-      const int32_t q = tt + s * kThreads;
+      const int32_t q = tt + s * T;
       at[s] = -1;
       if (q >= npairs) continue;
       int32_t go, vp;
@@ -1064,9 +1065,9 @@ __device__ __forceinline__ void stream_body(const Job& job, const LdsTile& tile,
     const int32_t wave = tt & ~63;
 #pragma unroll
     for (int s = 0; s < kStreamSlots; ++s) {
-      const int32_t q0 = wave + s * kThreads;  // the wave's first pair: uniform
+      const int32_t q0 = wave + s * T;  // the wave's first pair: uniform
       if (q0 >= npairs) break;
-      const int32_t q = tt + s * kThreads;
+      const int32_t q = tt + s * T;
       if (q < npairs) {
         int32_t go, at, vp;
         slot_of(q, go, at, vp);

@@ -306,7 +306,14 @@ std::vector<StreamChunk> stream_chunks_of(const PlanGroup& g, int32_t gi, int32_
           const int32_t o = g.srcstart[d][size_t(r)] - ch.slo[d];
           ok = ok && o >= 0 && o + P <= ch.sext[d];
         }
-      if (!ok) fail("internal: a streamed restriction chunk does not fit its kernel");
+      if (!ok) fail("internal: a streamed interpolation chunk does not fit its kernel");
+      // the smallest workgroup its threads and plane fit (chunks of few
+      // targets, such as edges along z and corners, keep their threads busy)
+      ch.threads = kThreads;
+      for (int32_t T : {64, 128})
+        if (ch.threads == kThreads && int64_t(ch.tn[0]) * ch.tn[1] * L <= T &&
+            int64_t(ch.sext[0]) * ch.sext[1] * L <= int64_t(kStreamSlots) * T)
+          ch.threads = T;
       out.push_back(ch);
     }
   return out;
@@ -765,36 +772,43 @@ void build_launches(tah_plan* p) {
       int64_t values;
       std::vector<std::pair<int32_t, LdsTile>> ltiles;  // LDS-staged: (source block, tile)
       int32_t lds_bytes = 0;
+      int32_t threads = kThreads;  // LDS-staged: the workgroup size
     };
     std::vector<Built> built;
     for (int cls : classes) {
       if (cls >= kClsStream) {
         // A transfer's chunks one after another, each for every tile of
-        // variables in turn, ordered by source block as the unit tiles of
-        // restriction are.
-        Built B{cls, {}, 0, {}, 0};
-        for (size_t gi = 0; gi < p->groups.size(); ++gi) {
-          const PlanGroup& g = p->groups[gi];
-          if (g.points == 0 || str_of[gi][1] == 0 || stream_launch(g) != cls) continue;
-          for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t) {
-            if (run_in[gi][t] != k) continue;
-            for (int32_t c = str_of[gi][0]; c < str_of[gi][0] + str_of[gi][1]; ++c) {
-              const StreamChunk& ch = p->stream_chunks[size_t(c)];
-              for (int32_t v0 = 0; v0 < nvars; v0 += ch.nvt) {
-                const int32_t nv = std::min(ch.nvt, nvars - v0);
-                // two plane buffers
-                B.lds_bytes = std::max(B.lds_bytes, int32_t(2 * ch.sext[0] * ch.sext[1] * 2 * ((nv + 1) / 2) * 8));
-                B.ltiles.push_back({g.sblocks[t], LdsTile{0, c, t, v0}});
-                B.values += int64_t(ch.tn[0]) * ch.tn[1] * g.n[2] * nv;
+        // variables in turn, ordered by source block (restriction) or
+        // target block (prolongation) as the unit tiles are; one launch
+        // per workgroup size.
+        for (int32_t T : {kThreads, 128, 64}) {
+          Built B{cls, {}, 0, {}, 0};
+          B.threads = T;
+          for (size_t gi = 0; gi < p->groups.size(); ++gi) {
+            const PlanGroup& g = p->groups[gi];
+            if (g.points == 0 || str_of[gi][1] == 0 || stream_launch(g) != cls) continue;
+            for (int32_t t = 0; t < int32_t(g.tblocks.size()); ++t) {
+              if (run_in[gi][t] != k) continue;
+              for (int32_t c = str_of[gi][0]; c < str_of[gi][0] + str_of[gi][1]; ++c) {
+                const StreamChunk& ch = p->stream_chunks[size_t(c)];
+                if (ch.threads != T) continue;
+                for (int32_t v0 = 0; v0 < nvars; v0 += ch.nvt) {
+                  const int32_t nv = std::min(ch.nvt, nvars - v0);
+                  // two plane buffers
+                  B.lds_bytes =
+                      std::max(B.lds_bytes, int32_t(2 * ch.sext[0] * ch.sext[1] * 2 * ((nv + 1) / 2) * 8));
+                  B.ltiles.push_back({g.kind == 2 ? g.sblocks[t] : g.tblocks[t], LdsTile{0, c, t, v0}});
+                  B.values += int64_t(ch.tn[0]) * ch.tn[1] * g.n[2] * nv;
+                }
               }
             }
           }
+          std::stable_sort(B.ltiles.begin(), B.ltiles.end(),
+                           [](const std::pair<int32_t, LdsTile>& x, const std::pair<int32_t, LdsTile>& y) {
+                             return x.first < y.first;
+                           });
+          if (!B.ltiles.empty()) built.push_back(std::move(B));
         }
-        std::stable_sort(B.ltiles.begin(), B.ltiles.end(),
-                         [](const std::pair<int32_t, LdsTile>& x, const std::pair<int32_t, LdsTile>& y) {
-                           return x.first < y.first;
-                         });
-        if (!B.ltiles.empty()) built.push_back(std::move(B));
         continue;
       }
       if (cls >= kClsLdsProlong) {
@@ -952,7 +966,7 @@ void build_launches(tah_plan* p) {
     }
     for (Built& B : built) {
       if (!B.ltiles.empty()) {
-        Launch L{B.cls, int64_t(p->lds_tiles.size()), int64_t(B.ltiles.size()), B.values, 0, B.lds_bytes};
+        Launch L{B.cls, int64_t(p->lds_tiles.size()), int64_t(B.ltiles.size()), B.values, 0, B.lds_bytes, B.threads};
         for (const auto& e : B.ltiles) p->lds_tiles.push_back(e.second);
         ls.push_back(L);
         continue;
@@ -1075,7 +1089,7 @@ const std::vector<hipStream_t>& ctx_streams(tah_ctx* c) {
 }
 
 void launch_one(const Launch& L, hipStream_t s, const Job* jobs, const Tile* tiles, const LdsTile* ltiles) {
-  if (L.lds > 0) launch_lds(L.cls, uint32_t(L.count), s, jobs, ltiles + L.first, L.lds);
+  if (L.lds > 0) launch_lds(L.cls, uint32_t(L.count), s, jobs, ltiles + L.first, L.lds, L.threads);
   else launch_class(L.cls, uint32_t(L.count), s, jobs, tiles + L.first, L.grid, L.threads);
 }
 

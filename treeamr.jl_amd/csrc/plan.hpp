@@ -76,7 +76,7 @@ struct Launch {
   int64_t values = 0;  // values the launch writes
   int32_t grid = 0;    // copies: workgroups of a bounded grid (0: one per tile)
   int32_t lds = 0;     // LDS-staged: bytes of LDS per workgroup; first/count index LdsTiles
-  int32_t threads = kThreads;  // workgroup size (interpolation families: 64, 128 or 256)
+  int32_t threads = kThreads;  // workgroup size (interpolation families and streamed chunks: 64, 128 or 256)
 };
 
 // The interpolation families: the index into kInterpTable of order P and
@@ -235,8 +235,10 @@ void launch_queue(uint32_t grid, hipStream_t s, const Job* jobs, const Tile* til
                   uint32_t* counters);
 void launch_queue_reset(hipStream_t s, uint32_t* counters, uint32_t n);
 int queue_waves_per_cu();
-// An LDS-staged class over ntiles tiles with `bytes` of LDS per workgroup (fill.hip).
-void launch_lds(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const LdsTile* tiles, int32_t bytes);
+// An LDS-staged class over ntiles tiles with `bytes` of LDS per workgroup
+// (fill.hip); threads: the workgroup size of a streamed class.
+void launch_lds(int cls, uint32_t ntiles, hipStream_t s, const Job* jobs, const LdsTile* tiles, int32_t bytes,
+                int32_t threads = kThreads);
 // Launch one Launch (copy, interpolation or LDS-staged) on stream s.
 void launch_one(const Launch& L, hipStream_t s, const Job* jobs, const Tile* tiles, const LdsTile* ltiles);
 // Launch one class over ntiles tiles (fill.hip); grid > 0: a bounded grid.
