@@ -410,9 +410,10 @@ typedef struct tah_plan_options {
   int32_t fuse;            /* must be 0: the fused copy-and-interpolation launch is switched off */
   int32_t copy_wgs;        /* copies from a bounded grid of this many workgroups (a multiple of 8), or 0 */
   int32_t interp_pairs;    /* merged interpolations over variable pairs where the layout allows (1, default) */
-  int32_t lds;             /* interpolation from LDS copies of its source boxes: bit 2 restriction streamed
-                              along z through LDS, every layout (4, the default); else, where the layout
-                              holds variables innermost in even number, bit 0 restriction and bit 1
+  int32_t lds;             /* interpolation from LDS copies of its source boxes: bit 2 restriction and bit 3
+                              prolongation streamed along z through LDS, every layout (default 4 where the
+                              layout holds variables innermost, 12 otherwise); else, where the layout holds
+                              variables innermost in even number, bit 0 restriction and bit 1
                               prolongation's unit classes of 64 units or more chunk by chunk; unit kernels
                               for the rest */
   int32_t queue;           /* in a batched fill of plans that all set it: one persistent launch per stage

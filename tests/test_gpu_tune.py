@@ -34,7 +34,7 @@ def test_every_option_combination_fills_alike(which):
     ref = fs.work.clone()
     plan = s.plan(fs.layout)
     for fork, merge, defer, cpt, wgs, ip, lds in itertools.product((0, 1), (0, 1), (0, 1), (4, 8, 16),
-                                                                   (0, 64, 512), (0, 1), (0, 1, 2, 3, 4, 6)):
+                                                                   (0, 64, 512), (0, 1), (0, 1, 2, 3, 4, 6, 8, 12)):
         print(f"[tune] {which} fork={fork} merge={merge} defer={defer} copy_per_thread={cpt} copy_wgs={wgs} "
               f"interp_pairs={ip} lds={lds}", flush=True)
         plan.set_options(fork=fork, merge=merge, fuse=0, defer=defer, copy_per_thread=cpt, copy_wgs=wgs,
@@ -153,7 +153,7 @@ def test_option_values_refused():
     plan = s.plan(fs.layout)
     before = plan.options
     for bad in (dict(interp_pairs=2), dict(interp_pairs=-1), dict(copy_per_thread=5), dict(copy_wgs=12),
-                dict(fork=2), dict(lds=8), dict(lds=-1), dict(queue=2), dict(queue=-1)):
+                dict(fork=2), dict(lds=16), dict(lds=-1), dict(queue=2), dict(queue=-1)):
         with pytest.raises(ValueError):
             plan.set_options(**bad)
         assert plan.options == before, bad

@@ -156,10 +156,11 @@ struct tah_plan {
   int32_t interp_pairs = 1;
   // Interpolation from LDS copies of its source boxes. Bit 2 (the default):
   // restriction streamed along z through LDS (interp_stream_kernel), in any
-  // layout. Bits 0 and 1, where the layout holds variables innermost in even
-  // number: restriction (without bit 2) and the prolongation of the unit
-  // classes of at least 64 units (faces) from chunks of their source boxes.
-  // The rest through the unit kernels.
+  // layout; bit 3 (the default where variables are not innermost):
+  // prolongation streamed the same way. Bits 0 and 1, where the layout holds
+  // variables innermost in even number: restriction (without bit 2) and the
+  // prolongation of the unit classes of at least 64 units (faces) from
+  // chunks of their source boxes. The rest through the unit kernels.
   int32_t lds = 4;
   // In a batched fill of plans that all have it set: one persistent launch
   // per stage position over a work queue of every plan's tiles (1), or a
