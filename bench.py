@@ -170,7 +170,7 @@ def main():
     from treeamr_hip import _abi
     from treeamr_hip import (FieldSet, FillGraph, GhostSchedule, fill_ghosts, fill_ghosts_adjoint,
                              fill_ghosts_adjoint_timed, fill_ghosts_timed, gather, ownextents, regrid_transfer,
-                             scatter, statelength, tune, Operators, RegridSchedule, unify)
+                             scatter, statelength, tune, tune_batched, Operators, RegridSchedule, unify)
     from treeamr_hip import workloads as W
 
     def run_batched(stream):
@@ -190,6 +190,13 @@ def main():
 
         bplain = time_fill(seq_plain, args.batched_steps, 1, stream)
         log(f"[bench] batch, 64 sequential plain fills: {bplain}")
+        btune = None
+        if not args.no_tune:
+            # the batch's launch options, picked by the library's batch tuner
+            # before the timed region (as a warm-up would)
+            choice, tms = tune_batched(pairs, None)
+            btune = {"options": choice, "tuner_median_ms": round(tms, 4) if tms else None}
+            log(f"[bench] batch tuner: {btune}")
         bgraph_obj = FillGraph(pairs)
         bgraph = time_fill(bgraph_obj.launch, args.batched_steps * 3, 1, stream)
         log(f"[bench] batch, graph-captured batched fill: {bgraph}")
@@ -199,6 +206,7 @@ def main():
                         "<=2 GiB each)",
             "field_sets": len(pairs), "blocks": bblocks, "values_per_fill": bvalues,
             "queue": args.batch_queue,
+            "tuned": btune,
             "plain_sequential_ms": round(bplain["mean_ms"], 3),
             "graph_batched_ms": round(bgraph["mean_ms"], 3),
             "speedup": round(bplain["mean_ms"] / bgraph["mean_ms"], 2),

@@ -466,6 +466,15 @@ tah_status tah_plan_set_family_options(tah_plan* p, const tah_family_options* o)
  * options' median fill time. */
 tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char* cache_path,
                     double* best_ms);
+/* The autotuner of a batched fill (no counterpart in the reference, which
+ * fills one field set at a time): times tah_fill_batched over the batch's
+ * arrays for every combination of the plan-wide options fork, defer and
+ * copy_per_thread, the same on every plan (the other options stay each
+ * plan's own), and sets the fastest on every plan. Cached as tah_tune's
+ * choice is, under a signature of every plan of the batch; best_ms as
+ * tah_tune's. */
+tah_status tah_tune_batched(tah_ctx* ctx, int32_t n, tah_plan* const* plans, double* const* works,
+                            const double* const* factors, const char* cache_path, double* best_ms);
 
 /* --- The state vector (src/state.jl) -------------------------------- */
 

@@ -146,3 +146,40 @@ def test_one_plan_two_streams():
         torch.cuda.synchronize()
         assert torch.equal(a.work.view(torch.int64), ra.view(torch.int64)), f"rep {rep}: first array"
         assert torch.equal(b.work.view(torch.int64), rb.view(torch.int64)), f"rep {rep}: second array"
+
+
+def test_tune_batched_picks_options_and_fills_alike(tmp_path):
+    """tah_tune_batched sets one choice of fork, defer and copy_per_thread on
+    every plan of the batch, the batched fill and its graph stay bit for bit
+    the single fills, and a second call reads the choice from the cache."""
+    from treeamr_hip import tune_batched
+    pairs = _forests(range(2, 8))
+    print(f"[tune batched] {len(pairs)} field sets", flush=True)
+    ref = _reference(pairs)
+    cache = tmp_path / "tune.txt"
+    choice, ms = tune_batched(pairs, cache)
+    assert ms is not None and ms > 0
+    assert choice["copy_per_thread"] in (4, 8, 16)
+    for fs, s in pairs:
+        o = s.plan(fs.layout).options
+        assert (o["fork"], o["defer"], o["copy_per_thread"]) == (choice["fork"], choice["defer"],
+                                                                  choice["copy_per_thread"])
+    inits = [fs.work.clone() for fs, _ in pairs]
+    fill_ghosts_batched(pairs)
+    torch.cuda.synchronize()
+    for (fs, _), r in zip(pairs, ref):
+        assert torch.equal(fs.work.view(torch.int64), r.view(torch.int64))
+    for (fs, _), i in zip(pairs, inits):
+        fs.work.copy_(i)
+    g = FillGraph(pairs)
+    g.launch()
+    torch.cuda.synchronize()
+    for (fs, _), r in zip(pairs, ref):
+        assert torch.equal(fs.work.view(torch.int64), r.view(torch.int64))
+    # from the cache, after the plans were set otherwise
+    for fs, s in pairs:
+        s.plan(fs.layout).set_options(fork=1 - choice["fork"], copy_per_thread=4 if choice["copy_per_thread"] != 4 else 8)
+    again, ms2 = tune_batched(pairs, cache)
+    assert ms2 is None and again == choice
+    with pytest.raises(ValueError):
+        tune_batched([])

@@ -447,4 +447,117 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
   });
 }
 
+tah_status tah_tune_batched(tah_ctx* ctx, int32_t n, tah_plan* const* plans, double* const* works,
+                            const double* const* factors, const char* cache_path, double* best_ms) {
+  return guard([&] {
+    if (!ctx || n < 1 || !plans || !works) fail("null argument or empty batch");
+    for (int32_t i = 0; i < n; ++i) {
+      if (!plans[i] || !works[i]) fail("null plan or array in the batch");
+      if (plans[i]->ctx != ctx) fail("a plan of the batch belongs to another context");
+      if (plans[i]->regrid) fail("tah_tune_batched times fills; a plan of the batch is a regrid transfer's");
+    }
+    HIPCHECK(hipSetDevice(ctx->device));
+    Hash H;
+    H.add(n);
+    for (int32_t i = 0; i < n; ++i) {
+      const std::string k = signature(plans[i]);
+      H.add(k.data(), k.size());
+    }
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%016llx", static_cast<unsigned long long>(H.h));
+    const std::string key = std::string(ctx->prop.gcnArchName) + "-" +
+                            std::to_string(ctx->prop.multiProcessorCount) + "-batch-" + buf;
+    auto apply = [&](int32_t fork, int32_t defer, int32_t cpt) {
+      for (int32_t i = 0; i < n; ++i) {
+        tah_plan* p = plans[i];
+        // set_options sets every family to the plan-wide options: keep the
+        // families' own
+        std::vector<tah_family_options> fams;
+        for (int f : plan_families(p)) fams.push_back(family_options(p, f));
+        tah_plan_options o{fork, p->merge, defer, cpt, 0, p->copy_wgs, p->interp_pairs, p->lds, p->queue, {0, 0, 0}};
+        set_options(p, o);
+        for (const auto& f : fams) set_family(p, f);
+      }
+    };
+    if (cache_path) {
+      std::ifstream in(cache_path);
+      std::string line, best_line;
+      while (std::getline(in, line)) {
+        std::istringstream ss(line);
+        std::string tag, k;
+        if (ss >> tag >> k && tag == "tah-tune-batch-1" && k == key) best_line = line;
+      }
+      if (!best_line.empty()) {
+        std::istringstream ss(best_line);
+        std::string tag, k;
+        int32_t fork = 0, defer = 0, cpt = 8;
+        if (ss >> tag >> k >> fork >> defer >> cpt && fork >= 0 && fork <= 1 && defer >= 0 && defer <= 1 &&
+            (cpt == 4 || cpt == 8 || cpt == 16)) {
+          apply(fork, defer, cpt);
+          if (best_ms) *best_ms = -1.0;
+          return;
+        }
+      }
+    }
+    hipStream_t s;
+    HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    auto run = [&]() {
+      tah_status st = tah_fill_batched(ctx, n, plans, works, factors, s);
+      if (st != TAH_OK) fail(std::string("tah_tune_batched: a trial fill failed: ") + tah_last_error(), st);
+    };
+    // the median of five batched fills after two unmeasured ones (the
+    // first builds the batch for these options)
+    auto time_fill = [&]() {
+      for (int w = 0; w < 2; ++w) run();
+      std::vector<float> ts;
+      for (int r = 0; r < 5; ++r) {
+        HIPCHECK(hipEventRecord(e0, s));
+        run();
+        HIPCHECK(hipEventRecord(e1, s));
+        HIPCHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+        ts.push_back(ms);
+      }
+      std::sort(ts.begin(), ts.end());
+      return ts[2];
+    };
+    int32_t bf = plans[0]->fork, bd = plans[0]->defer, bc = plans[0]->copy_per_thread;
+    float best_t = 1e30f;
+    try {
+      for (int32_t fork : {0, 1})
+        for (int32_t defer : {0, 1})
+          for (int32_t cpt : {4, 8, 16}) {
+            apply(fork, defer, cpt);
+            const float t = time_fill();
+            if (t < best_t) {
+              best_t = t;
+              bf = fork;
+              bd = defer;
+              bc = cpt;
+            }
+          }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      (void)hipStreamDestroy(s);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHECK(hipStreamSynchronize(s));
+    (void)hipStreamDestroy(s);
+    apply(bf, bd, bc);
+    if (best_ms) *best_ms = best_t;
+    if (cache_path) {
+      std::ofstream out(cache_path, std::ios::app);
+      if (!out) fail(std::string("tah_tune_batched: cannot write the cache file ") + cache_path);
+      out << "tah-tune-batch-1 " << key << " " << bf << " " << bd << " " << bc << " " << best_t << "\n";
+    }
+  });
+}
+
 }  // extern "C"

@@ -17,6 +17,7 @@ from .core import (  # noqa: F401
     Forest, MortonKey, Operators, FieldSet, GhostSchedule, fill_ghosts, fill_ghosts_adjoint,
     PointValue, Conservative, NoParity, EvenParity, OddParity, cellcentered, vertexcentered,
     Context, default_context, Plan, fill_ghosts_batched, FillGraph, fill_ghosts_timed, kernel_class_name, tune,
+    tune_batched,
     ownextents, statelength, statevector, staterange, stateview, statearray, scatter, gather,
     RegridSchedule, regrid, regrid_transfer, UnifySchedule, unify, unify_desc,
     AllVariables, WithPatch, Relative, CellBoundary, boundary_by_coordinates, boundary_values,

@@ -112,6 +112,8 @@ SIGNATURES = {
     "tah_plan_get_family_options": [P, I32P, C.POINTER(tah_family_options)],
     "tah_plan_set_family_options": [P, C.POINTER(tah_family_options)],
     "tah_tune": [P, P, P, C.c_char_p, C.POINTER(C.c_double)],
+    "tah_tune_batched": [P, C.c_int32, C.POINTER(P), C.POINTER(P), C.POINTER(P), C.c_char_p,
+                         C.POINTER(C.c_double)],
     "tah_state_length": [C.POINTER(tah_layout), I32P, C.POINTER(C.c_int64)],
     "tah_forest_coarsen": [P, C.c_int32, I32P, I32P],
     "tah_forest_positions": [P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, DP],

@@ -1208,6 +1208,21 @@ def fill_ghosts_batched(pairs, *, ctx=None):
     return pairs
 
 
+def tune_batched(pairs, cache_path=None, *, ctx=None):
+    """tah_tune_batched: time the batched fill of these (fs, schedule) pairs
+    for every combination of fork, defer and copy_per_thread (the same on
+    every plan) and keep the fastest; cached by the batch's signature.
+    Returns (dict(fork=, defer=, copy_per_thread=), median ms of the choice,
+    or None when taken from the cache)."""
+    ctx, n, plans, works, facs, keep = _batch_args(pairs, ctx)
+    ms = C.c_double()
+    path = None if cache_path is None else str(cache_path).encode()
+    check(lib().tah_tune_batched(ctx._h, n, plans, works, facs, path, C.byref(ms)))
+    o = pairs[0][1].plan(pairs[0][0].layout, ctx).options
+    return (dict(fork=o["fork"], defer=o["defer"], copy_per_thread=o["copy_per_thread"]),
+            None if ms.value < 0 else ms.value)
+
+
 class FillGraph:
     """A batched fill captured once in a hipGraph and replayed by launch().
     The field sets' working arrays must stay where they are."""
