@@ -24,6 +24,10 @@ import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+# The CPU baseline's threads stay on their cores (the reference pins its
+# CPU fill's threads the same way); set before any OpenMP runtime starts.
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 sys.path.insert(0, os.path.join(ROOT, "treeamr.jl_amd"))
 sys.path.insert(0, ROOT)
 
@@ -84,7 +88,61 @@ def cpu_baseline(sched_builder, seconds=12.0):
     values = sched.points() * nvars
     return dict(value=16.0 * values / dt / 1e9, unit="GB/s", cores=nthreads, kind="port",
                 sample=f"the same mesh with 1 of the 20 variables ({values} values per fill), "
-                       f"{reps} fills, {dt * 1e3:.1f} ms per fill, OpenMP threads pinned by block owner")
+                       f"{reps} fills, {dt * 1e3:.1f} ms per fill, OpenMP threads owning contiguous "
+                       f"block ranges", host=host_cpu())
+
+
+def host_cpu():
+    """The host the CPU baseline runs on: CPU model, sockets and logical CPUs
+    (from /proc/cpuinfo), and the CPUs this process may use."""
+    model, sockets, logical = None, set(), 0
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                key, _, val = line.partition(":")
+                key = key.strip()
+                if key == "model name" and model is None:
+                    model = val.strip()
+                elif key == "physical id":
+                    sockets.add(val.strip())
+                elif key == "processor":
+                    logical += 1
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return dict(model=model, sockets=len(sockets) or None, logical_cpus=logical or os.cpu_count(),
+                usable_cpus=usable, omp_proc_bind=os.environ.get("OMP_PROC_BIND"),
+                omp_places=os.environ.get("OMP_PLACES"))
+
+
+def footprint(sched, nvars):
+    """The reference's footprint figure (bench/nested.jl:67-73): the distinct
+    points a fill reads -- the union, per source block, of every transfer's
+    source windows -- and the points it writes (each ghost once), as
+    (points read, points written, bytes = 8 * nvars * their sum)."""
+    import numpy as np
+    groups = sched.groups()
+    nblocks = 1 + max((int(max(g["sourceblocks"].max(), g["targetblocks"].max()))
+                       for g in groups if g["ntransfers"]), default=-1)
+    ext = [0, 0, 0]
+    for g in groups:
+        for d, st in enumerate(g["stencils"]):
+            if st["ntarget"]:
+                ext[d] = max(ext[d], int(st["srcstart"].max()) + st["order"])
+    read = np.zeros((max(nblocks, 1), ext[2], ext[1], ext[0]), dtype=bool)
+    written = 0
+    for g in groups:
+        st = g["stencils"]
+        n = [x["ntarget"] for x in st]
+        if g["ntransfers"] == 0 or min(n) == 0:
+            continue
+        lo = [int(x["srcstart"].min()) for x in st]
+        hi = [int(x["srcstart"].max()) + x["order"] for x in st]
+        written += n[0] * n[1] * n[2] * g["ntransfers"]
+        for b in np.unique(g["sourceblocks"]):
+            read[b, lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]] = True
+    nread = int(read.sum())
+    return nread, written, 8 * nvars * (nread + written)
 
 
 def device_copy_rate(gib=2.0, reps=10):
@@ -554,6 +612,10 @@ def main():
 
     copy_ref = device_copy_rate()
     log(f"[bench] device copy of 2 GiB: {copy_ref} GB/s")
+    fp_read, fp_written, fp_bytes = footprint(sched, nvars)
+    fp = {"points_read": fp_read, "points_written": fp_written, "bytes": fp_bytes,
+          "GBs": round(fp_bytes / (t_ms * 1e-3) / 1e9, 1)}
+    log(f"[bench] footprint: {fp}")
 
     out = {
         "metric": METRIC,
@@ -578,6 +640,7 @@ def main():
             "plan_options": tuned["options"] if tuned else "defaults (fork 0, merge 1, defer 1, copy tiles of 8)",
         },
         "pct_of_peak": round(100.0 * gbs / HBM_PEAK_GBS, 2),
+        "footprint": fp,
         "graph_ms": r(head),
         "graph_ms_default_options": r(graphed),
         "staged_ms": r(staged),
