@@ -130,12 +130,17 @@ struct AdjointTables {
   AdjEntry* dentries = nullptr;
   Tile* dzero = nullptr;
   std::vector<AdjStage> stages;
+  // fork and join of a stage's zeroing on a second stream (made at set-up,
+  // so that a run creates nothing)
+  hipEvent_t zero_fork = nullptr, zero_join = nullptr;
 };
 
 void free_adjoint(tah_plan* p) {
   if (!p->adj) return;
   if (p->adj->dmem) (void)hipFree(p->adj->dmem);
   if (p->adj->scratch) (void)hipFree(p->adj->scratch);
+  if (p->adj->zero_fork) (void)hipEventDestroy(p->adj->zero_fork);
+  if (p->adj->zero_join) (void)hipEventDestroy(p->adj->zero_join);
   delete p->adj;
   p->adj = nullptr;
 }
@@ -440,6 +445,23 @@ __global__ void __launch_bounds__(kThreads) zero_kernel(const Job* __restrict__ 
                 s2 = int32_t(job.stride[2]), s3 = int32_t(job.stride[3]);
   const uint32_t total = uint32_t(g.points) * uint32_t(job.nvars);
   double* __restrict__ tbase = job.work + job.stride[4] * g.tblocks[tile.transfer];
+  if (job.pairs) {
+    // variables innermost in even number, 16-byte aligned: values 2e and
+    // 2e+1 are one point's neighbouring variables, zeroed by one 16-byte
+    // store that streams past the caches (nothing reads them again)
+    typedef double pair_t __attribute__((ext_vector_type(2)));
+    const pair_t zero2 = {0.0, 0.0};
+    for (int r = 0; r < kCopyPerThread / 2; ++r) {
+      const uint32_t e = uint32_t(tile.e0) + 2u * (uint32_t(r * kThreads) + threadIdx.x);
+      if (e >= total) break;
+      int32_t i, j, k, v;
+      split_element(g, job, e, i, j, k, v);
+      __builtin_nontemporal_store(
+          zero2, reinterpret_cast<pair_t*>(tbase + (v + s0 * (g.tfirst[0] + i) + s1 * (g.tfirst[1] + j) +
+                                                    s2 * (g.tfirst[2] + k))));
+    }
+    return;
+  }
   for (int r = 0; r < kCopyPerThread; ++r) {
     const uint32_t e = uint32_t(tile.e0) + uint32_t(r * kThreads) + threadIdx.x;
     if (e >= total) break;
@@ -781,6 +803,9 @@ void build_adjoint_tables(tah_plan* p) {
 void build_adjoint(tah_plan* p) {
   try {
     build_adjoint_tables(p);
+    HIPCHECK(hipEventCreateWithFlags(&p->adj->zero_fork, hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&p->adj->zero_join, hipEventDisableTiming));
+    ctx_streams(p->ctx);
   } catch (...) {
     free_adjoint(p);
     throw;
@@ -811,15 +836,29 @@ void check_adjoint_call(tah_plan* p, double* adj, const double* factors) {
 // staged value the gather after it reads, so the buffer needs no clearing.
 // `after` (may be null) is called after each launch with its kind (0-2 the
 // z, y, x passes, 3 the gather, 4 the zeroing) and algorithmic bytes.
+// fork: a stage's zeroing runs on a second stream once the z pass (the one
+// launch that reads the interpolated targets) is done, beside the y and x
+// passes and the gather, which neither read nor write those targets (a
+// stage's targets are never its sources); the stage ends when both have.
 template <class After>
 void run_adjoint_with(tah_plan* p, double* adj, const double* factors, hipStream_t s, double* staged,
-                      const After& after) {
+                      const After& after, bool fork = false) {
   plan_upload_job(p, plan_job(p, adj, factors), s);
   const AdjointTables& A = *p->adj;
   for (int32_t st = p->nstages - 1; st >= 0; --st) {
     const AdjStage& S = A.stages[st];
     launch_pass<2>(p, S, 0, s, staged);
     if (S.pass_tiles[0][1] > 0) after(0, S.bytes[0]);
+    const bool forked = fork && S.zero[1] > 0;
+    if (forked) {
+      const hipStream_t zs = ctx_streams(p->ctx)[0];
+      HIPCHECK(hipEventRecord(A.zero_fork, s));
+      HIPCHECK(hipStreamWaitEvent(zs, A.zero_fork, 0));
+      hipLaunchKernelGGL(zero_kernel, dim3(uint32_t(S.zero[1])), dim3(kThreads), 0, zs, p->djob,
+                         A.dzero + S.zero[0]);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipEventRecord(A.zero_join, zs));
+    }
     launch_pass<1>(p, S, 1, s, staged);
     if (S.pass_tiles[1][1] > 0) after(1, S.bytes[1]);
     launch_pass<0>(p, S, 2, s, staged);
@@ -830,7 +869,9 @@ void run_adjoint_with(tah_plan* p, double* adj, const double* factors, hipStream
       HIPCHECK(hipGetLastError());
       after(3, S.bytes[3]);
     }
-    if (S.zero[1] > 0) {
+    if (forked) {
+      HIPCHECK(hipStreamWaitEvent(s, A.zero_join, 0));
+    } else if (S.zero[1] > 0) {
       hipLaunchKernelGGL(zero_kernel, dim3(uint32_t(S.zero[1])), dim3(kThreads), 0, s, p->djob,
                          A.dzero + S.zero[0]);
       HIPCHECK(hipGetLastError());
@@ -840,7 +881,7 @@ void run_adjoint_with(tah_plan* p, double* adj, const double* factors, hipStream
 }
 
 void run_adjoint(tah_plan* p, double* adj, const double* factors, hipStream_t s, double* staged) {
-  run_adjoint_with(p, adj, factors, s, staged, [](int, int64_t) {});
+  run_adjoint_with(p, adj, factors, s, staged, [](int, int64_t) {}, true);
 }
 
 }  // namespace
