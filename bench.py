@@ -28,6 +28,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 # CPU fill's threads the same way); set before any OpenMP runtime starts.
 os.environ.setdefault("OMP_PROC_BIND", "close")
 os.environ.setdefault("OMP_PLACES", "cores")
+# the CPUs this process may use, read before an OpenMP runtime binds the
+# main thread to one of them
+USABLE_CPUS = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
 sys.path.insert(0, os.path.join(ROOT, "treeamr.jl_amd"))
 sys.path.insert(0, ROOT)
 
@@ -109,9 +112,8 @@ def host_cpu():
                     logical += 1
     except OSError:
         pass
-    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     return dict(model=model, sockets=len(sockets) or None, logical_cpus=logical or os.cpu_count(),
-                usable_cpus=usable, omp_proc_bind=os.environ.get("OMP_PROC_BIND"),
+                usable_cpus=USABLE_CPUS, omp_proc_bind=os.environ.get("OMP_PROC_BIND"),
                 omp_places=os.environ.get("OMP_PLACES"))
 
 
