@@ -271,6 +271,8 @@ def main():
             "graph_batched_ms": round(bgraph["mean_ms"], 3),
             "speedup": round(bplain["mean_ms"] / bgraph["mean_ms"], 2),
             "graph_effective_GBs": round(16.0 * bvalues / (bgraph["mean_ms"] * 1e-3) / 1e9, 1),
+            "graph_pct_of_peak": round(100.0 * 16.0 * bvalues / (bgraph["mean_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
+            "plain_effective_GBs": round(16.0 * bvalues / (bplain["mean_ms"] * 1e-3) / 1e9, 1),
         }
         del bgraph_obj, pairs
         return rec
@@ -648,12 +650,15 @@ def main():
         "staged_ms": r(staged),
         "plain_ms": r(plain),
         "plain_dense_layout_ms": r(plain_dense),
+        "plain_GBs": round(gbs_of(plain["mean_ms"]), 1) if plain else None,
+        "plain_pct_of_peak": round(100.0 * gbs_of(plain["mean_ms"]) / HBM_PEAK_GBS, 2) if plain else None,
         "speedup_vs_plain": round(plain["mean_ms"] / t_ms, 3) if plain else None,
         "speedup_vs_plain_dense_layout": round(plain_dense["mean_ms"] / t_ms, 3) if plain_dense else None,
         "tuned": tuned,
         "adjoint_ms": r(adjoint),
         "adjoint_over_forward": round(adjoint["mean_ms"] / t_ms, 3) if adjoint else None,
         "adjoint_effective_GBs": round(gbs_of(adjoint["mean_ms"]), 1) if adjoint else None,
+        "adjoint_pct_of_peak": round(100.0 * gbs_of(adjoint["mean_ms"]) / HBM_PEAK_GBS, 2) if adjoint else None,
         "adjoint_roofline": adjoint_roofline(adj_kernels),
         "adjoint_kernels": adj_kernels or None,
         "batched": batched,
