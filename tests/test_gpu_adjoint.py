@@ -187,3 +187,25 @@ def test_adjoint_timed_bytes_closed_form():
     ghosts = (N + 2 * G) ** 3 - N ** 3
     assert sum(nb for _, nb, _ in recs) == 16 * nv * f.nleaves * (own_near_face + ghosts)
     assert all(ms > 0 for _, _, ms in recs)
+
+
+@pytest.mark.parametrize("which,layout", [("vertex", "arena"), ("reflecting", "arena"), ("reflecting", "dense"),
+                                          ("config1", "arena")])
+def test_forked_zeroing_equals_launch_by_launch(which, layout):
+    """tah_fill_adjoint runs a stage's zeroing of its interpolated targets on
+    a second stream beside the stage's y and x passes and gather;
+    tah_fill_adjoint_timed runs every launch in order on one. On meshes with
+    restriction and prolongation both give the same bits, twice in a row."""
+    from treeamr_hip import fill_ghosts_adjoint_timed
+    fs, s = _setup(which, layout)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    y = torch.rand(fs.work.shape, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    for _ in range(2):
+        a, b = y.clone(), y.clone()
+        fill_ghosts_adjoint(fs, s, a)
+        recs = fill_ghosts_adjoint_timed(fs, s, b)
+        torch.cuda.synchronize()
+        assert any(name == "zero_kernel" for name, _, _ in recs)
+        assert torch.equal(a.view(torch.int64), b.view(torch.int64))
+        y = a
