@@ -314,7 +314,9 @@ __device__ __forceinline__ void gather_paired(const Job& job, const AdjCell& c, 
     p[k][2] = c.lo[2] + int32_t(zq);
     off[k] = var[k] + s0 * p[k][0] + s1 * p[k][1] + s2 * p[k][2];
     tot[k] = zero2;
-    if (ok[k] && !c.target) tot[k] = *reinterpret_cast<const pair_t*>(sb + off[k]);
+    // (each source value is read and written by this thread alone, once:
+    // both streamed past the caches)
+    if (ok[k] && !c.target) tot[k] = __builtin_nontemporal_load(reinterpret_cast<const pair_t*>(sb + off[k]));
   }
   const int32_t first = uniform(c.first), count = uniform(c.count);
   for (int32_t q = 0; q < count; ++q) {
@@ -329,8 +331,8 @@ __device__ __forceinline__ void gather_paired(const Job& job, const AdjCell& c, 
       const int32_t o = E.a[0] * p[k][0] + E.a[1] * p[k][1] + E.a[2] * p[k][2] + E.av * var[k];
       pair_t acc;
       // A copy's targets are read once and zeroed: both streamed past the
-      // caches (the large forest's gather: 15.7 against 17.1 ms with cached
-      // loads).
+      // caches, as the source values are (the large forest's gather: 16.4–16.5
+      // against 17.5–17.6 ms with cached loads and stores, on one box).
       if (single) {
         if (copy) {
           acc = __builtin_nontemporal_load(reinterpret_cast<const pair_t*>(src + o));
@@ -362,7 +364,7 @@ __device__ __forceinline__ void gather_paired(const Job& job, const AdjCell& c, 
   }
 #pragma unroll
   for (int k = 0; k < K; ++k)
-    if (ok[k]) *reinterpret_cast<pair_t*>(sb + off[k]) = tot[k];
+    if (ok[k]) __builtin_nontemporal_store(tot[k], reinterpret_cast<pair_t*>(sb + off[k]));
 }
 
 __global__ void __launch_bounds__(kThreads) adjoint_gather_kernel(const Job* __restrict__ jobs,
@@ -406,7 +408,7 @@ __global__ void __launch_bounds__(kThreads) adjoint_gather_kernel(const Job* __r
     p[k][1] = c.lo[1] + int32_t(q - zq * uint32_t(c.ext[1]));
     p[k][2] = c.lo[2] + int32_t(zq);
     off[k] = s3 * var[k] + s0 * p[k][0] + s1 * p[k][1] + s2 * p[k][2];
-    tot[k] = ok[k] && !c.target ? sb[off[k]] : 0.0;
+    tot[k] = ok[k] && !c.target ? __builtin_nontemporal_load(sb + off[k]) : 0.0;
   }
   const int32_t first = uniform(c.first), count = uniform(c.count);
   for (int32_t q = 0; q < count; ++q) {
@@ -444,7 +446,7 @@ __global__ void __launch_bounds__(kThreads) adjoint_gather_kernel(const Job* __r
   }
 #pragma unroll
   for (int k = 0; k < kAdjPerThread; ++k)
-    if (ok[k]) sb[off[k]] = tot[k];
+    if (ok[k]) __builtin_nontemporal_store(tot[k], sb + off[k]);
 }
 
 // Zero the target boxes of one stage (tiles as the copy kernel's).
