@@ -330,15 +330,16 @@ __device__ __forceinline__ void gather_paired(const Job& job, const AdjCell& c, 
       if (!ok[k]) continue;
       const int32_t o = E.a[0] * p[k][0] + E.a[1] * p[k][1] + E.a[2] * p[k][2] + E.av * var[k];
       pair_t acc;
-      // A copy's targets are read once and zeroed: both streamed past the
-      // caches, as the source values are (the large forest's gather: 16.4–16.5
-      // against 17.5–17.6 ms with cached loads and stores, on one box).
+      // A copy's targets are read once and zeroed, a staged value read once:
+      // streamed past the caches, as the source values are (the large
+      // forest's gather: 16.4–16.5 against 17.5–17.6 ms with cached loads and
+      // stores, on one box).
       if (single) {
         if (copy) {
           acc = __builtin_nontemporal_load(reinterpret_cast<const pair_t*>(src + o));
           __builtin_nontemporal_store(zero2, reinterpret_cast<pair_t*>(src + o));
         } else {
-          acc = *reinterpret_cast<const pair_t*>(src + o);
+          acc = __builtin_nontemporal_load(reinterpret_cast<const pair_t*>(src + o));
         }
       } else {
         acc = zero2;
@@ -346,8 +347,7 @@ __device__ __forceinline__ void gather_paired(const Job& job, const AdjCell& c, 
           for (int32_t r1 = 0; r1 < c1; ++r1)
             for (int32_t r0 = 0; r0 < c0; ++r0) {
               const int32_t oo = o + E.rs[0] * r0 + E.rs[1] * r1 + E.rs[2] * r2;
-              const pair_t x = copy ? __builtin_nontemporal_load(reinterpret_cast<const pair_t*>(src + oo))
-                                    : *reinterpret_cast<const pair_t*>(src + oo);
+              const pair_t x = __builtin_nontemporal_load(reinterpret_cast<const pair_t*>(src + oo));
               acc.x = __dadd_rn(acc.x, x.x);
               acc.y = __dadd_rn(acc.y, x.y);
               if (copy) __builtin_nontemporal_store(zero2, reinterpret_cast<pair_t*>(src + oo));
@@ -428,7 +428,7 @@ __global__ void __launch_bounds__(kThreads) adjoint_gather_kernel(const Job* __r
           acc = __builtin_nontemporal_load(src + o);
           __builtin_nontemporal_store(0.0, src + o);
         } else {
-          acc = src[o];
+          acc = __builtin_nontemporal_load(src + o);
         }
       } else {
         acc = 0.0;
@@ -436,7 +436,7 @@ __global__ void __launch_bounds__(kThreads) adjoint_gather_kernel(const Job* __r
           for (int32_t r1 = 0; r1 < c1; ++r1)
             for (int32_t r0 = 0; r0 < c0; ++r0) {
               const int32_t oo = o + E.rs[0] * r0 + E.rs[1] * r1 + E.rs[2] * r2;
-              acc = __dadd_rn(acc, copy ? __builtin_nontemporal_load(src + oo) : src[oo]);
+              acc = __dadd_rn(acc, __builtin_nontemporal_load(src + oo));
               if (copy) __builtin_nontemporal_store(0.0, src + oo);
             }
       }
