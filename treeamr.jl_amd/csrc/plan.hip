@@ -657,8 +657,10 @@ void build_launches(tah_plan* p) {
   }
 
   // Tiles: per stage group, per class; within a launch ordered by target
-  // block (restriction: by source block), so that the tiles of one block run
-  // close together in time.
+  // block (restriction and copies: by source block), so that the tiles of
+  // one block run close together in time. (Copies by source block: 6.60 ms
+  // per fill on the large forest against 6.70 ms by target block and 6.64
+  // ms by the lower of the two, interleaved runs on one box.)
   const int32_t nvars = p->layout.nvars;
   const bool vfast = p->layout.stride[3] < p->layout.stride[0];
   p->tiles.clear();
@@ -774,7 +776,7 @@ void build_launches(tah_plan* p) {
     classes.erase(std::unique(classes.begin(), classes.end()), classes.end());
     struct Built {
       int cls;
-      std::vector<std::pair<int32_t, Tile>> tiles;  // (target block, tile)
+      std::vector<std::pair<int32_t, Tile>> tiles;  // (ordering block, tile)
       int64_t values;
       std::vector<std::pair<int32_t, LdsTile>> ltiles;  // LDS-staged: (source block, tile)
       int32_t lds_bytes = 0;
@@ -950,7 +952,7 @@ void build_launches(tah_plan* p) {
               }
               int32_t t1 = t;
               while (t1 < nt && run_in[gi][t1] == k && t1 - t < pack) ++t1;
-              tl.push_back({g.tblocks[t], Tile{0, int32_t(gi), t, 0, t1 - t, 0, 0, 0}});
+              tl.push_back({g.sblocks[t], Tile{0, int32_t(gi), t, 0, t1 - t, 0, 0, 0}});
               values += elems * (t1 - t);
               t = t1;
             }
@@ -958,7 +960,8 @@ void build_launches(tah_plan* p) {
             for (int32_t t = 0; t < nt; ++t) {
               if (run_in[gi][t] != k) continue;
               for (int64_t e = 0; e < elems; e += per)
-                tl.push_back({g.tblocks[t], Tile{0, int32_t(gi), t, int32_t(e), 0, 0, 0, 0}});
+                tl.push_back({g.cls == kClsCopy ? g.sblocks[t] : g.tblocks[t],
+                              Tile{0, int32_t(gi), t, int32_t(e), 0, 0, 0, 0}});
               values += elems;
             }
           }
