@@ -184,7 +184,7 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
                           kv.first.second});
       ltiles.insert(ltiles.end(), kv.second.begin(), kv.second.end());
     }
-    std::stable_sort(ls.begin(), ls.end(), [](const Launch& x, const Launch& y) { return x.count > y.count; });
+    order_launches(ls);
     b->launches.push_back(std::move(ls));
   }
   HIPCHECK(hipSetDevice(ctx->device));

@@ -994,8 +994,7 @@ void build_launches(tah_plan* p) {
       for (const auto& e : B.tiles) p->tiles.push_back(e.second);
       ls.push_back(L);
     }
-    // Largest launches first, so that they start first on their streams.
-    std::stable_sort(ls.begin(), ls.end(), [](const Launch& x, const Launch& y) { return x.count > y.count; });
+    order_launches(ls);
     p->launches.push_back(std::move(ls));
   }
   // A new state of the plan: batches built from an earlier one are stale.

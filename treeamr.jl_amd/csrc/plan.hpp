@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <string>
 #include <vector>
@@ -78,6 +79,21 @@ struct Launch {
   int32_t lds = 0;     // LDS-staged: bytes of LDS per workgroup; first/count index LdsTiles
   int32_t threads = kThreads;  // workgroup size (interpolation families and streamed chunks: 64, 128 or 256)
 };
+
+// The launches of one stage group in the order they are issued:
+// interpolations before copies, then the largest first. Forked, the
+// interpolations' workgroups are then dispatched first and the copies fill
+// the room they leave; one after another, the copies read the sources
+// restriction has just read. The large forest's tuned fill: 11.16-11.22
+// against 12.07-12.11 ms with the largest launch (the copies) first; its
+// copies alone 5.65 against 6.60 ms.
+inline void order_launches(std::vector<Launch>& ls) {
+  auto is_copy = [](const Launch& L) { return L.cls > kClsCopyLaunch && L.cls <= kClsCopyLaunch + 16; };
+  std::stable_sort(ls.begin(), ls.end(), [&](const Launch& x, const Launch& y) {
+    if (is_copy(x) != is_copy(y)) return !is_copy(x);
+    return x.count > y.count;
+  });
+}
 
 // The interpolation families: the index into kInterpTable of order P and
 // shift (restriction 2, prolongation 0).
