@@ -184,7 +184,10 @@ std::unique_ptr<Batch> make_batch(tah_ctx* ctx, int32_t n, tah_plan* const* plan
                           kv.first.second});
       ltiles.insert(ltiles.end(), kv.second.begin(), kv.second.end());
     }
-    order_launches(ls);
+    // The largest launch first, copies or not: on the 64-forest batch the
+    // forked graph fill took 20.30-20.40 ms so, against 20.71-20.87 ms with
+    // interpolations before copies (the order of a single plan's launches).
+    std::stable_sort(ls.begin(), ls.end(), [](const Launch& x, const Launch& y) { return x.count > y.count; });
     b->launches.push_back(std::move(ls));
   }
   HIPCHECK(hipSetDevice(ctx->device));
