@@ -398,8 +398,12 @@ tah_status tah_fill_adjoint_noalloc(tah_plan* p, double* adj, const double* fact
  * from a bounded grid, leaving wave slots to interpolations forked beside
  * them; interp_pairs gives each thread of a merged interpolation two
  * neighbouring variables (layouts with variables innermost and an even
- * count; 16-byte loads where the array is 16-byte aligned) or one (0).
- * Setting merge, defer, copy_per_thread, copy_wgs or interp_pairs rebuilds the
+ * count; 16-byte loads where the array is 16-byte aligned) or one (0);
+ * copy_order orders a copy launch's tiles by source block (0, default: the
+ * slabs that share source points are read together) or by target block (1:
+ * the writes into one block's ghost layer meet; faster where blocks are
+ * small). Setting merge, defer, copy_per_thread, copy_wgs, interp_pairs or
+ * copy_order rebuilds the
  * plan's tiles (it waits for the device first);
  * batches and graphs made earlier keep the options they were made with. */
 typedef struct tah_plan_options {
@@ -418,8 +422,9 @@ typedef struct tah_plan_options {
                               for the rest */
   int32_t queue;           /* in a batched fill of plans that all set it: one persistent launch per stage
                               position over a work queue of every plan's tiles (1), or a launch per kernel
-                              class (0, default); reserved_ must be zero */
-  int32_t reserved_[3];
+                              class (0, default) */
+  int32_t copy_order;      /* copy tiles by source block (0, default) or by target block (1) */
+  int32_t reserved_[2];    /* must be zero */
 } tah_plan_options;
 tah_status tah_plan_get_options(const tah_plan* p, tah_plan_options* out);
 /* Setting the plan-wide options sets every interpolation family's below to

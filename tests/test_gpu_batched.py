@@ -159,11 +159,11 @@ def test_tune_batched_picks_options_and_fills_alike(tmp_path):
     cache = tmp_path / "tune.txt"
     choice, ms = tune_batched(pairs, cache)
     assert ms is not None and ms > 0
-    assert choice["copy_per_thread"] in (4, 8, 16)
+    assert choice["copy_per_thread"] in (4, 8, 16) and choice["copy_order"] in (0, 1)
     for fs, s in pairs:
         o = s.plan(fs.layout).options
-        assert (o["fork"], o["defer"], o["copy_per_thread"]) == (choice["fork"], choice["defer"],
-                                                                  choice["copy_per_thread"])
+        assert (o["fork"], o["defer"], o["copy_per_thread"], o["copy_order"]) == (
+            choice["fork"], choice["defer"], choice["copy_per_thread"], choice["copy_order"])
     inits = [fs.work.clone() for fs, _ in pairs]
     fill_ghosts_batched(pairs)
     torch.cuda.synchronize()
@@ -178,7 +178,8 @@ def test_tune_batched_picks_options_and_fills_alike(tmp_path):
         assert torch.equal(fs.work.view(torch.int64), r.view(torch.int64))
     # from the cache, after the plans were set otherwise
     for fs, s in pairs:
-        s.plan(fs.layout).set_options(fork=1 - choice["fork"], copy_per_thread=4 if choice["copy_per_thread"] != 4 else 8)
+        s.plan(fs.layout).set_options(fork=1 - choice["fork"], copy_per_thread=4 if choice["copy_per_thread"] != 4 else 8,
+                                      copy_order=1 - choice["copy_order"])
     again, ms2 = tune_batched(pairs, cache)
     assert ms2 is None and again == choice
     with pytest.raises(ValueError):

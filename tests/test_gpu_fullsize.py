@@ -94,7 +94,8 @@ def test_large_forest_headline_form_against_oracle():
     plan = s.plan(fs.layout)
     for opts in (dict(fork=0, merge=1, defer=1, copy_per_thread=8, copy_wgs=0, interp_pairs=1),
                  dict(fork=1, merge=1, defer=1, copy_per_thread=16, copy_wgs=512, interp_pairs=1),
-                 dict(fork=1, merge=1, defer=0, copy_per_thread=16, copy_wgs=0, interp_pairs=1)):
+                 dict(fork=1, merge=1, defer=0, copy_per_thread=16, copy_wgs=0, interp_pairs=1),
+                 dict(fork=1, merge=1, defer=0, copy_per_thread=8, copy_wgs=0, interp_pairs=1, copy_order=1)):
         plan.set_options(**opts)
         graph = FillGraph([(fs, s)])
         for form, run in (("staged", lambda: fill_ghosts(fs, s)), ("graph", graph.launch)):

@@ -44,7 +44,18 @@ def test_every_option_combination_fills_alike(which):
         torch.cuda.synchronize()
         assert torch.equal(fs.work.view(torch.int64), ref.view(torch.int64)), (fork, merge, defer, cpt, wgs, ip,
                                                                                 lds)
-    plan.set_options(fork=0, merge=1, fuse=0, defer=1, copy_per_thread=8, copy_wgs=0, interp_pairs=1, lds=4)
+    # copy tiles by target block
+    for fork, defer, cpt, wgs in itertools.product((0, 1), (0, 1), (4, 8, 16), (0, 64)):
+        print(f"[tune] {which} copy_order=1 fork={fork} defer={defer} copy_per_thread={cpt} copy_wgs={wgs}",
+              flush=True)
+        plan.set_options(fork=fork, merge=1, fuse=0, defer=defer, copy_per_thread=cpt, copy_wgs=wgs,
+                         interp_pairs=1, lds=4, copy_order=1)
+        fs.work.copy_(init)
+        fill_ghosts(fs, s)
+        torch.cuda.synchronize()
+        assert torch.equal(fs.work.view(torch.int64), ref.view(torch.int64)), (fork, defer, cpt, wgs)
+    plan.set_options(fork=0, merge=1, fuse=0, defer=1, copy_per_thread=8, copy_wgs=0, interp_pairs=1, lds=4,
+                     copy_order=0)
 
 
 @pytest.mark.parametrize("which", ["cell", "cell-even", "vertex"])
@@ -116,7 +127,7 @@ def test_tune_caches_by_signature(tmp_path):
     opts, ms = tune(fs, s, path)
     assert ms is not None and ms > 0
     lines = path.read_text().splitlines()
-    assert len(lines) == 1 and lines[0].startswith("tah-tune-6 ")
+    assert len(lines) == 1 and lines[0].startswith("tah-tune-7 ")
     assert opts["families"] and all(f["threads"] in (64, 128, 256) for f in opts["families"])
     s.plan(fs.layout).set_options(fork=1 - opts["fork"])
     s.plan(fs.layout).set_family(opts["families"][0]["order"], opts["families"][0]["restriction"],
@@ -153,7 +164,8 @@ def test_option_values_refused():
     plan = s.plan(fs.layout)
     before = plan.options
     for bad in (dict(interp_pairs=2), dict(interp_pairs=-1), dict(copy_per_thread=5), dict(copy_wgs=12),
-                dict(fork=2), dict(lds=16), dict(lds=-1), dict(queue=2), dict(queue=-1)):
+                dict(fork=2), dict(lds=16), dict(lds=-1), dict(queue=2), dict(queue=-1), dict(copy_order=2),
+                dict(copy_order=-1)):
         with pytest.raises(ValueError):
             plan.set_options(**bad)
         assert plan.options == before, bad

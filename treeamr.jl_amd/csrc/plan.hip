@@ -657,10 +657,13 @@ void build_launches(tah_plan* p) {
   }
 
   // Tiles: per stage group, per class; within a launch ordered by target
-  // block (restriction and copies: by source block), so that the tiles of
-  // one block run close together in time. (Copies by source block: 6.60 ms
-  // per fill on the large forest against 6.70 ms by target block and 6.64
-  // ms by the lower of the two, interleaved runs on one box.)
+  // block (restriction: by source block; copies: by the plan option
+  // copy_order), so that the tiles of one block run close together in time.
+  // (Copies by source block: 6.60 ms per fill on the large forest against
+  // 6.70 ms by target block and 6.64 ms by the lower of the two; the
+  // 64-forest batch's forked fill 20.29-20.30 ms against 19.92-19.96 ms by
+  // target block; interleaved runs on one box.)
+  const bool copy_by_source = p->copy_order == 0;
   const int32_t nvars = p->layout.nvars;
   const bool vfast = p->layout.stride[3] < p->layout.stride[0];
   p->tiles.clear();
@@ -952,7 +955,7 @@ void build_launches(tah_plan* p) {
               }
               int32_t t1 = t;
               while (t1 < nt && run_in[gi][t1] == k && t1 - t < pack) ++t1;
-              tl.push_back({g.sblocks[t], Tile{0, int32_t(gi), t, 0, t1 - t, 0, 0, 0}});
+              tl.push_back({copy_by_source ? g.sblocks[t] : g.tblocks[t], Tile{0, int32_t(gi), t, 0, t1 - t, 0, 0, 0}});
               values += elems * (t1 - t);
               t = t1;
             }
@@ -960,7 +963,7 @@ void build_launches(tah_plan* p) {
             for (int32_t t = 0; t < nt; ++t) {
               if (run_in[gi][t] != k) continue;
               for (int64_t e = 0; e < elems; e += per)
-                tl.push_back({g.cls == kClsCopy ? g.sblocks[t] : g.tblocks[t],
+                tl.push_back({g.cls == kClsCopy && copy_by_source ? g.sblocks[t] : g.tblocks[t],
                               Tile{0, int32_t(gi), t, int32_t(e), 0, 0, 0, 0}});
               values += elems;
             }

@@ -167,6 +167,7 @@ struct tah_plan {
   // Workgroups of the copies' bounded grid (a multiple of 8), or 0 (the
   // default) for one workgroup per tile.
   int32_t copy_wgs = 0;
+  int32_t copy_order = 0;  // copy tiles by source block (0) or target block (1)
   // Merged interpolation launches over tiles of two neighbouring variables
   // per thread where the layout has variables innermost and an even count
   // (1, the default), or of one variable per thread (0).

@@ -89,14 +89,17 @@ void set_options(tah_plan* p, const tah_plan_options& o) {
     fail("plan option lds is 0 to 15 (bits 0 and 1: restriction and prolongation from LDS chunks; bits 2 and "
          "3: restriction and prolongation streamed along z)");
   if (o.queue < 0 || o.queue > 1) fail("plan option queue is 0 or 1");
-  if (o.reserved_[0] || o.reserved_[1] || o.reserved_[2]) fail("tah_plan_options.reserved_ must be zero");
+  if (o.copy_order < 0 || o.copy_order > 1) fail("plan option copy_order is 0 or 1");
+  if (o.reserved_[0] || o.reserved_[1]) fail("tah_plan_options.reserved_ must be zero");
   bool rebuild = o.merge != p->merge || o.defer != p->defer || o.copy_per_thread != p->copy_per_thread ||
-                 wgs != p->copy_wgs || o.interp_pairs != p->interp_pairs || o.lds != p->lds;
+                 wgs != p->copy_wgs || o.interp_pairs != p->interp_pairs || o.lds != p->lds ||
+                 o.copy_order != p->copy_order;
   p->fork = o.fork;
   p->merge = o.merge;
   p->defer = o.defer;
   p->copy_per_thread = o.copy_per_thread;
   p->copy_wgs = wgs;
+  p->copy_order = o.copy_order;
   p->interp_pairs = o.interp_pairs;
   p->lds = o.lds;
   if (o.queue != p->queue) {
@@ -169,8 +172,9 @@ void set_family(tah_plan* p, const tah_family_options& o) {
   build_launches(p);
 }
 
-// A cache line: "tah-tune-6 <signature> fork merge defer cpt wgs pairs lds
-// <families> (order restriction threads merge pairs lds vars)... <ms>".
+// A cache line: "tah-tune-7 <signature> fork merge defer cpt wgs pairs lds
+// copy_order <families> (order restriction threads merge pairs lds vars)...
+// <ms>".
 bool lookup(const std::string& path, const std::string& key, tah_plan_options& o,
             std::vector<tah_family_options>& fams) {
   std::ifstream in(path);
@@ -180,16 +184,16 @@ bool lookup(const std::string& path, const std::string& key, tah_plan_options& o
   while (std::getline(in, line)) {
     std::istringstream ss(line);
     std::string tag, k;
-    int f, m, d, c, w, ip, l, nf;
-    if (!(ss >> tag >> k >> f >> m >> d >> c >> w >> ip >> l >> nf) || tag != "tah-tune-6" || k != key || nf < 0 ||
-        nf > kFamilies)
+    int f, m, d, c, w, ip, l, co, nf;
+    if (!(ss >> tag >> k >> f >> m >> d >> c >> w >> ip >> l >> co >> nf) || tag != "tah-tune-7" || k != key ||
+        nf < 0 || nf > kFamilies)
       continue;
     std::vector<tah_family_options> fs(static_cast<size_t>(nf));
     bool ok = true;
     for (auto& e : fs)
       ok = ok && bool(ss >> e.order >> e.restriction >> e.threads >> e.merge >> e.pairs >> e.lds >> e.vars);
     if (!ok) continue;
-    o = tah_plan_options{f, m, d, c, 0, w, ip, l, 0, {0, 0, 0}};
+    o = tah_plan_options{f, m, d, c, 0, w, ip, l, 0, co, {0, 0}};
     fams = std::move(fs);
     found = true;  // the last entry wins
   }
@@ -207,7 +211,8 @@ std::vector<tah_plan_options> candidates(const tah_plan* p) {
       for (int c : {4, 8, 16})
         for (int w : {0, 512, 1024, 2048}) {
           if (w && !f) continue;
-          out.push_back(tah_plan_options{f, p->merge, d, c, 0, w, p->interp_pairs, p->lds, p->queue, {0, 0, 0}});
+          out.push_back(
+              tah_plan_options{f, p->merge, d, c, 0, w, p->interp_pairs, p->lds, p->queue, p->copy_order, {0, 0}});
         }
   return out;
 }
@@ -259,7 +264,8 @@ tah_status tah_plan_get_options(const tah_plan* p, tah_plan_options* out) {
     out->interp_pairs = p->interp_pairs;
     out->lds = p->lds;
     out->queue = p->queue;
-    out->reserved_[0] = out->reserved_[1] = out->reserved_[2] = 0;
+    out->copy_order = p->copy_order;
+    out->reserved_[0] = out->reserved_[1] = 0;
   });
 }
 
@@ -360,7 +366,7 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
       return ts[2];
     };
     tah_plan_options best{p->fork, p->merge, p->defer, p->copy_per_thread, 0, p->copy_wgs, p->interp_pairs, p->lds,
-                          p->queue, {0, 0, 0}};
+                          p->queue, p->copy_order, {0, 0}};
     std::vector<tah_family_options> best_fams;
     float best_t = 1e30f;
     try {
@@ -418,6 +424,27 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
         }
         set_family(p, tc < tb ? chosen : base);
       }
+      // 4. the copies' tile order (by source or by target block), the
+      // families as chosen: the two timed in turn, three times each
+      {
+        std::vector<tah_family_options> fams;
+        for (int f : plan_families(p)) fams.push_back(family_options(p, f));
+        auto with_order = [&](int32_t order) {
+          tah_plan_options o = best;
+          o.copy_order = order;
+          set_options(p, o);  // sets every family to the plan-wide options
+          for (const auto& fo : fams) set_family(p, fo);
+        };
+        float t0 = 1e30f, t1 = 1e30f;
+        for (int r = 0; r < 3; ++r) {
+          with_order(0);
+          t0 = std::min(t0, time_fill());
+          with_order(1);
+          t1 = std::min(t1, time_fill());
+        }
+        best.copy_order = t1 < t0 ? 1 : 0;
+        with_order(best.copy_order);
+      }
       best_t = time_fill();
       for (int f : plan_families(p)) best_fams.push_back(family_options(p, f));
     } catch (...) {
@@ -436,9 +463,9 @@ tah_status tah_tune(tah_plan* p, double* work, const double* factors, const char
     if (cache_path) {
       std::ofstream out(cache_path, std::ios::app);
       if (!out) fail(std::string("tah_tune: cannot write the cache file ") + cache_path);
-      out << "tah-tune-6 " << key << " " << best.fork << " " << best.merge << " " << best.defer << " "
+      out << "tah-tune-7 " << key << " " << best.fork << " " << best.merge << " " << best.defer << " "
           << best.copy_per_thread << " " << best.copy_wgs << " " << best.interp_pairs << " " << best.lds << " "
-          << best_fams.size();
+          << best.copy_order << " " << best_fams.size();
       for (const auto& f : best_fams)
         out << " " << f.order << " " << f.restriction << " " << f.threads << " " << f.merge << " " << f.pairs << " "
             << f.lds << " " << f.vars;
@@ -467,14 +494,14 @@ tah_status tah_tune_batched(tah_ctx* ctx, int32_t n, tah_plan* const* plans, dou
     std::snprintf(buf, sizeof buf, "%016llx", static_cast<unsigned long long>(H.h));
     const std::string key = std::string(ctx->prop.gcnArchName) + "-" +
                             std::to_string(ctx->prop.multiProcessorCount) + "-batch-" + buf;
-    auto apply = [&](int32_t fork, int32_t defer, int32_t cpt) {
+    auto apply = [&](int32_t fork, int32_t defer, int32_t cpt, int32_t order) {
       for (int32_t i = 0; i < n; ++i) {
         tah_plan* p = plans[i];
         // set_options sets every family to the plan-wide options: keep the
         // families' own
         std::vector<tah_family_options> fams;
         for (int f : plan_families(p)) fams.push_back(family_options(p, f));
-        tah_plan_options o{fork, p->merge, defer, cpt, 0, p->copy_wgs, p->interp_pairs, p->lds, p->queue, {0, 0, 0}};
+        tah_plan_options o{fork, p->merge, defer, cpt, 0, p->copy_wgs, p->interp_pairs, p->lds, p->queue, order, {0, 0}};
         set_options(p, o);
         for (const auto& f : fams) set_family(p, f);
       }
@@ -485,15 +512,15 @@ tah_status tah_tune_batched(tah_ctx* ctx, int32_t n, tah_plan* const* plans, dou
       while (std::getline(in, line)) {
         std::istringstream ss(line);
         std::string tag, k;
-        if (ss >> tag >> k && tag == "tah-tune-batch-1" && k == key) best_line = line;
+        if (ss >> tag >> k && tag == "tah-tune-batch-2" && k == key) best_line = line;
       }
       if (!best_line.empty()) {
         std::istringstream ss(best_line);
         std::string tag, k;
-        int32_t fork = 0, defer = 0, cpt = 8;
-        if (ss >> tag >> k >> fork >> defer >> cpt && fork >= 0 && fork <= 1 && defer >= 0 && defer <= 1 &&
-            (cpt == 4 || cpt == 8 || cpt == 16)) {
-          apply(fork, defer, cpt);
+        int32_t fork = 0, defer = 0, cpt = 8, order = 0;
+        if (ss >> tag >> k >> fork >> defer >> cpt >> order && fork >= 0 && fork <= 1 && defer >= 0 && defer <= 1 &&
+            (cpt == 4 || cpt == 8 || cpt == 16) && order >= 0 && order <= 1) {
+          apply(fork, defer, cpt, order);
           if (best_ms) *best_ms = -1.0;
           return;
         }
@@ -525,13 +552,13 @@ tah_status tah_tune_batched(tah_ctx* ctx, int32_t n, tah_plan* const* plans, dou
       std::sort(ts.begin(), ts.end());
       return ts[2];
     };
-    int32_t bf = plans[0]->fork, bd = plans[0]->defer, bc = plans[0]->copy_per_thread;
+    int32_t bf = plans[0]->fork, bd = plans[0]->defer, bc = plans[0]->copy_per_thread, bo = plans[0]->copy_order;
     float best_t = 1e30f;
     try {
       for (int32_t fork : {0, 1})
         for (int32_t defer : {0, 1})
           for (int32_t cpt : {4, 8, 16}) {
-            apply(fork, defer, cpt);
+            apply(fork, defer, cpt, bo);
             const float t = time_fill();
             if (t < best_t) {
               best_t = t;
@@ -540,6 +567,17 @@ tah_status tah_tune_batched(tah_ctx* ctx, int32_t n, tah_plan* const* plans, dou
               bc = cpt;
             }
           }
+      // then the copies' tile order at those options: the two timed in
+      // turn, twice each
+      float t0 = 1e30f, t1 = 1e30f;
+      for (int r = 0; r < 2; ++r) {
+        apply(bf, bd, bc, 0);
+        t0 = std::min(t0, time_fill());
+        apply(bf, bd, bc, 1);
+        t1 = std::min(t1, time_fill());
+      }
+      bo = t1 < t0 ? 1 : 0;
+      best_t = std::min(t0, t1);
     } catch (...) {
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
@@ -550,12 +588,12 @@ tah_status tah_tune_batched(tah_ctx* ctx, int32_t n, tah_plan* const* plans, dou
     (void)hipEventDestroy(e1);
     HIPCHECK(hipStreamSynchronize(s));
     (void)hipStreamDestroy(s);
-    apply(bf, bd, bc);
+    apply(bf, bd, bc, bo);
     if (best_ms) *best_ms = best_t;
     if (cache_path) {
       std::ofstream out(cache_path, std::ios::app);
       if (!out) fail(std::string("tah_tune_batched: cannot write the cache file ") + cache_path);
-      out << "tah-tune-batch-1 " << key << " " << bf << " " << bd << " " << bc << " " << best_t << "\n";
+      out << "tah-tune-batch-2 " << key << " " << bf << " " << bd << " " << bc << " " << bo << " " << best_t << "\n";
     }
   });
 }

@@ -64,7 +64,7 @@ class tah_plan_options(C.Structure):
     _fields_ = [("fork", C.c_int32), ("merge", C.c_int32), ("defer", C.c_int32),
                 ("copy_per_thread", C.c_int32), ("fuse", C.c_int32), ("copy_wgs", C.c_int32),
                 ("interp_pairs", C.c_int32), ("lds", C.c_int32), ("queue", C.c_int32),
-                ("reserved_", C.c_int32 * 3)]
+                ("copy_order", C.c_int32), ("reserved_", C.c_int32 * 2)]
 
 
 # (name, argtypes) for every entry point of include/tah.h.

@@ -343,21 +343,22 @@ class Plan:
         each 0 or 1 (fuse always 0), lds 0 to 7 (bit 2, the default:
         restriction streamed along z; bits 0 and 1: restriction and
         prolongation from LDS chunks), copy_per_thread 4, 8 or 16, copy_wgs 0
-        or a multiple of 8."""
+        or a multiple of 8, queue 0 or 1, copy_order 0 (copy tiles by source
+        block) or 1 (by target block)."""
         o = _abi.tah_plan_options()
         check(lib().tah_plan_get_options(self._h, C.byref(o)))
         return dict(fork=o.fork, merge=o.merge, defer=o.defer, copy_per_thread=o.copy_per_thread,
                     fuse=o.fuse, copy_wgs=o.copy_wgs, interp_pairs=o.interp_pairs, lds=o.lds,
-                    queue=o.queue)
+                    queue=o.queue, copy_order=o.copy_order)
 
     def set_options(self, fork=None, merge=None, defer=None, copy_per_thread=None, fuse=None,
-                    copy_wgs=None, interp_pairs=None, lds=None, queue=None):
+                    copy_wgs=None, interp_pairs=None, lds=None, queue=None, copy_order=None):
         cur = self.options
         pick = lambda name, v: cur[name] if v is None else int(v)
         o = _abi.tah_plan_options(pick("fork", fork), pick("merge", merge), pick("defer", defer),
                                   pick("copy_per_thread", copy_per_thread), pick("fuse", fuse),
                                   pick("copy_wgs", copy_wgs), pick("interp_pairs", interp_pairs),
-                                  pick("lds", lds), pick("queue", queue))
+                                  pick("lds", lds), pick("queue", queue), pick("copy_order", copy_order))
         check(lib().tah_plan_set_options(self._h, C.byref(o)))
 
     @property
@@ -1211,15 +1212,17 @@ def fill_ghosts_batched(pairs, *, ctx=None):
 def tune_batched(pairs, cache_path=None, *, ctx=None):
     """tah_tune_batched: time the batched fill of these (fs, schedule) pairs
     for every combination of fork, defer and copy_per_thread (the same on
-    every plan) and keep the fastest; cached by the batch's signature.
-    Returns (dict(fork=, defer=, copy_per_thread=), median ms of the choice,
-    or None when taken from the cache)."""
+    every plan), keep the fastest, then the copies' tile order at it;
+    cached by the batch's signature. Returns (dict(fork=, defer=,
+    copy_per_thread=, copy_order=), median ms of the choice, or None when
+    taken from the cache)."""
     ctx, n, plans, works, facs, keep = _batch_args(pairs, ctx)
     ms = C.c_double()
     path = None if cache_path is None else str(cache_path).encode()
     check(lib().tah_tune_batched(ctx._h, n, plans, works, facs, path, C.byref(ms)))
     o = pairs[0][1].plan(pairs[0][0].layout, ctx).options
-    return (dict(fork=o["fork"], defer=o["defer"], copy_per_thread=o["copy_per_thread"]),
+    return (dict(fork=o["fork"], defer=o["defer"], copy_per_thread=o["copy_per_thread"],
+                 copy_order=o["copy_order"]),
             None if ms.value < 0 else ms.value)
 
 
